@@ -1,0 +1,132 @@
+"""ctypes bindings to libgpucsl_hip.so (the C ABI in include/gpucsl_hip.h).
+
+There is no CPU fallback: if the shared library is missing or no GPU is
+visible, every entry point raises GpucslError.  Buffers are torch tensors on
+the current HIP device; the current torch stream is passed to every call.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+from gpucsl_amd.cli.cli_util import GpucslError
+
+LIB_PATH = Path(__file__).resolve().parent / "libgpucsl_hip.so"
+
+_P = ct.c_void_p
+_I = ct.c_int
+_I64 = ct.c_int64
+_SZ = ct.c_size_t
+_D = ct.c_double
+_F = ct.c_float
+
+# name -> (restype, argtypes); must match include/gpucsl_hip.h
+SIGNATURES = {
+    "gpucsl_last_error": (ct.c_char_p, []),
+    "gpucsl_abi_version": (_I, []),
+    "gpucsl_max_built_level": (_I, []),
+    "gpucsl_device_info": (_I, [_P, _P, _P]),
+    "gpucsl_corr_workspace_bytes": (_SZ, [_I64, _I]),
+    "gpucsl_corr": (_I, [_P, _I, _I64, _I, _P, _P, _P, _SZ, _P]),
+    "gpucsl_corr_c64_to_c32": (_I, [_P, _I, _P, _P]),
+    "gpucsl_gauss_workspace_bytes": (_SZ, [_I, _I]),
+    "gpucsl_gauss_init": (_I, [_I, _I, _P, _P, _P, _P, _P]),
+    "gpucsl_gauss_level0": (_I, [_P, _P, _I, _D, _F, _P, _P, _P, _P]),
+    "gpucsl_gauss_level": (_I, [_I, _P, _P, _I, _I, _D, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gpucsl_gauss_skeleton": (_I, [_P, _P, _I, _I, _P, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gpucsl_pmax_from_zmin": (_I, [_P, _I, _I64, _P, _P]),
+    "gpucsl_compact": (_I, [_P, _I, _P, _P, _SZ, _P]),
+    "gpucsl_skeleton_to_u16": (_I, [_P, _I, _P, _P]),
+    "gpucsl_unrank": (_I, [_I, _I, _P, _I, _P, _P]),
+    "gpucsl_profile_begin": (_I, []),
+    "gpucsl_profile_end": (_I, [ct.c_char_p, _SZ]),
+}
+
+STAT_SLOTS = 8
+STAT_NAMES = ("executed", "total", "rechecked", "flipped", "max_err_ratio", "max_degree", "useful")
+
+_lib = None
+
+
+def load():
+    """Load the library (once).  Raises GpucslError if it is not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise GpucslError(
+                f"{LIB_PATH.name} is not built; run `python -m gpucsl_amd.build` "
+                "(hipcc --offload-arch=gfx950)")
+        lib = ct.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    return load().gpucsl_last_error().decode(errors="replace")
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        raise GpucslError(f"{what}: {last_error()} (status {rc})" if what else last_error())
+
+
+def call(name: str, *args):
+    """Call an int-returning entry point and raise on a non-zero status."""
+    rc = getattr(load(), name)(*args)
+    check(rc, name)
+
+
+def require_gpu():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise GpucslError("no HIP device is visible; gpucsl_amd runs only on a GPU")
+    return torch
+
+
+def stream_handle(stream=None) -> int:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+class KernelTimer:
+    """Context manager: per-kernel launch durations from hipEvents recorded on
+    the launch stream by the library.  ``timings`` maps kernel name to
+    (launches, total_ms, min_ms, max_ms)."""
+
+    def __enter__(self):
+        call("gpucsl_profile_begin")
+        self.timings = {}
+        return self
+
+    def __exit__(self, *exc):
+        buf = ct.create_string_buffer(1 << 16)
+        rc = load().gpucsl_profile_end(buf, len(buf))
+        check(rc, "gpucsl_profile_end")
+        for line in buf.value.decode().splitlines():
+            name, cnt, tot, mn, mx = line.split("\t")
+            self.timings[name] = (int(cnt), float(tot), float(mn), float(mx))
+        return False
+
+
+def decode_stats(host_stats, max_level: int):
+    """uint64 [(max_level+1) * 8] -> list of per-level dicts."""
+    import struct
+
+    out = []
+    for l in range(max_level + 1):
+        row = [int(v) for v in host_stats[l * STAT_SLOTS:(l + 1) * STAT_SLOTS]]
+        d = dict(zip(STAT_NAMES, row))
+        d["max_err_ratio"] = struct.unpack("<f", struct.pack("<I", row[4] & 0xFFFFFFFF))[0]
+        out.append(d)
+    return out

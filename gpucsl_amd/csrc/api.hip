@@ -1,0 +1,316 @@
+// extern "C" entry points of libgpucsl_hip.so (declared in include/gpucsl_hip.h)
+// and the library's error reporting.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpucsl_hip.h"
+#include "common.h"
+
+namespace gpucsl {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+    if (e == hipSuccess) return OK;
+    return set_error(ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ----------------------------------------------------------------------------
+// per-launch event timing
+// ----------------------------------------------------------------------------
+namespace prof {
+struct Rec {
+    const char *name;
+    hipEvent_t a, b;
+};
+static std::mutex g_mu;
+static bool g_on = false;
+static std::vector<Rec> g_recs;
+static std::vector<hipEvent_t> g_pool;
+static thread_local hipEvent_t t_start = nullptr;
+
+static hipEvent_t take() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+bool enabled() { return g_on; }
+
+void record_start(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    t_start = take();
+    hipEventRecord(t_start, s);
+}
+
+void record_stop(const char *name, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    hipEvent_t e = take();
+    hipEventRecord(e, s);
+    g_recs.push_back(Rec{name, t_start, e});
+    t_start = nullptr;
+}
+}  // namespace prof
+
+// defined in corr.hip / gauss.hip
+size_t corr_workspace_bytes(int64_t n, int p);
+int corr_launch(const void *X, int dtype, int64_t n, int p, double *C64, float *C32, void *ws,
+                size_t ws_bytes, hipStream_t s);
+int c64_to_c32_launch(const double *C64, int p, float *C32, hipStream_t s);
+size_t gauss_workspace_bytes(int p);
+int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, hipStream_t s);
+int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, float margin_k,
+                        uint8_t *Gm, float *zmin, unsigned long long *stats, hipStream_t s);
+int gauss_level_launch(int level, const float *C32, const double *C64, int p, int max_level,
+                       double thr, float margin_k, int group_size, uint8_t *Gm, int32_t *sep,
+                       float *zmin, unsigned long long *stats, void *ws, size_t ws_bytes,
+                       hipStream_t s);
+int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats,
+                   hipStream_t s);
+int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s);
+int to_u16_launch(const uint8_t *Gm, int p, uint16_t *out, hipStream_t s);
+
+// test hook: lexicographic unranking
+template <int L>
+__global__ void unrank_kernel(int N, const uint64_t *__restrict__ ranks, int count,
+                              int32_t *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int pos[L];
+    unrank_lex<L>(N, ranks[t], pos);
+#pragma unroll
+    for (int k = 0; k < L; ++k) out[(size_t)t * L + k] = pos[k];
+}
+
+// nbr/deg (library layout) -> reference Gc layout [deg, nbrs..., 0...]
+__global__ void gc_layout_kernel(const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg, int p,
+                                 int32_t *__restrict__ Gc) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= p) return;
+    const int d = deg[r];
+    int32_t v;
+    if (c == 0)
+        v = d;
+    else
+        v = (c - 1 < d) ? nbr[(size_t)r * p + c - 1] : 0;
+    Gc[(size_t)r * p + c] = v;
+}
+
+}  // namespace gpucsl
+
+using namespace gpucsl;
+
+static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+extern "C" {
+
+const char *gpucsl_last_error(void) { return g_err; }
+
+int gpucsl_abi_version(void) { return GPUCSL_ABI_VERSION; }
+
+int gpucsl_max_built_level(void) { return GPUCSL_MAX_LEVEL_TEMPLATE; }
+
+int gpucsl_device_info(int *compute_units, size_t *free_bytes, size_t *total_bytes) {
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (compute_units) {
+        rc = check_hip(hipDeviceGetAttribute(compute_units, hipDeviceAttributeMultiprocessorCount, dev),
+                       "hipDeviceGetAttribute");
+        if (rc) return rc;
+    }
+    size_t fr = 0, to = 0;
+    rc = check_hip(hipMemGetInfo(&fr, &to), "hipMemGetInfo");
+    if (rc) return rc;
+    if (free_bytes) *free_bytes = fr;
+    if (total_bytes) *total_bytes = to;
+    return OK;
+}
+
+size_t gpucsl_corr_workspace_bytes(int64_t n, int p) { return corr_workspace_bytes(n, p); }
+
+int gpucsl_corr(const void *X, int dtype, int64_t n, int p, double *C64, float *C32, void *ws,
+                size_t ws_bytes, void *stream) {
+    return corr_launch(X, dtype, n, p, C64, C32, ws, ws_bytes, S(stream));
+}
+
+int gpucsl_corr_c64_to_c32(const double *C64, int p, float *C32, void *stream) {
+    return c64_to_c32_launch(C64, p, C32, S(stream));
+}
+
+size_t gpucsl_gauss_workspace_bytes(int p, int max_level) {
+    (void)max_level;
+    return gauss_workspace_bytes(p);
+}
+
+int gpucsl_gauss_init(int p, int max_level, uint8_t *G, int32_t *sepsets, float *zmin,
+                      uint64_t *stats, void *stream) {
+    int rc = gauss_init_launch(p, max_level, G, sepsets, zmin, S(stream));
+    if (rc) return rc;
+    if (stats) launch_fill<uint64_t>(stats, 0ull, (size_t)(max_level + 1) * GPUCSL_STAT_SLOTS, S(stream));
+    return check_hip(hipGetLastError(), "gauss_init stats");
+}
+
+int gpucsl_gauss_level0(const float *C32, const double *C64, int p, double thr0, float margin,
+                        uint8_t *G, float *zmin, uint64_t *stats_level0, void *stream) {
+    return gauss_level0_launch(C32, C64, p, thr0, margin, G, zmin,
+                               reinterpret_cast<unsigned long long *>(stats_level0), S(stream));
+}
+
+int gpucsl_gauss_level(int level, const float *C32, const double *C64, int p, int max_level,
+                       double thr, float margin, int group_size, uint8_t *G, int32_t *sepsets,
+                       float *zmin, uint64_t *stats_level, void *ws, size_t ws_bytes, void *stream) {
+    return gauss_level_launch(level, C32, C64, p, max_level, thr, margin, group_size, G, sepsets, zmin,
+                              reinterpret_cast<unsigned long long *>(stats_level), ws, ws_bytes,
+                              S(stream));
+}
+
+int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_level,
+                          const double *thr, float margin, int group_size, uint8_t *G,
+                          int32_t *sepsets, float *zmin, uint64_t *stats, void *ws, size_t ws_bytes,
+                          void *stream) {
+    if (!thr || !stats) return set_error(ERR_ARG, "gauss_skeleton: null thr or stats");
+    int rc = gpucsl_gauss_init(p, max_level, G, sepsets, zmin, stats, stream);
+    if (rc) return rc;
+    rc = gpucsl_gauss_level0(C32, C64, p, thr[0], margin, G, zmin, stats, stream);
+    if (rc) return rc;
+    const int built = max_level < GPUCSL_MAX_LEVEL_TEMPLATE ? max_level : GPUCSL_MAX_LEVEL_TEMPLATE;
+    for (int l = 1; l <= built; ++l) {
+        rc = gpucsl_gauss_level(l, C32, C64, p, max_level, thr[l], margin, group_size, G, sepsets, zmin,
+                                stats + (size_t)l * GPUCSL_STAT_SLOTS, ws, ws_bytes, stream);
+        if (rc) return rc;
+    }
+    if (max_level > GPUCSL_MAX_LEVEL_TEMPLATE) {
+        // a level above the built range only matters if some row still has
+        // enough neighbours for it: compact once more and read the max degree.
+        const int next = GPUCSL_MAX_LEVEL_TEMPLATE + 1;
+        unsigned long long *st = reinterpret_cast<unsigned long long *>(stats) + (size_t)next * GPUCSL_STAT_SLOTS;
+        rc = compact_launch(G, p, next, ws, st, S(stream));
+        if (rc) return rc;
+        unsigned long long maxdeg = 0;
+        rc = check_hip(hipMemcpyAsync(&maxdeg, st + 5, sizeof(maxdeg), hipMemcpyDeviceToHost, S(stream)),
+                       "read max degree");
+        if (rc) return rc;
+        rc = check_hip(hipStreamSynchronize(S(stream)), "sync");
+        if (rc) return rc;
+        if ((long long)maxdeg - 1 >= next)
+            return set_error(ERR_UNSUPPORTED, "level %d has work (max degree %llu) but only levels <= %d are built",
+                             next, maxdeg, GPUCSL_MAX_LEVEL_TEMPLATE);
+    }
+    return OK;
+}
+
+int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void *stream) {
+    return pmax_launch(zmin, p, n, pmax, S(stream));
+}
+
+int gpucsl_compact(const uint8_t *G, int p, int32_t *Gc, void *ws, size_t ws_bytes, void *stream) {
+    if (!G || !Gc || !ws || p < 1) return set_error(ERR_ARG, "compact: bad arguments");
+    if (ws_bytes < gauss_workspace_bytes(p)) return set_error(ERR_WORKSPACE, "compact: workspace too small");
+    int rc = compact_launch(G, p, 1, ws, nullptr, S(stream));
+    if (rc) return rc;
+    const int32_t *nbr = (const int32_t *)ws;
+    const int32_t *deg = (const int32_t *)((char *)ws + align_up(sizeof(int32_t) * (size_t)p * p, 256));
+    hipLaunchKernelGGL(gc_layout_kernel, dim3((p + 255) / 256, p), dim3(256), 0, S(stream), nbr, deg, p, Gc);
+    return check_hip(hipGetLastError(), "gc layout");
+}
+
+int gpucsl_profile_begin(void) {
+    std::lock_guard<std::mutex> lk(prof::g_mu);
+    for (auto &r : prof::g_recs) {
+        prof::g_pool.push_back(r.a);
+        prof::g_pool.push_back(r.b);
+    }
+    prof::g_recs.clear();
+    prof::g_on = true;
+    return OK;
+}
+
+int gpucsl_profile_end(char *out, size_t cap) {
+    std::vector<prof::Rec> recs;
+    {
+        std::lock_guard<std::mutex> lk(prof::g_mu);
+        prof::g_on = false;
+        recs.swap(prof::g_recs);
+    }
+    struct Agg {
+        long long count = 0;
+        double total = 0.0, mn = 1e300, mx = 0.0;
+    };
+    std::map<std::string, Agg> agg;
+    for (auto &r : recs) {
+        int rc = check_hip(hipEventSynchronize(r.b), "profile sync");
+        if (rc) return rc;
+        float ms = 0.0f;
+        rc = check_hip(hipEventElapsedTime(&ms, r.a, r.b), "profile elapsed");
+        if (rc) return rc;
+        Agg &a = agg[r.name];
+        a.count += 1;
+        a.total += ms;
+        a.mn = ms < a.mn ? ms : a.mn;
+        a.mx = ms > a.mx ? ms : a.mx;
+    }
+    {
+        std::lock_guard<std::mutex> lk(prof::g_mu);
+        for (auto &r : recs) {
+            prof::g_pool.push_back(r.a);
+            prof::g_pool.push_back(r.b);
+        }
+    }
+    std::string txt;
+    char line[256];
+    for (auto &kv : agg) {
+        snprintf(line, sizeof(line), "%s\t%lld\t%.6f\t%.6f\t%.6f\n", kv.first.c_str(), kv.second.count,
+                 kv.second.total, kv.second.mn, kv.second.mx);
+        txt += line;
+    }
+    if (out && cap) {
+        size_t n = txt.size() < cap - 1 ? txt.size() : cap - 1;
+        memcpy(out, txt.data(), n);
+        out[n] = 0;
+        if (txt.size() >= cap) return set_error(ERR_ARG, "profile_end: buffer too small (%zu needed)", txt.size() + 1);
+    }
+    return OK;
+}
+
+int gpucsl_skeleton_to_u16(const uint8_t *G, int p, uint16_t *out, void *stream) {
+    if (!G || !out || p < 1) return set_error(ERR_ARG, "skeleton_to_u16: bad arguments");
+    return to_u16_launch(G, p, out, S(stream));
+}
+
+int gpucsl_unrank(int N, int l, const uint64_t *ranks, int count, int32_t *out, void *stream) {
+    if (!ranks || !out || count < 0 || N < 0) return set_error(ERR_ARG, "unrank: bad arguments");
+    if (count == 0) return OK;
+    dim3 grid((count + 255) / 256);
+    switch (l) {
+#define GPUCSL_U(LV)                                                                                 \
+    case LV:                                                                                         \
+        hipLaunchKernelGGL(unrank_kernel<LV>, grid, dim3(256), 0, S(stream), N, ranks, count, out);  \
+        break;
+        GPUCSL_U(1) GPUCSL_U(2) GPUCSL_U(3) GPUCSL_U(4) GPUCSL_U(5) GPUCSL_U(6) GPUCSL_U(7) GPUCSL_U(8)
+#undef GPUCSL_U
+    default: return set_error(ERR_ARG, "unrank: l must be in 1..8");
+    }
+    return check_hip(hipGetLastError(), "unrank launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,624 @@
+// Gaussian PC-stable skeleton on gfx950.
+//
+// Replaces gpucsl/pc/cuda/gaussian_ci.cu (level 0 and level n), the compaction
+// of gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56 and the host level loop of
+// gpucsl/pc/discover_skeleton_gaussian.py:30-113.
+//
+// Layout in HBM (p variables, L = max level):
+//   C32  p*p float   correlation, FP32 (read by the CI tests)
+//   C64  p*p double  correlation, FP64 (read only by rechecks)
+//   G    p*p uint8   adjacency (1 edge, 0 none), kept symmetric
+//   sep  p*p*L int32 separation sets, -1 padded, [(i*p+j)*L + k]
+//   zmin p*p float   min Fisher-z per edge, upper triangle [min*p+max]
+//   workspace: nbr p*p int32 (level-start neighbour lists), deg/cntA/cntB p
+//              int32, rowoffA/rowoffB (p+1) int32, two work counters.
+//
+// A level l >= 1 runs as compaction -> scan -> phase A -> phase B.  Phase A
+// tests every level-start edge (a, b), a < b, with S drawn from adj(a)\{b};
+// phase B tests (b, a) with S from adj(b)\{a} for the edges phase A kept.
+// Lane groups walk a pair's subset ranks in increasing order and stop at the
+// first passing test, so the removing sepset is the lexicographically first
+// one in pcalg order and zmin covers exactly the tests up to it.  No host sync
+// happens between levels: a level whose work list is empty costs a few
+// near-empty launches.
+#include "common.h"
+
+namespace gpucsl {
+
+constexpr float EPS32 = 5.9604645e-8f;  // 2^-24
+constexpr int PHASE_BLOCKS_PER_CU = 8;
+
+// per-level statistics slots (uint64 each)
+enum StatSlot { ST_EXEC = 0, ST_TOTAL = 1, ST_RECHECK = 2, ST_FLIP = 3, ST_MAXRATIO = 4, ST_MAXDEG = 5, ST_USEFUL = 6, ST_SLOTS = 8 };
+
+// ----------------------------------------------------------------------------
+// level 0: gpucsl/pc/cuda/gaussian_ci.cu:18-51
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void gauss_level0_kernel(const float *__restrict__ C32,
+                                                           const double *__restrict__ C64, int p,
+                                                           float thr32, double thr, float margin_k,
+                                                           uint8_t *__restrict__ Gm,
+                                                           float *__restrict__ zmin,
+                                                           unsigned long long *__restrict__ st) {
+    const int a = blockIdx.y;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long n_re = 0, n_flip = 0, n_exec = 0;
+    float mr = 0.0f;
+    if (b < p) {
+        if (b == a) {
+            Gm[(size_t)a * p + a] = 0;
+        } else if (b > a) {
+            const float r = C32[(size_t)a * p + b];
+            const float z = fabsf(atanhf(r));
+            const float err = EPS32 * (fabsf(r) + z + 1e-30f) / fmaxf(1.0f - r * r, 1e-30f);
+            const bool bad = !(z <= 3.0e38f);
+            float zf;
+            bool pass;
+            if (bad || fabsf(z - thr32) <= margin_k * err) {
+                const double z64 = ref64::gauss_z0(C64[(size_t)a * p + b]);
+                pass = z64 < thr;
+                zf = (float)z64;
+                ++n_re;
+                if (!bad) {
+                    n_flip += (pass != (z < thr32));
+                    mr = fmaxf(mr, fabsf(z - zf) / err);
+                }
+            } else {
+                pass = z < thr32;
+                zf = z;
+            }
+            ++n_exec;
+            const uint8_t keep = pass ? 0 : 1;
+            Gm[(size_t)a * p + b] = keep;
+            Gm[(size_t)b * p + a] = keep;
+            if (!(zf != zf)) zmin[(size_t)a * p + b] = zf;
+        }
+    }
+    // wave reduction of the statistics
+    for (int off = 32; off >= 1; off >>= 1) {
+        n_re += __shfl_xor(n_re, off);
+        n_flip += __shfl_xor(n_flip, off);
+        n_exec += __shfl_xor(n_exec, off);
+        mr = fmaxf(mr, __shfl_xor(mr, off));
+    }
+    if (__lane_id() == 0) {
+        if (n_exec) {
+            atomicAdd(&st[ST_EXEC], n_exec);
+            atomicAdd(&st[ST_TOTAL], n_exec);
+            atomicAdd(&st[ST_USEFUL], n_exec);
+        }
+        if (n_re) atomicAdd(&st[ST_RECHECK], n_re);
+        if (n_flip) atomicAdd(&st[ST_FLIP], n_flip);
+        if (mr > 0.0f) atomicMax(&st[ST_MAXRATIO], (unsigned long long)__float_as_uint(mr));
+    }
+}
+
+// ----------------------------------------------------------------------------
+// compaction: one wave per row (gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56)
+// nbr[row] = ascending neighbours (diagonal excluded), deg, and the split into
+// neighbours below (cntB) and above (cntA) the row index.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void compact_kernel(const uint8_t *__restrict__ Gm, int p,
+                                                      int32_t *__restrict__ nbr,
+                                                      int32_t *__restrict__ deg,
+                                                      int32_t *__restrict__ cntA,
+                                                      int32_t *__restrict__ cntB) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= p) return;
+    const int lane = __lane_id();
+    const uint8_t *g = Gm + (size_t)row * p;
+    int32_t *out = nbr + (size_t)row * p;
+    int base = 0, below = 0;
+    for (int c0 = 0; c0 < p; c0 += 64) {
+        const int c = c0 + lane;
+        const bool e = c < p && c != row && g[c] != 0;
+        const uint64_t m = __ballot(e);
+        if (e) out[base + __popcll(m & lanemask_lt())] = c;
+        base += __popcll(m);
+        if (c0 + 64 <= row)
+            below += __popcll(m);
+        else if (c0 < row)
+            below += __popcll(m & ((row - c0) >= 64 ? ~0ull : ((1ull << (row - c0)) - 1)));
+    }
+    if (lane == 0) {
+        deg[row] = base;
+        cntB[row] = below;
+        cntA[row] = base - below;
+    }
+}
+
+// Exclusive scans of cntA/cntB into rowoffA/rowoffB (p+1 entries each), the
+// level's full enumeration size T_l and max degree, and work-counter reset.
+// One block of 1024 threads.
+__global__ __launch_bounds__(1024) void scan_kernel(const int32_t *__restrict__ deg,
+                                                    const int32_t *__restrict__ cntA,
+                                                    const int32_t *__restrict__ cntB, int p,
+                                                    int level, int32_t *__restrict__ rowoffA,
+                                                    int32_t *__restrict__ rowoffB,
+                                                    unsigned int *__restrict__ counters,
+                                                    unsigned long long *__restrict__ st) {
+    __shared__ int sA[1024], sB[1024];
+    __shared__ unsigned long long sT[1024];
+    __shared__ int sM[1024];
+    const int t = threadIdx.x;
+    const int per = (p + 1023) / 1024;
+    const int r0 = t * per;
+    int r1 = r0 + per;
+    if (r1 > p) r1 = p;
+    int la = 0, lb = 0, md = 0;
+    unsigned long long tt = 0;
+    for (int r = r0; r < r1; ++r) {
+        la += cntA[r];
+        lb += cntB[r];
+        const int d = deg[r];
+        md = d > md ? d : md;
+        tt += (unsigned long long)d * binom(d - 1, level);
+    }
+    sA[t] = la;
+    sB[t] = lb;
+    sT[t] = tt;
+    sM[t] = md;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int va = 0, vb = 0;
+        if (t >= off) {
+            va = sA[t - off];
+            vb = sB[t - off];
+        }
+        __syncthreads();
+        sA[t] += va;
+        sB[t] += vb;
+        __syncthreads();
+    }
+    int ea = sA[t] - la, eb = sB[t] - lb;
+    for (int r = r0; r < r1; ++r) {
+        rowoffA[r] = ea;
+        rowoffB[r] = eb;
+        ea += cntA[r];
+        eb += cntB[r];
+    }
+    if (t == 1023) {
+        rowoffA[p] = sA[1023];
+        rowoffB[p] = sB[1023];
+        for (int c = 0; c < 6; ++c) counters[c] = 0;
+    }
+    for (int off = 512; off >= 1; off >>= 1) {
+        __syncthreads();
+        if (t < off) {
+            sT[t] += sT[t + off];
+            sM[t] = sM[t] > sM[t + off] ? sM[t] : sM[t + off];
+        }
+    }
+    if (t == 0 && st) {
+        st[ST_TOTAL] = sT[0];
+        st[ST_MAXDEG] = (unsigned long long)sM[0];
+    }
+}
+
+// ----------------------------------------------------------------------------
+// FP32 partial-correlation test on the (l+2) x (l+2) submatrix ordered
+// (S..., i, j): Cholesky of C_SS, forward solves for C_Si and C_Sj, then the
+// 2x2 Schur complement H = M0 - M1 M2^-1 M1^T of gaussian_helpers.cu:78-89 and
+// the Fisher transform of :220-227.  dmin is the smallest pivot (residual
+// variance), which scales the rounding error of H.
+// ----------------------------------------------------------------------------
+template <int L>
+__device__ __forceinline__ float z32_test(const float *__restrict__ C, int p, int i, int j,
+                                          const int (&S)[L], float &dmin, float &rho) {
+    float c[L][L];
+    float ci[L], cj[L];
+    const float cij = C[(size_t)i * p + j];
+#pragma unroll
+    for (int a = 0; a < L; ++a) {
+        ci[a] = C[(size_t)i * p + S[a]];
+        cj[a] = C[(size_t)j * p + S[a]];
+#pragma unroll
+        for (int b = 0; b <= a; ++b) c[a][b] = C[(size_t)S[a] * p + S[b]];
+    }
+    float rinv[L];
+    dmin = 3.0e38f;
+#pragma unroll
+    for (int a = 0; a < L; ++a) {
+#pragma unroll
+        for (int b = 0; b < a; ++b) {
+            float s = c[a][b];
+#pragma unroll
+            for (int k = 0; k < b; ++k) s -= c[a][k] * c[b][k];
+            c[a][b] = s * rinv[b];
+        }
+        float d = c[a][a];
+#pragma unroll
+        for (int k = 0; k < a; ++k) d -= c[a][k] * c[a][k];
+        dmin = fminf(dmin, d);
+        rinv[a] = rsqrtf(d);
+    }
+    float wi[L], wj[L];
+    float h00 = 1.0f, h11 = 1.0f, h01 = cij;
+#pragma unroll
+    for (int a = 0; a < L; ++a) {
+        float si = ci[a], sj = cj[a];
+#pragma unroll
+        for (int k = 0; k < a; ++k) {
+            si -= c[a][k] * wi[k];
+            sj -= c[a][k] * wj[k];
+        }
+        wi[a] = si * rinv[a];
+        wj[a] = sj * rinv[a];
+        h00 -= wi[a] * wi[a];
+        h11 -= wj[a] * wj[a];
+        h01 -= wi[a] * wj[a];
+    }
+    dmin = fminf(dmin, fminf(h00, h11));
+    rho = h01 * rsqrtf(fabsf(h00) * fabsf(h11));
+    return fabsf(atanhf(rho));
+}
+
+// ----------------------------------------------------------------------------
+// level l >= 1, one phase.  GS lanes form a group that owns one ordered pair
+// (i, j) at a time; groups fetch pairs from a global counter.
+// gpucsl/pc/cuda/gaussian_ci.cu:66-120, set_helpers.cu:25-71,
+// graph_helpers.cu:81-133.
+//
+// FAST (SLOW = false): FP32 tests only.  When a test of the current iteration
+// lies within the error margin of the threshold (or is ill-conditioned), the
+// group stops before deciding anything from that iteration and queues
+// (pair, iteration, zmin so far); the pair is finished by the SLOW variant,
+// which reads the queue and rechecks such tests in FP64 with the reference's
+// operation order.  Keeping the FP64 code out of the FAST kernel keeps its
+// register count low.
+// ----------------------------------------------------------------------------
+struct SlowItem {
+    int w;
+    unsigned int iter;
+    float zl;
+    int pad;
+};
+
+template <int L, int GS, bool SLOW>
+__global__ __launch_bounds__(256) void gauss_phase_kernel(
+    const float *__restrict__ C32, const double *__restrict__ C64, int p,
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+    const int32_t *__restrict__ rowoff, int side, float thr32, double thr, float margin_k,
+    uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, int maxL, float *__restrict__ zmin,
+    unsigned int *__restrict__ counter, SlowItem *__restrict__ queue,
+    unsigned int *__restrict__ qcount, unsigned long long *__restrict__ st) {
+    const int lane = __lane_id();
+    const int gbase = lane & ~(GS - 1);
+    const int gl = lane & (GS - 1);
+    const uint64_t gmask = (GS == 64) ? ~0ull : (((1ull << GS) - 1ull) << gbase);
+    const int total = SLOW ? (int)*qcount : rowoff[p];
+
+    int w = 0, i = 0, j = 0, jpos = 0, Nn = 0;
+    uint64_t count = 0, iter = 0;
+    float zl = __builtin_inff();
+    bool have = false, done = false;
+    unsigned long long n_exec = 0, n_re = 0, n_flip = 0, n_useful = 0;
+    float mr = 0.0f;
+
+    while (true) {
+        if (!have && !done) {
+            int t = 0;
+            if (gl == 0) t = (int)atomicAdd(counter, 1u);
+            t = __shfl(t, gbase);
+            if (t >= total) {
+                done = true;
+            } else {
+                unsigned int it0 = 0;
+                float z0 = __builtin_inff();
+                if (SLOW) {
+                    const SlowItem q = queue[t];
+                    w = q.w;
+                    it0 = q.iter;
+                    z0 = q.zl;
+                } else {
+                    w = t;
+                }
+                int lo = 0, hi = p;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (rowoff[mid] <= w) lo = mid; else hi = mid;
+                }
+                i = lo;
+                const int k = w - rowoff[i];
+                const int d = deg[i];
+                jpos = side == 0 ? d - (rowoff[i + 1] - rowoff[i]) + k : k;
+                j = nbr[(size_t)i * p + jpos];
+                Nn = d - 1;
+                count = binom(Nn, L);
+                iter = it0;
+                zl = gl == 0 ? z0 : __builtin_inff();
+                have = count > 0 && (SLOW || side == 0 || Gm[(size_t)i * p + j] != 0);
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (!have) continue;
+
+        const uint64_t rank = iter * GS + gl;
+        const bool act = rank < count;
+        bool pass = false, near = false;
+        float zf = __builtin_inff();
+        int S[L];
+        if (act) {
+            int pos[L];
+            unrank_lex<L>(Nn, rank, pos);
+#pragma unroll
+            for (int c = 0; c < L; ++c) {
+                int idx = pos[c] - 1;
+                idx += (idx >= jpos);
+                S[c] = nbr[(size_t)i * p + idx];
+            }
+            float dmin, rho;
+            const float z = z32_test<L>(C32, p, i, j, S, dmin, rho);
+            const float err = EPS32 * (float)(L + 2) * (1.0f + 1.0f / fmaxf(dmin, 1e-30f)) /
+                              fmaxf(1.0f - rho * rho, 1e-30f);
+            const bool bad = !(dmin > 1e-5f) || !(z <= 3.0e38f) || !(fabsf(rho) < 1.0f);
+            near = bad || fabsf(z - thr32) <= margin_k * err;
+            if (SLOW && near) {
+                const double z64 = ref64::gauss_z<L>(C64, p, i, j, S);
+                pass = z64 < thr;
+                zf = (float)z64;
+                ++n_re;
+                if (!bad) {
+                    n_flip += (pass != (z < thr32));
+                    mr = fmaxf(mr, fabsf(z - zf) / err);
+                }
+            } else {
+                pass = z < thr32;
+                zf = z;
+            }
+        }
+        if (!SLOW) {
+            if (__ballot(near) & gmask) {
+                // hand the pair to the FP64-capable kernel from this iteration on
+#pragma unroll
+                for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
+                if (gl == 0) {
+                    const unsigned int slot = atomicAdd(qcount, 1u);
+                    queue[slot] = SlowItem{w, (unsigned int)iter, zl, 0};
+                }
+                have = false;
+                continue;
+            }
+        }
+        if (act) ++n_exec;
+        const uint64_t pm = __ballot(pass) & gmask;
+        bool finish = false;
+        if (pm) {
+            const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
+            if (act && gl <= P) {
+                ++n_useful;
+                if (zf < zl) zl = zf;
+            }
+            if (gl == P) {
+                Gm[(size_t)i * p + j] = 0;
+                Gm[(size_t)j * p + i] = 0;
+#pragma unroll
+                for (int c = 0; c < L; ++c) {
+                    sep[((size_t)i * p + j) * maxL + c] = S[c];
+                    sep[((size_t)j * p + i) * maxL + c] = S[c];
+                }
+            }
+            finish = true;
+        } else {
+            if (act) {
+                ++n_useful;
+                if (zf < zl) zl = zf;
+            }
+            ++iter;
+            finish = iter * GS >= count;
+        }
+        if (finish) {
+#pragma unroll
+            for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
+            if (gl == 0) {
+                const int a = i < j ? i : j, b = i < j ? j : i;
+                float *zp = zmin + (size_t)a * p + b;
+                if (zl < *zp) *zp = zl;
+            }
+            have = false;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        n_exec += __shfl_xor(n_exec, off);
+        n_re += __shfl_xor(n_re, off);
+        n_flip += __shfl_xor(n_flip, off);
+        n_useful += __shfl_xor(n_useful, off);
+        mr = fmaxf(mr, __shfl_xor(mr, off));
+    }
+    if (lane == 0) {
+        if (n_useful) atomicAdd(&st[ST_USEFUL], n_useful);
+        if (n_exec) atomicAdd(&st[ST_EXEC], n_exec);
+        if (n_re) atomicAdd(&st[ST_RECHECK], n_re);
+        if (n_flip) atomicAdd(&st[ST_FLIP], n_flip);
+        if (mr > 0.0f) atomicMax(&st[ST_MAXRATIO], (unsigned long long)__float_as_uint(mr));
+    }
+}
+
+// ----------------------------------------------------------------------------
+// pmax, init and output conversion
+// ----------------------------------------------------------------------------
+// pmax = 2 (1 - Phi(sqrt(n-3) |zmin|)) = erfc(sqrt(n-3) |zmin| / sqrt 2); -1 on
+// and below the diagonal.  gpucsl/pc/helpers.py:62-82.
+__global__ void pmax_kernel(const float *__restrict__ zmin, int p, double sqrt_nm3,
+                            float *__restrict__ pmax) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (b >= p) return;
+    const size_t t = (size_t)a * p + b;
+    if (b <= a) {
+        pmax[t] = -1.0f;
+        return;
+    }
+    const double x = fabs(sqrt_nm3 * (double)zmin[t]);
+    pmax[t] = (float)erfc(x * 0.70710678118654752440);
+}
+
+__global__ void to_u16_kernel(const uint8_t *__restrict__ g, size_t count, uint16_t *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) out[i] = g[i];
+}
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+// counters[0..1]: FAST work counters (phase A, B); [2..3]: SLOW work
+// counters; [4..5]: queue lengths.  All reset by scan_kernel.
+struct GaussWs {
+    int32_t *nbr, *deg, *cntA, *cntB, *rowoffA, *rowoffB;
+    unsigned int *counters;
+    SlowItem *queueA, *queueB;
+};
+
+// A phase has at most one queue entry per ordered pair of its side, and a side
+// has at most p(p-1)/2 pairs.
+static size_t queue_capacity(int p) { return (size_t)p * (p - 1) / 2 + 1; }
+
+size_t gauss_workspace_bytes(int p) {
+    size_t off = 0;
+    off = align_up(off + sizeof(int32_t) * (size_t)p * p, 256);
+    off = align_up(off + sizeof(int32_t) * (size_t)p * 3, 256);
+    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 2, 256);
+    off = align_up(off + 256, 256);
+    off = align_up(off + sizeof(SlowItem) * queue_capacity(p) * 2, 256);
+    return off;
+}
+
+static GaussWs carve(void *ws, int p) {
+    GaussWs w;
+    char *b = (char *)ws;
+    size_t off = 0;
+    w.nbr = (int32_t *)(b + off);
+    off = align_up(off + sizeof(int32_t) * (size_t)p * p, 256);
+    w.deg = (int32_t *)(b + off);
+    w.cntA = w.deg + p;
+    w.cntB = w.deg + 2 * p;
+    off = align_up(off + sizeof(int32_t) * (size_t)p * 3, 256);
+    w.rowoffA = (int32_t *)(b + off);
+    w.rowoffB = w.rowoffA + (p + 1);
+    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 2, 256);
+    w.counters = (unsigned int *)(b + off);
+    off = align_up(off + 256, 256);
+    w.queueA = (SlowItem *)(b + off);
+    w.queueB = w.queueA + queue_capacity(p);
+    return w;
+}
+
+static int phase_grid() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus * PHASE_BLOCKS_PER_CU;
+}
+
+int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, hipStream_t s) {
+    if (p < 1 || max_level < 0 || !Gm || !zmin) return set_error(ERR_ARG, "gauss_init: bad arguments");
+    launch_fill<uint8_t>(Gm, (uint8_t)1, (size_t)p * p, s);
+    if (max_level > 0 && sep) launch_fill<int32_t>(sep, -1, (size_t)p * p * max_level, s);
+    launch_fill<float>(zmin, __builtin_inff(), (size_t)p * p, s);
+    return check_hip(hipGetLastError(), "gauss_init launch");
+}
+
+int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, float margin_k,
+                        uint8_t *Gm, float *zmin, unsigned long long *stats, hipStream_t s) {
+    if (!C32 || !C64 || !Gm || !zmin || !stats || p < 1) return set_error(ERR_ARG, "gauss_level0: bad arguments");
+    dim3 grid((p + 255) / 256, p);
+    GPUCSL_LAUNCH("gauss_level0", gauss_level0_kernel, grid, dim3(256), 0, s, C32, C64, p, (float)thr, thr,
+                       margin_k, Gm, zmin, stats);
+    return check_hip(hipGetLastError(), "gauss_level0 launch");
+}
+
+int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats,
+                   hipStream_t s) {
+    GaussWs w = carve(ws, p);
+    GPUCSL_LAUNCH("compact", compact_kernel, dim3((p + 3) / 4), dim3(256), 0, s, Gm, p, w.nbr, w.deg, w.cntA,
+                       w.cntB);
+    GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(1), dim3(1024), 0, s, w.deg, w.cntA, w.cntB, p, level,
+                       w.rowoffA, w.rowoffB, w.counters, stats);
+    return check_hip(hipGetLastError(), "compact launch");
+}
+
+static const char *const kPhaseFastName[] = {
+    "", "gauss_level1_fast", "gauss_level2_fast", "gauss_level3_fast", "gauss_level4_fast",
+    "gauss_level5_fast", "gauss_level6_fast", "gauss_level7_fast", "gauss_level8_fast"};
+static const char *const kPhaseSlowName[] = {
+    "", "gauss_level1_slow", "gauss_level2_slow", "gauss_level3_slow", "gauss_level4_slow",
+    "gauss_level5_slow", "gauss_level6_slow", "gauss_level7_slow", "gauss_level8_slow"};
+
+template <int L, int GS>
+static void launch_phase(const float *C32, const double *C64, int p, const GaussWs &w, int side,
+                         double thr, float margin_k, uint8_t *Gm, int32_t *sep, int maxL,
+                         float *zmin, unsigned long long *st, hipStream_t s) {
+    const int32_t *rowoff = side == 0 ? w.rowoffA : w.rowoffB;
+    SlowItem *queue = side == 0 ? w.queueA : w.queueB;
+    unsigned int *qcount = w.counters + 4 + side;
+    GPUCSL_LAUNCH(kPhaseFastName[L], (gauss_phase_kernel<L, GS, false>), dim3(phase_grid()), dim3(256), 0, s, C32,
+                       C64, p, w.nbr, w.deg, rowoff, side, (float)thr, thr, margin_k, Gm, sep, maxL,
+                       zmin, w.counters + side, queue, qcount, st);
+    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_phase_kernel<L, GS, true>), dim3(phase_grid() / 4), dim3(256), 0, s, C32,
+                       C64, p, w.nbr, w.deg, rowoff, side, (float)thr, thr, margin_k, Gm, sep, maxL,
+                       zmin, w.counters + 2 + side, queue, qcount, st);
+}
+
+template <int L>
+static int phase_dispatch_gs(int gs, const float *C32, const double *C64, int p, const GaussWs &w,
+                             int side, double thr, float margin_k, uint8_t *Gm, int32_t *sep,
+                             int maxL, float *zmin, unsigned long long *st, hipStream_t s) {
+    switch (gs) {
+    case 8: launch_phase<L, 8>(C32, C64, p, w, side, thr, margin_k, Gm, sep, maxL, zmin, st, s); break;
+    case 16: launch_phase<L, 16>(C32, C64, p, w, side, thr, margin_k, Gm, sep, maxL, zmin, st, s); break;
+    case 32: launch_phase<L, 32>(C32, C64, p, w, side, thr, margin_k, Gm, sep, maxL, zmin, st, s); break;
+    case 64: launch_phase<L, 64>(C32, C64, p, w, side, thr, margin_k, Gm, sep, maxL, zmin, st, s); break;
+    default: return set_error(ERR_ARG, "group size must be 8, 16, 32 or 64 (got %d)", gs);
+    }
+    return OK;
+}
+
+int gauss_level_launch(int level, const float *C32, const double *C64, int p, int max_level,
+                       double thr, float margin_k, int group_size, uint8_t *Gm, int32_t *sep,
+                       float *zmin, unsigned long long *stats, void *ws, size_t ws_bytes,
+                       hipStream_t s) {
+    if (level < 1 || level > max_level) return set_error(ERR_ARG, "gauss_level: level %d outside [1, %d]", level, max_level);
+    if (level > GPUCSL_MAX_LEVEL_TEMPLATE)
+        return set_error(ERR_UNSUPPORTED, "gauss_level: levels above %d are not built", GPUCSL_MAX_LEVEL_TEMPLATE);
+    if (!C32 || !C64 || !Gm || !sep || !zmin || !stats || !ws) return set_error(ERR_ARG, "gauss_level: null pointer");
+    if (ws_bytes < gauss_workspace_bytes(p)) return set_error(ERR_WORKSPACE, "gauss_level: workspace too small");
+    int rc = compact_launch(Gm, p, level, ws, stats, s);
+    if (rc) return rc;
+    GaussWs w = carve(ws, p);
+    for (int side = 0; side < 2; ++side) {
+        switch (level) {
+#define GPUCSL_CASE(LV)                                                                              \
+    case LV:                                                                                         \
+        rc = phase_dispatch_gs<LV>(group_size, C32, C64, p, w, side, thr, margin_k, Gm, sep, max_level, \
+                                   zmin, stats, s);                                                  \
+        break;
+            GPUCSL_CASE(1) GPUCSL_CASE(2) GPUCSL_CASE(3) GPUCSL_CASE(4)
+            GPUCSL_CASE(5) GPUCSL_CASE(6) GPUCSL_CASE(7) GPUCSL_CASE(8)
+#undef GPUCSL_CASE
+        }
+        if (rc) return rc;
+    }
+    return check_hip(hipGetLastError(), "gauss_level launch");
+}
+
+int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s) {
+    if (!zmin || !pmax || p < 1 || n < 4) return set_error(ERR_ARG, "pmax: bad arguments");
+    GPUCSL_LAUNCH("pmax", pmax_kernel, dim3((p + 255) / 256, p), dim3(256), 0, s, zmin, p,
+                       sqrt((double)(n - 3)), pmax);
+    return check_hip(hipGetLastError(), "pmax launch");
+}
+
+int to_u16_launch(const uint8_t *Gm, int p, uint16_t *out, hipStream_t s) {
+    const size_t cnt = (size_t)p * p;
+    size_t blocks = (cnt + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    GPUCSL_LAUNCH("to_u16", to_u16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Gm, cnt, out);
+    return check_hip(hipGetLastError(), "to_u16 launch");
+}
+
+}  // namespace gpucsl

@@ -1,0 +1,84 @@
+"""Gaussian skeleton discovery, drop-in for gpucsl/pc/discover_skeleton_gaussian.py.
+
+The level loop runs inside libgpucsl_hip.so on one GPU: correlation on MFMA,
+level 0, then compaction and the level-l CI kernels, with no host
+synchronisation between levels.
+"""
+from __future__ import annotations
+
+from timeit import default_timer as timer
+from typing import List
+
+import numpy as np
+
+from gpucsl_amd.cli.cli_util import GpucslError
+from gpucsl_amd.pc.helpers import SkeletonResult, get_gaussian_thresholds, log, log_time, timed
+from gpucsl_amd.pc.pc_validations import validate_given_devices
+
+
+def _tuning(kernels):
+    """Group size and recheck margin carried by a Kernels object, if any."""
+    from gpucsl_amd.gaussian import DEFAULT_GROUP_SIZE, DEFAULT_MARGIN
+
+    k = kernels[0] if isinstance(kernels, (list, tuple)) and kernels else kernels
+    ci = getattr(k, "ci_test", None)
+    return (getattr(ci, "group_size", DEFAULT_GROUP_SIZE), getattr(ci, "margin", DEFAULT_MARGIN))
+
+
+@timed
+def discover_skeleton_gpu_gaussian(
+    skeleton: np.ndarray,
+    data: np.ndarray,
+    correlation_matrix: np.ndarray,
+    alpha: float,
+    max_level: int,
+    num_variables: int,
+    num_observations: int,
+    kernels=None,
+    is_debug: bool = False,
+    should_log: bool = False,
+    devices: List[int] = [0],
+    sync_device: int = None,
+) -> SkeletonResult:
+    """Signature and result of gpucsl/pc/discover_skeleton_gaussian.py:116-172.
+
+    ``skeleton`` contributes its shape only (level 0 sets every pair, as in
+    gpucsl/pc/cuda/gaussian_ci.cu:39-50).  ``devices`` must name one GPU.
+    """
+    import torch
+
+    from gpucsl_amd.gaussian import GaussianEngine
+
+    validate_given_devices(devices, sync_device)
+    if skeleton is not None and tuple(np.shape(skeleton)) != (num_variables, num_variables):
+        raise GpucslError(f"skeleton must be {num_variables}x{num_variables}")
+    thresholds = get_gaussian_thresholds(data, alpha)
+    group_size, margin = _tuning(kernels)
+    with torch.cuda.device(devices[0]):
+        start_init = timer()
+        engine = GaussianEngine(num_variables, num_observations, max_level, thresholds,
+                                margin=margin, group_size=group_size)
+        if correlation_matrix is None:
+            engine.set_data(data)
+        else:
+            engine.set_correlation(correlation_matrix)
+        torch.cuda.synchronize()
+        log_time("Initialization time", timer() - start_init)
+
+        start = timer()
+        if should_log:
+            def on_level(level):
+                torch.cuda.synchronize()
+                log_time(f"    Level {level} time", timer() - start)
+
+            engine.run_levels(on_level)
+        else:
+            engine.run()
+        res = engine.results()
+        computation_time = timer() - start
+        log_time("Computation time", computation_time)
+        if should_log:
+            for level, st in enumerate(res.stats):
+                log(f"level {level}: {st}")
+
+    return SkeletonResult(res.skeleton, res.separation_sets, res.pmax, computation_time)

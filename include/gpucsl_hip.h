@@ -1,0 +1,140 @@
+/*
+ * gpucsl_hip.h -- C ABI of libgpucsl_hip.so, the gfx950 PC-stable skeleton
+ * library.
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a non-zero status (1 bad argument,
+ *     2 HIP error, 3 unsupported, 4 workspace too small); gpucsl_last_error()
+ *     returns a thread-local message for the last failure.
+ *   - Every device pointer is caller-owned device memory.  The library never
+ *     allocates or frees caller memory; workspaces are sized by the
+ *     *_workspace_bytes functions and passed in.
+ *   - Every call is stream-ordered on the caller's hipStream_t (passed as
+ *     void*, NULL = default stream) and does not synchronise the host, except
+ *     gpucsl_gauss_skeleton / gpucsl_disc_skeleton when max_level exceeds the
+ *     built level range (they then read the per-level max degree once).
+ *   - Matrices are row-major.  G is uint8 p*p (1 edge, 0 none); separation sets
+ *     are int32 p*p*max_level, -1 padded, entry [(i*p + j)*max_level + k]; zmin
+ *     and pmax are float p*p with the upper triangle meaningful.
+ *   - stats is a device array of uint64 [(max_level+1) * 8]; slot s of level l
+ *     is stats[l*8 + s]: 0 executed CI tests, 1 full enumeration size T_l,
+ *     2 FP64 rechecks, 3 rechecks whose FP32 decision differed, 4 max
+ *     |z32 - z64| / error-estimate over rechecks (float bits), 5 max degree,
+ *     6 tests with key <= the removing test's key (the count a sequential
+ *     pcalg-order loop runs; executed >= this because lane groups evaluate a
+ *     whole iteration before they see the first pass).
+ *
+ * Reference interfaces (paths in the gpucsl repository) are cited per call.
+ */
+#ifndef GPUCSL_HIP_H
+#define GPUCSL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPUCSL_ABI_VERSION 1
+#define GPUCSL_STAT_SLOTS 8
+
+/* Thread-local description of the last failed call. */
+const char *gpucsl_last_error(void);
+
+/* ABI version (GPUCSL_ABI_VERSION) and the highest level with a built kernel. */
+int gpucsl_abi_version(void);
+int gpucsl_max_built_level(void);
+
+/* Device properties used for planning: compute units, free and total bytes.
+ * Replaces the CuPy queries of gpucsl/pc/kernel_management.py:394-400 and
+ * gpucsl/pc/pc_validations.py:30-42. */
+int gpucsl_device_info(int *compute_units, size_t *free_bytes, size_t *total_bytes);
+
+/* ---------------------------------------------------------------------------
+ * correlation: replaces correlation_matrix_of / np.corrcoef
+ * (gpucsl/pc/helpers.py:58-59) and the host-to-device copy of C
+ * (gpucsl/pc/gaussian_device_manager.py:114-116).
+ * X: n x p row-major, dtype 0 = float32, 1 = float64.  C64 may be NULL.
+ * ------------------------------------------------------------------------- */
+size_t gpucsl_corr_workspace_bytes(int64_t n, int p);
+int gpucsl_corr(const void *X, int dtype, int64_t n, int p, double *C64, float *C32, void *ws,
+                size_t ws_bytes, void *stream);
+/* C32 = (float)C64 for a caller-supplied correlation matrix
+ * (the correlation_matrix argument of gpucsl/pc/pc.py:88-102). */
+int gpucsl_corr_c64_to_c32(const double *C64, int p, float *C32, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Gaussian skeleton
+ * ------------------------------------------------------------------------- */
+size_t gpucsl_gauss_workspace_bytes(int p, int max_level);
+
+/* G = 1, sepsets = -1, zmin = +inf, stats = 0.  Replaces the buffer set-up of
+ * gpucsl/pc/gaussian_device_manager.py:106-133. */
+int gpucsl_gauss_init(int p, int max_level, uint8_t *G, int32_t *sepsets, float *zmin,
+                      uint64_t *stats, void *stream);
+
+/* Level 0: Fisher-z of every pair, removal on z < thr0, zmin, diagonal
+ * cleared.  Replaces gaussian_ci_level_0 (gpucsl/pc/cuda/gaussian_ci.cu:18-51).
+ * margin scales the FP32 error estimate inside which the FP64 recheck runs. */
+int gpucsl_gauss_level0(const float *C32, const double *C64, int p, double thr0, float margin,
+                        uint8_t *G, float *zmin, uint64_t *stats_level0, void *stream);
+
+/* Level l >= 1: compaction, then every CI test with |S| = l in pcalg order.
+ * Replaces one iteration of gaussian_ci_worker
+ * (gpucsl/pc/discover_skeleton_gaussian.py:64-102): compact
+ * (gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56) and gaussian_ci_level_n
+ * (gpucsl/pc/cuda/gaussian_ci.cu:66-120).  group_size in {8, 16, 32, 64} lanes
+ * per ordered pair.  stats_level points at this level's 8 slots. */
+int gpucsl_gauss_level(int level, const float *C32, const double *C64, int p, int max_level,
+                       double thr, float margin, int group_size, uint8_t *G, int32_t *sepsets,
+                       float *zmin, uint64_t *stats_level, void *ws, size_t ws_bytes, void *stream);
+
+/* Levels 0..max_level back to back with no host synchronisation
+ * (gaussian_ci_worker, gpucsl/pc/discover_skeleton_gaussian.py:30-113).
+ * thr: HOST array of max_level+1 thresholds (gpucsl/pc/helpers.py:42-50). */
+int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_level,
+                          const double *thr, float margin, int group_size, uint8_t *G,
+                          int32_t *sepsets, float *zmin, uint64_t *stats, void *ws, size_t ws_bytes,
+                          void *stream);
+
+/* pmax = 2 (1 - Phi(sqrt(n-3) |zmin|)), -1 on and below the diagonal.
+ * Replaces transform_to_pmax_cupy + postprocess_pmax_cupy
+ * (gpucsl/pc/helpers.py:62-82). */
+int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * graph helpers
+ * ------------------------------------------------------------------------- */
+/* Reference layout of compact (gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56):
+ * row r of Gc = [deg, ascending neighbours..., 0...].  ws as for the Gaussian
+ * skeleton. */
+int gpucsl_compact(const uint8_t *G, int p, int32_t *Gc, void *ws, size_t ws_bytes, void *stream);
+
+/* uint8 adjacency -> the uint16 skeleton the Python API returns
+ * (gpucsl/pc/helpers.py:53-55). */
+int gpucsl_skeleton_to_u16(const uint8_t *G, int p, uint16_t *out, void *stream);
+
+/* Lexicographic unranking used by the CI kernels, for tests: out[c*l + k] is
+ * the k-th 1-based element of combination ranks[c] of l out of N
+ * (gpucsl/pc/cuda/helpers/set_helpers.cu:25-49).  1 <= l <= 8. */
+int gpucsl_unrank(int N, int l, const uint64_t *ranks, int count, int32_t *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * timing (the reference's only instrumentation is wall-clock @timed,
+ * gpucsl/pc/helpers.py:30-39, and per-level log_time,
+ * gpucsl/pc/discover_skeleton_gaussian.py:99-102)
+ * ------------------------------------------------------------------------- */
+/* While enabled, every kernel launch of this library is bracketed by a pair of
+ * hipEvents recorded on the stream it is launched on.  Do not enable while a
+ * stream is being captured into a graph. */
+int gpucsl_profile_begin(void);
+/* Disables timing, waits for the recorded events and writes one line per
+ * kernel name: "name\tlaunches\ttotal_ms\tmin_ms\tmax_ms\n". */
+int gpucsl_profile_end(char *out, size_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUCSL_HIP_H */

@@ -1,0 +1,217 @@
+"""Gaussian skeleton on the GPU against the FP64 oracle (GPU box only)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gpucsl_amd import _native as nat
+from gpucsl_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    nat.load()
+    yield
+    torch.cuda.synchronize()
+
+
+def _engine_run(X, n, alpha, L, C=None, margin=None, group_size=None):
+    from gpucsl_amd.gaussian import DEFAULT_GROUP_SIZE, DEFAULT_MARGIN, GaussianEngine
+    from gpucsl_amd.pc.helpers import get_gaussian_thresholds
+
+    p = X.shape[1] if X is not None else C.shape[0]
+    thr = get_gaussian_thresholds(np.zeros((n, 1)), alpha)
+    eng = GaussianEngine(p, n, L, thr, margin=margin or DEFAULT_MARGIN,
+                         group_size=group_size or DEFAULT_GROUP_SIZE)
+    if C is None:
+        eng.set_data(X)
+    else:
+        eng.set_correlation(C)
+    eng.run()
+    res = eng.results()
+    torch.cuda.synchronize()
+    return eng, res
+
+
+def _compare(res, ref, label, pmax_atol=2e-5):
+    same_skel = (res.skeleton == ref["skeleton"]).all()
+    same_sep = (res.separation_sets == ref["sepsets"]).all()
+    iu = np.triu_indices(res.skeleton.shape[0], 1)
+    dp = float(np.abs(res.pmax[iu].astype(np.float64) - ref["pmax"][iu]).max()) if iu[0].size else 0.0
+    say(f"[{label}] skeleton_equal={same_skel} sepsets_equal={same_sep} max|dpmax|={dp:.3g} "
+        f"edges={int(res.skeleton.sum()) // 2}")
+    for l, st in enumerate(res.stats):
+        say(f"   level {l}: {st} oracle_exec={int(ref['executed'][l])} oracle_total={int(ref['total'][l])}")
+    assert same_skel
+    assert same_sep
+    assert dp <= pmax_atol
+    assert ((res.pmax[np.tril_indices(res.skeleton.shape[0])]) == -1).all()
+
+
+def test_unrank_matches_oracle():
+    for l in range(1, 6):
+        for N in sorted({l, l + 1, 7, 13, 40, 150}):
+            total = oracle.binomial(N, l)
+            if total == 0:
+                continue
+            if total <= 4000:
+                ranks = np.arange(total, dtype=np.uint64)
+            else:
+                rng = np.random.default_rng(N * 10 + l)
+                ranks = np.unique(np.concatenate([rng.integers(0, total, 3000), [0, total - 1]])).astype(np.uint64)
+            d_r = torch.from_numpy(ranks.view(np.int64)).cuda()
+            d_o = torch.empty((ranks.size * l,), dtype=torch.int32, device="cuda")
+            nat.call("gpucsl_unrank", N, l, d_r.data_ptr(), ranks.size, d_o.data_ptr(), nat.stream_handle())
+            got = d_o.cpu().numpy().reshape(-1, l)
+            for k, r in enumerate(ranks):
+                assert got[k].tolist() == oracle.unrank(N, l, int(r)).tolist(), (N, l, int(r))
+
+
+@pytest.mark.parametrize("case", ["compact_random", "compact_full"])
+def test_compact_golden(golden, case):
+    G = torch.tensor(golden[case]["G"], dtype=torch.uint8, device="cuda")
+    p = G.shape[0]
+    ws_bytes = nat.load().gpucsl_gauss_workspace_bytes(p, 1)
+    ws = torch.empty((ws_bytes,), dtype=torch.uint8, device="cuda")
+    Gc = torch.empty((p, p), dtype=torch.int32, device="cuda")
+    nat.call("gpucsl_compact", G.data_ptr(), p, Gc.data_ptr(), ws.data_ptr(), ws_bytes, nat.stream_handle())
+    assert Gc.cpu().numpy().tolist() == golden[case]["Gc"]
+
+
+def test_compact_random_large():
+    rng = np.random.default_rng(0)
+    p = 777
+    G = (rng.random((p, p)) < 0.3).astype(np.uint8)
+    G = np.maximum(G, G.T)
+    d_G = torch.from_numpy(G).cuda()
+    ws_bytes = nat.load().gpucsl_gauss_workspace_bytes(p, 1)
+    ws = torch.empty((ws_bytes,), dtype=torch.uint8, device="cuda")
+    Gc = torch.empty((p, p), dtype=torch.int32, device="cuda")
+    nat.call("gpucsl_compact", d_G.data_ptr(), p, Gc.data_ptr(), ws.data_ptr(), ws_bytes, nat.stream_handle())
+    assert (Gc.cpu().numpy() == oracle.compact(G.astype(np.uint16))).all()
+
+
+@pytest.mark.parametrize("p,n,dtype", [(50, 2000, np.float64), (64, 1000, np.float32),
+                                       (65, 333, np.float32), (130, 2049, np.float64),
+                                       (1000, 10000, np.float32)])
+def test_corr_matches_numpy(p, n, dtype):
+    X = syn.gaussian_sem(p, n, 3.0, seed=p + n).data.astype(dtype)
+    from gpucsl_amd.gaussian import GaussianEngine
+
+    eng = GaussianEngine(p, n, 1, [0.1, 0.1])
+    eng.set_data(X)
+    C64 = eng.C64.cpu().numpy()
+    C32 = eng.C32.cpu().numpy()
+    want = np.corrcoef(X.astype(np.float64), rowvar=False)
+    err = float(np.abs(C64 - want).max())
+    say(f"corr p={p} n={n} {np.dtype(dtype).name}: max|C64 - numpy| = {err:.3g}")
+    assert err < 1e-12
+    assert (C32 == C64.astype(np.float32)).all()
+    assert (C64 == C64.T).all()
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_c1_parity(seed):
+    """Config C1: p=50, n=2000 FP64 samples, degree 3, alpha=0.01, L=3."""
+    d = syn.gaussian_sem(50, 2000, 3.0, seed=seed)
+    _, res = _engine_run(d.data, 2000, 0.01, 3)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), 2000, 0.01, 3)
+    _compare(res, ref, f"C1 seed {seed}")
+    for l in range(4):
+        assert res.stats[l]["useful"] == ref["executed"][l]
+        assert res.stats[l]["total"] == ref["total"][l]
+
+
+@pytest.mark.parametrize("group_size", [8, 32, 64])
+def test_group_sizes_agree(group_size):
+    d = syn.gaussian_sem(80, 3000, 4.0, seed=7)
+    _, res = _engine_run(d.data, 3000, 0.01, 4, group_size=group_size)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), 3000, 0.01, 4)
+    _compare(res, ref, f"group {group_size}")
+
+
+def test_c2_parity():
+    """Config C2: p=1000, n=10000 FP32, degree 4, alpha=0.01, L=4 (seed 100)."""
+    d = syn.gaussian_sem(1000, 10000, 4.0, seed=100)
+    X = d.data.astype(np.float32)
+    _, res = _engine_run(X, 10000, 0.01, 4)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), 10000, 0.01, 4)
+    _compare(res, ref, "C2", pmax_atol=5e-5)
+    for l in range(5):
+        assert res.stats[l]["useful"] == ref["executed"][l]
+
+
+def test_user_correlation_matrix_path():
+    d = syn.gaussian_sem(120, 1500, 3.0, seed=21)
+    C = oracle.corrcoef(d.data)
+    _, res = _engine_run(None, 1500, 0.05, 3, C=C)
+    ref = oracle.gauss_skeleton(C, 1500, 0.05, 3)
+    _compare(res, ref, "user C")
+
+
+def test_tight_margin_forces_no_recheck_paths_and_huge_margin_rechecks_all():
+    d = syn.gaussian_sem(60, 800, 3.0, seed=3)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), 800, 0.05, 3)
+    _, res_all = _engine_run(d.data, 800, 0.05, 3, margin=1e30)
+    _compare(res_all, ref, "recheck everything", pmax_atol=1e-6)
+    for l in range(1, 4):
+        assert res_all.stats[l]["rechecked"] == res_all.stats[l]["executed"]
+
+
+def test_deterministic_repeat():
+    d = syn.gaussian_sem(200, 3000, 4.0, seed=5)
+    eng, r1 = _engine_run(d.data, 3000, 0.01, 4)
+    eng.run()
+    r2 = eng.results()
+    assert (r1.skeleton == r2.skeleton).all()
+    assert (r1.separation_sets == r2.separation_sets).all()
+    assert (r1.pmax.view(np.uint32) == r2.pmax.view(np.uint32)).all()
+
+
+def test_edge_cases_singular_and_constant():
+    rng = np.random.default_rng(1)
+    n = 1500
+    X = syn.gaussian_sem(12, n, 3.0, seed=9).data
+    X = np.concatenate([X, X[:, :2], np.ones((n, 1))], axis=1)   # duplicates + constant column
+    _, res = _engine_run(X, n, 0.01, 3)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X), n, 0.01, 3)
+    _compare(res, ref, "singular/constant")
+    del rng
+
+
+def test_max_level_zero_and_tiny():
+    d = syn.gaussian_sem(30, 500, 3.0, seed=2)
+    _, res = _engine_run(d.data, 500, 0.01, 0)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), 500, 0.01, 0)
+    assert res.separation_sets.shape == (30, 30, 0)
+    assert (res.skeleton == ref["skeleton"]).all()
+    X2 = d.data[:, :2]
+    _, r2 = _engine_run(X2, 500, 0.01, 0)
+    assert r2.skeleton.shape == (2, 2)
+
+
+def test_dropin_api_shapes():
+    from gpucsl_amd.pc.discover_skeleton_gaussian import discover_skeleton_gpu_gaussian
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    d = syn.gaussian_sem(40, 1000, 3.0, seed=4)
+    out = discover_skeleton_gpu_gaussian(init_pc_graph(d.data), d.data, None, 0.05, 3, 40, 1000)
+    res, runtime = out
+    assert runtime > 0 and res.discover_skeleton_kernel_time > 0
+    assert res.skeleton.dtype == np.uint16 and res.skeleton.shape == (40, 40)
+    assert res.seperation_sets.shape == (40, 40, 3) and res.seperation_sets.dtype == np.int32
+    assert res.pmax.dtype == np.float32
+    ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), 1000, 0.05, 3)
+    assert (res.skeleton == ref["skeleton"]).all()
+    assert (res.seperation_sets == ref["sepsets"]).all()
