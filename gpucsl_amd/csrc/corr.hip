@@ -116,8 +116,8 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const T *__restrict__ X,
         __syncthreads();
     }
 
-    // D layout of v_mfma_f64_16x16x4_f64: lane holds column (lane & 15) and rows
-    // 4 * (lane >> 4) + v, v = 0..3.
+    // D layout of v_mfma_f64_16x16x4_f64 (checked on gfx950): accumulator item v
+    // of a lane holds column (lane & 15) and row (lane >> 4) + 4 * v.
     double *out = Gp + (size_t)split * p * p;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const T *__restrict__ X,
             const int col = b0 + wc + j * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = a0 + wr + i * 16 + 4 * (lane >> 4) + v;
+                const int row = a0 + wr + i * 16 + (lane >> 4) + 4 * v;
                 if (row < p && col < p) out[(size_t)row * p + col] = acc[i][j][v];
             }
         }
@@ -142,8 +142,9 @@ __global__ void sd_kernel(const double *__restrict__ Gp, int splits, int64_t n, 
     sd[a] = sqrt(g / (double)(n - 1));
 }
 
-// C64[a][b] = clip((G_ab / (n-1)) / sd_a / sd_b, -1, 1) in numpy's corrcoef
-// order; C32 = (float)C64.  Reads the upper triangle (a <= b), writes both.
+// C64[a][b] = clip((G_ab / (n-1)) / sd_lo / sd_hi, -1, 1) with lo = min(a, b):
+// numpy corrcoef's operation order for its upper triangle, mirrored so that C is
+// exactly symmetric; C32 = (float)C64.
 __global__ __launch_bounds__(256) void normalize_kernel(const double *__restrict__ Gp, int splits,
                                                         int64_t n, int p,
                                                         const double *__restrict__ sd,
@@ -156,8 +157,8 @@ __global__ __launch_bounds__(256) void normalize_kernel(const double *__restrict
     double g = 0.0;
     for (int s = 0; s < splits; ++s) g += Gp[(size_t)s * p * p + (size_t)lo * p + hi];
     double c = g / (double)(n - 1);
-    c = c / sd[a];
-    c = c / sd[b];
+    c = c / sd[lo];
+    c = c / sd[hi];
     if (c > 1.0) c = 1.0;
     if (c < -1.0) c = -1.0;
     if (C64) C64[(size_t)a * p + b] = c;
@@ -182,7 +183,7 @@ struct CorrPlan {
 
 static CorrPlan corr_plan(int64_t n, int p) {
     CorrPlan c{};
-    c.row_splits = (int)((n + 1023) / 1024);
+    c.row_splits = (int)((n + 127) / 128);
     if (c.row_splits > 64) c.row_splits = 64;
     if (c.row_splits < 1) c.row_splits = 1;
     c.rows_per_split = (n + c.row_splits - 1) / c.row_splits;

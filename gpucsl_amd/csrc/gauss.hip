@@ -27,6 +27,15 @@ namespace gpucsl {
 
 constexpr float EPS32 = 5.9604645e-8f;  // 2^-24
 constexpr int PHASE_BLOCKS_PER_CU = 8;
+constexpr int MAX_PHASE_BLOCKS = 4096;
+constexpr int STRIPES = 8;
+// counters: 8 stripes for each of FAST A, SLOW A, FAST B, SLOW B, then the two
+// queue lengths.
+constexpr int NUM_COUNTERS = 64;
+constexpr int CNT_QUEUE = 32;
+// per-block statistics partials: 4 kernel slots x MAX_PHASE_BLOCKS x 8 words
+constexpr int BSTAT_WORDS = 8;
+enum BStat { BS_EXEC = 0, BS_USEFUL = 1, BS_RECHECK = 2, BS_FLIP = 3, BS_MAXRATIO = 4 };
 
 // per-level statistics slots (uint64 each)
 enum StatSlot { ST_EXEC = 0, ST_TOTAL = 1, ST_RECHECK = 2, ST_FLIP = 3, ST_MAXRATIO = 4, ST_MAXDEG = 5, ST_USEFUL = 6, ST_SLOTS = 8 };
@@ -42,7 +51,7 @@ __global__ __launch_bounds__(256) void gauss_level0_kernel(const float *__restri
                                                            unsigned long long *__restrict__ st) {
     const int a = blockIdx.y;
     const int b = blockIdx.x * 256 + threadIdx.x;
-    unsigned long long n_re = 0, n_flip = 0, n_exec = 0;
+    unsigned long long n_re = 0, n_flip = 0;
     float mr = 0.0f;
     if (b < p) {
         if (b == a) {
@@ -67,29 +76,32 @@ __global__ __launch_bounds__(256) void gauss_level0_kernel(const float *__restri
                 pass = z < thr32;
                 zf = z;
             }
-            ++n_exec;
             const uint8_t keep = pass ? 0 : 1;
             Gm[(size_t)a * p + b] = keep;
             Gm[(size_t)b * p + a] = keep;
             if (!(zf != zf)) zmin[(size_t)a * p + b] = zf;
         }
     }
-    // wave reduction of the statistics
-    for (int off = 32; off >= 1; off >>= 1) {
-        n_re += __shfl_xor(n_re, off);
-        n_flip += __shfl_xor(n_flip, off);
-        n_exec += __shfl_xor(n_exec, off);
-        mr = fmaxf(mr, __shfl_xor(mr, off));
+    // executed = total = useful = p(p-1)/2 is known; only the rare recheck
+    // counters need atomics.
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        const unsigned long long pairs = (unsigned long long)p * (p - 1) / 2;
+        st[ST_EXEC] = pairs;
+        st[ST_TOTAL] = pairs;
+        st[ST_USEFUL] = pairs;
     }
-    if (__lane_id() == 0) {
-        if (n_exec) {
-            atomicAdd(&st[ST_EXEC], n_exec);
-            atomicAdd(&st[ST_TOTAL], n_exec);
-            atomicAdd(&st[ST_USEFUL], n_exec);
+    const unsigned long long any = __ballot(n_re != 0);
+    if (any) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            n_re += __shfl_xor(n_re, off);
+            n_flip += __shfl_xor(n_flip, off);
+            mr = fmaxf(mr, __shfl_xor(mr, off));
         }
-        if (n_re) atomicAdd(&st[ST_RECHECK], n_re);
-        if (n_flip) atomicAdd(&st[ST_FLIP], n_flip);
-        if (mr > 0.0f) atomicMax(&st[ST_MAXRATIO], (unsigned long long)__float_as_uint(mr));
+        if (__lane_id() == 0) {
+            atomicAdd(&st[ST_RECHECK], n_re);
+            if (n_flip) atomicAdd(&st[ST_FLIP], n_flip);
+            if (mr > 0.0f) atomicMax(&st[ST_MAXRATIO], (unsigned long long)__float_as_uint(mr));
+        }
     }
 }
 
@@ -180,7 +192,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int32_t *__restrict__ 
     if (t == 1023) {
         rowoffA[p] = sA[1023];
         rowoffB[p] = sB[1023];
-        for (int c = 0; c < 6; ++c) counters[c] = 0;
+        for (int c = 0; c < NUM_COUNTERS; ++c) counters[c] = 0;
     }
     for (int off = 512; off >= 1; off >>= 1) {
         __syncthreads();
@@ -280,13 +292,31 @@ __global__ __launch_bounds__(256) void gauss_phase_kernel(
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
     const int32_t *__restrict__ rowoff, int side, float thr32, double thr, float margin_k,
     uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, int maxL, float *__restrict__ zmin,
-    unsigned int *__restrict__ counter, SlowItem *__restrict__ queue,
-    unsigned int *__restrict__ qcount, unsigned long long *__restrict__ st) {
+    unsigned int *__restrict__ stripe_counters, SlowItem *__restrict__ queue,
+    unsigned int *__restrict__ qcount, unsigned long long *__restrict__ bstats) {
     const int lane = __lane_id();
     const int gbase = lane & ~(GS - 1);
     const int gl = lane & (GS - 1);
     const uint64_t gmask = (GS == 64) ? ~0ull : (((1ull << GS) - 1ull) << gbase);
     const int total = SLOW ? (int)*qcount : rowoff[p];
+
+    // Work distribution without hot atomics: each group first takes
+    // S = total / groups pairs statically (interleaved, so neighbouring groups
+    // share row i), then the remainder through one of 8 striped counters
+    // chosen by block index.
+    constexpr int GPB = 256 / GS;
+    const int groups = gridDim.x * GPB;
+    const int gid = blockIdx.x * GPB + (threadIdx.x / GS);
+    // at least one static pair per group, so a level with fewer pairs than
+    // groups runs without any atomics
+    const int S_static = total <= groups ? 1 : total / groups;
+    const int dyn_base = total <= groups ? total : S_static * groups;
+    const int nstripes = gridDim.x < STRIPES ? gridDim.x : STRIPES;
+    const int stripe = blockIdx.x % nstripes;
+    const int rem = total - dyn_base;
+    const int stripe_len = rem > stripe ? (rem - stripe + nstripes - 1) / nstripes : 0;
+    unsigned int *counter = stripe_counters + stripe;
+    int k_static = 0;
 
     int w = 0, i = 0, j = 0, jpos = 0, Nn = 0;
     uint64_t count = 0, iter = 0;
@@ -297,9 +327,22 @@ __global__ __launch_bounds__(256) void gauss_phase_kernel(
 
     while (true) {
         if (!have && !done) {
-            int t = 0;
-            if (gl == 0) t = (int)atomicAdd(counter, 1u);
-            t = __shfl(t, gbase);
+            int t = total;
+            if (k_static < S_static) {
+                t = gid + k_static * groups;
+                ++k_static;
+                if (t >= dyn_base) t = total;
+            } else if (stripe_len > 0) {
+                if (gl == 0) {
+                    const unsigned int cur =
+                        __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((int)cur < stripe_len) {
+                        const unsigned int k = atomicAdd(counter, 1u);
+                        if ((int)k < stripe_len) t = dyn_base + stripe + nstripes * (int)k;
+                    }
+                }
+                t = __shfl(t, gbase);
+            }
             if (t >= total) {
                 done = true;
             } else {
@@ -425,12 +468,59 @@ __global__ __launch_bounds__(256) void gauss_phase_kernel(
         n_useful += __shfl_xor(n_useful, off);
         mr = fmaxf(mr, __shfl_xor(mr, off));
     }
+    __shared__ unsigned long long wstat[4][5];
+    const int wave = threadIdx.x >> 6;
     if (lane == 0) {
-        if (n_useful) atomicAdd(&st[ST_USEFUL], n_useful);
-        if (n_exec) atomicAdd(&st[ST_EXEC], n_exec);
-        if (n_re) atomicAdd(&st[ST_RECHECK], n_re);
-        if (n_flip) atomicAdd(&st[ST_FLIP], n_flip);
-        if (mr > 0.0f) atomicMax(&st[ST_MAXRATIO], (unsigned long long)__float_as_uint(mr));
+        wstat[wave][BS_EXEC] = n_exec;
+        wstat[wave][BS_USEFUL] = n_useful;
+        wstat[wave][BS_RECHECK] = n_re;
+        wstat[wave][BS_FLIP] = n_flip;
+        wstat[wave][BS_MAXRATIO] = __float_as_uint(mr);
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long v = 0;
+        for (int w2 = 0; w2 < 4; ++w2) {
+            const unsigned long long x = wstat[w2][threadIdx.x];
+            v = threadIdx.x == BS_MAXRATIO ? (x > v ? x : v) : v + x;
+        }
+        bstats[(size_t)blockIdx.x * BSTAT_WORDS + threadIdx.x] = v;
+    }
+}
+
+// Sums the per-block partials of the four phase kernels of one level into its
+// statistics slots.  One block.
+__global__ __launch_bounds__(256) void level_stats_kernel(const unsigned long long *__restrict__ bstats,
+                                                          int fast_blocks, int slow_blocks,
+                                                          unsigned long long *__restrict__ st) {
+    __shared__ unsigned long long red[256][5];
+    unsigned long long v[5] = {0, 0, 0, 0, 0};
+    for (int slot = 0; slot < 4; ++slot) {
+        const int nb = (slot & 1) ? slow_blocks : fast_blocks;
+        const unsigned long long *b = bstats + (size_t)slot * MAX_PHASE_BLOCKS * BSTAT_WORDS;
+        for (int i = threadIdx.x; i < nb; i += 256) {
+            for (int k = 0; k < 4; ++k) v[k] += b[(size_t)i * BSTAT_WORDS + k];
+            const unsigned long long m = b[(size_t)i * BSTAT_WORDS + BS_MAXRATIO];
+            v[4] = m > v[4] ? m : v[4];
+        }
+    }
+    for (int k = 0; k < 5; ++k) red[threadIdx.x][k] = v[k];
+    __syncthreads();
+    for (int off = 128; off >= 1; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int k = 0; k < 5; ++k) {
+                const unsigned long long o = red[threadIdx.x + off][k];
+                red[threadIdx.x][k] = k == 4 ? (o > red[threadIdx.x][k] ? o : red[threadIdx.x][k])
+                                             : red[threadIdx.x][k] + o;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        st[ST_EXEC] = red[0][BS_EXEC];
+        st[ST_USEFUL] = red[0][BS_USEFUL];
+        st[ST_RECHECK] = red[0][BS_RECHECK];
+        st[ST_FLIP] = red[0][BS_FLIP];
+        st[ST_MAXRATIO] = red[0][BS_MAXRATIO];
     }
 }
 
@@ -462,12 +552,13 @@ __global__ void to_u16_kernel(const uint8_t *__restrict__ g, size_t count, uint1
 // ----------------------------------------------------------------------------
 // host side
 // ----------------------------------------------------------------------------
-// counters[0..1]: FAST work counters (phase A, B); [2..3]: SLOW work
-// counters; [4..5]: queue lengths.  All reset by scan_kernel.
+// counters: see NUM_COUNTERS; all reset by scan_kernel.  bstats: per-block
+// statistics partials of the four phase kernels of a level.
 struct GaussWs {
     int32_t *nbr, *deg, *cntA, *cntB, *rowoffA, *rowoffB;
     unsigned int *counters;
     SlowItem *queueA, *queueB;
+    unsigned long long *bstats;
 };
 
 // A phase has at most one queue entry per ordered pair of its side, and a side
@@ -481,6 +572,7 @@ size_t gauss_workspace_bytes(int p) {
     off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 2, 256);
     off = align_up(off + 256, 256);
     off = align_up(off + sizeof(SlowItem) * queue_capacity(p) * 2, 256);
+    off = align_up(off + sizeof(unsigned long long) * 4 * MAX_PHASE_BLOCKS * BSTAT_WORDS, 256);
     return off;
 }
 
@@ -501,6 +593,8 @@ static GaussWs carve(void *ws, int p) {
     off = align_up(off + 256, 256);
     w.queueA = (SlowItem *)(b + off);
     w.queueB = w.queueA + queue_capacity(p);
+    off = align_up(off + sizeof(SlowItem) * queue_capacity(p) * 2, 256);
+    w.bstats = (unsigned long long *)(b + off);
     return w;
 }
 
@@ -508,12 +602,16 @@ static int phase_grid() {
     static int cus = 0;
     if (cus == 0) {
         int dev = 0;
-        hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
+        int n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cus = n;
     }
-    return cus * PHASE_BLOCKS_PER_CU;
+    int g = cus * PHASE_BLOCKS_PER_CU;
+    return g > MAX_PHASE_BLOCKS ? MAX_PHASE_BLOCKS : (g < 4 * STRIPES ? 4 * STRIPES : g);
 }
+static int slow_grid() { return phase_grid() / 4; }
 
 int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, hipStream_t s) {
     if (p < 1 || max_level < 0 || !Gm || !zmin) return set_error(ERR_ARG, "gauss_init: bad arguments");
@@ -553,15 +651,20 @@ template <int L, int GS>
 static void launch_phase(const float *C32, const double *C64, int p, const GaussWs &w, int side,
                          double thr, float margin_k, uint8_t *Gm, int32_t *sep, int maxL,
                          float *zmin, unsigned long long *st, hipStream_t s) {
+    (void)st;
     const int32_t *rowoff = side == 0 ? w.rowoffA : w.rowoffB;
     SlowItem *queue = side == 0 ? w.queueA : w.queueB;
-    unsigned int *qcount = w.counters + 4 + side;
+    unsigned int *qcount = w.counters + CNT_QUEUE + side;
+    unsigned int *fast_ctr = w.counters + (2 * side) * STRIPES;
+    unsigned int *slow_ctr = w.counters + (2 * side + 1) * STRIPES;
+    unsigned long long *fast_bs = w.bstats + (size_t)(2 * side) * MAX_PHASE_BLOCKS * BSTAT_WORDS;
+    unsigned long long *slow_bs = w.bstats + (size_t)(2 * side + 1) * MAX_PHASE_BLOCKS * BSTAT_WORDS;
     GPUCSL_LAUNCH(kPhaseFastName[L], (gauss_phase_kernel<L, GS, false>), dim3(phase_grid()), dim3(256), 0, s, C32,
                        C64, p, w.nbr, w.deg, rowoff, side, (float)thr, thr, margin_k, Gm, sep, maxL,
-                       zmin, w.counters + side, queue, qcount, st);
-    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_phase_kernel<L, GS, true>), dim3(phase_grid() / 4), dim3(256), 0, s, C32,
+                       zmin, fast_ctr, queue, qcount, fast_bs);
+    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_phase_kernel<L, GS, true>), dim3(slow_grid()), dim3(256), 0, s, C32,
                        C64, p, w.nbr, w.deg, rowoff, side, (float)thr, thr, margin_k, Gm, sep, maxL,
-                       zmin, w.counters + 2 + side, queue, qcount, st);
+                       zmin, slow_ctr, queue, qcount, slow_bs);
 }
 
 template <int L>
@@ -603,6 +706,8 @@ int gauss_level_launch(int level, const float *C32, const double *C64, int p, in
         }
         if (rc) return rc;
     }
+    GPUCSL_LAUNCH("level_stats", level_stats_kernel, dim3(1), dim3(256), 0, s, w.bstats, phase_grid(),
+                  slow_grid(), stats);
     return check_hip(hipGetLastError(), "gauss_level launch");
 }
 
