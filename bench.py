@@ -128,7 +128,8 @@ def main():
     cnt, tot, _, _ = timings[dominant]
     avg_s = tot / cnt / 1e3
     if dominant.startswith("corr_gram"):
-        achieved = 2.0 * n * p * p / avg_s / 1e12
+        # symmetric rank-n update: n * p * (p + 1) flops for the p(p+1)/2 distinct entries
+        achieved = 1.0 * n * p * (p + 1) / avg_s / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS)
     elif dominant.startswith("gauss_level") and dominant.endswith("_fast"):

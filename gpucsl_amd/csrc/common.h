@@ -46,59 +46,76 @@ __host__ __device__ __forceinline__ uint64_t binom(int64_t m, int k) {
     return r;
 }
 
-// Largest m in [j-1, hi] with C(m, j) < R, for R >= 1 and j >= 1.  A real-valued
-// inverse gives the start; exact integer binomials correct it.
-__device__ __forceinline__ int largest_binom_below(uint64_t R, int j, int hi) {
-    if (j == 1) {
-        uint64_t m = R - 1;
-        return m > (uint64_t)hi ? hi : (int)m;
-    }
-    double Rd = (double)R, est;
-    if (j == 2) {
-        est = 0.5 * (1.0 + sqrt(1.0 + 8.0 * Rd));
-    } else if (j == 3) {
-        est = cbrt(6.0 * Rd) + 1.0;
-    } else if (j == 4) {
-        est = sqrt(sqrt(24.0 * Rd)) + 1.5;
-    } else {
-        double lf = 0.0;
-        for (int t = 2; t <= j; ++t) lf += log((double)t);
-        est = exp((log(Rd) + lf) / j) + 0.5 * (j - 1);
-    }
-    int m = (int)est;
-    if (m > hi) m = hi;
-    if (m < j - 1) m = j - 1;
-    while (m < hi && binom(m + 1, j) < R) ++m;
-    while (m > j - 1 && binom(m, j) >= R) --m;
-    return m;
+// C(m, K) for a compile-time K: K multiply/divide steps by constants (the
+// divisions compile to multiply-high sequences).
+template <int K>
+__device__ __forceinline__ uint64_t binom_k(int m) {
+    if (m < K) return 0;
+    uint64_t r = 1;
+#pragma unroll
+    for (int t = 0; t < K; ++t) r = r * (uint64_t)(m - t) / (uint64_t)(t + 1);
+    return r;
 }
 
-// Rank -> 1-based lexicographic l-combination of {1..N} (the ordering of the
-// combinatorial unranking in gpucsl/pc/cuda/helpers/set_helpers.cu:25-49).
-// For position c with k = l-c-1 positions after it, the number of
-// combinations that place a value below v there is
-// C(N-prev, k+1) - C(N-v+1, k+1) (hockey-stick identity), so the value is found
-// from the inverse binomial instead of a linear walk.
-template <int L>
-__device__ __forceinline__ void unrank_lex(int N, uint64_t r, int (&pos)[L]) {
-    int prev = 0;
+// Largest m in [J-1, hi] with C(m, J) < R, for R >= 1.  A real-valued inverse
+// gives the start; exact integer binomials correct it by a step or two.
+template <int J>
+__device__ __forceinline__ int largest_binom_below(uint64_t R, int hi) {
+    if constexpr (J == 1) {
+        const uint64_t m = R - 1;
+        return m > (uint64_t)hi ? hi : (int)m;
+    } else {
+        const float Rf = (float)R;
+        float est;
+        if constexpr (J == 2) {
+            est = 0.5f * (1.0f + sqrtf(1.0f + 8.0f * Rf));
+        } else if constexpr (J == 3) {
+            est = cbrtf(6.0f * Rf) + 1.0f;
+        } else if constexpr (J == 4) {
+            est = sqrtf(sqrtf(24.0f * Rf)) + 1.5f;
+        } else {
+            float lf = 0.0f;
 #pragma unroll
-    for (int c = 0; c < L; ++c) {
-        const int k = L - c - 1;
-        if (k == 0) {
+            for (int t = 2; t <= J; ++t) lf += __logf((float)t);
+            est = __expf((__logf(Rf) + lf) / (float)J) + 0.5f * (float)(J - 1);
+        }
+        int m = (int)est;
+        if (m > hi) m = hi;
+        if (m < J - 1) m = J - 1;
+        while (m < hi && binom_k<J>(m + 1) < R) ++m;
+        while (m > J - 1 && binom_k<J>(m) >= R) --m;
+        return m;
+    }
+}
+
+// Rank -> 1-based lexicographic L-combination of {1..N} (the ordering of the
+// combinatorial unranking in gpucsl/pc/cuda/helpers/set_helpers.cu:25-49).
+// For position C with K = L-C-1 positions after it, the number of
+// combinations that place a value below v there is
+// C(N-prev, K+1) - C(N-v+1, K+1) (hockey-stick identity), so the value is found
+// from the inverse binomial instead of a linear walk.
+template <int L, int C>
+__device__ __forceinline__ void unrank_step(int N, uint64_t r, int prev, int (&pos)[L]) {
+    if constexpr (C < L) {
+        constexpr int K = L - C - 1;
+        if constexpr (K == 0) {
             // r < N - prev whenever C(N, L) fits in 64 bits; the clamp keeps the
             // position inside the list even if a caller exceeds that.
             const uint64_t room = (uint64_t)(N - prev);
-            pos[c] = prev + 1 + (int)(r < room ? r : room - 1);
+            pos[C] = prev + 1 + (int)(r < room ? r : room - 1);
         } else {
-            const uint64_t block = binom(N - prev, k + 1);
+            const uint64_t block = binom_k<K + 1>(N - prev);
             const uint64_t R = block - r;
-            const int m = largest_binom_below(R, k + 1, N - prev - 1);
-            pos[c] = N - m;
-            r = binom(m + 1, k + 1) - R;
-            prev = N - m;
+            const int m = largest_binom_below<K + 1>(R, N - prev - 1);
+            pos[C] = N - m;
+            unrank_step<L, C + 1>(N, binom_k<K + 1>(m + 1) - R, N - m, pos);
         }
     }
+}
+
+template <int L>
+__device__ __forceinline__ void unrank_lex(int N, uint64_t r, int (&pos)[L]) {
+    unrank_step<L, 0>(N, r, 0, pos);
 }
 
 // ----------------------------------------------------------------------------

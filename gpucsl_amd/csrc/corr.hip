@@ -42,14 +42,17 @@ __global__ void mean_kernel(const double *__restrict__ partial, int splits, int6
 
 // Gp[s] (p x p, only tiles with ta <= tb written) = Zs^T Zs over k-range s,
 // with Z = X - mean.  Block: 256 threads = 4 waves in a 2 x 2 arrangement,
-// each wave owns a 32 x 32 sub-tile (2 x 2 MFMA 16x16 blocks).
+// each wave owns a 32 x 32 sub-tile (2 x 2 MFMA 16x16 blocks).  Raw X slabs of
+// KT rows are double-buffered in LDS: the global loads of slab s+1 are in
+// flight while the MFMAs of slab s run, and centring (x - mean, in FP64)
+// happens when a fragment is read.
 template <typename T>
 __global__ __launch_bounds__(256) void gram_mfma_kernel(const T *__restrict__ X, int64_t n, int p,
                                                         const double *__restrict__ mean,
                                                         int64_t k_per_split,
                                                         double *__restrict__ Gp) {
-    __shared__ double As[KT][LDA];
-    __shared__ double Bs[KT][LDA];
+    __shared__ T As[2][KT][LDA];
+    __shared__ T Bs[2][KT][LDA];
 
     // upper-triangle tile id -> (ta, tb), ta <= tb, row-major over the triangle
     const int tiles = (p + CT - 1) / CT;
@@ -64,12 +67,14 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const T *__restrict__ X,
     const int64_t k0 = split * k_per_split;
     int64_t k1 = k0 + k_per_split;
     if (k1 > n) k1 = n;
+    const int nslab = k1 > k0 ? (int)((k1 - k0 + KT - 1) / KT) : 0;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wr = (wave >> 1) * 32;  // wave row offset in tile
     const int wc = (wave & 1) * 32;   // wave col offset in tile
+    const int ci = lane & 15;
 
     f64x4 acc[2][2];
 #pragma unroll
@@ -77,42 +82,65 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const T *__restrict__ X,
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    // loader mapping: 256 threads cover 16 rows x 64 cols as 4 passes of 4 rows
+    double ma[2], mb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int a = a0 + wr + i * 16 + ci, b = b0 + wc + i * 16 + ci;
+        ma[i] = a < p ? mean[a] : 0.0;
+        mb[i] = b < p ? mean[b] : 0.0;
+    }
+
+    // loader: 256 threads cover KT rows x 64 cols as KT/4 passes of 4 rows
     const int lcol = tid & 63;
     const int lrow = tid >> 6;
     const int ga = a0 + lcol, gb = b0 + lcol;
-    const double ma = ga < p ? mean[ga] : 0.0;
-    const double mb = gb < p ? mean[gb] : 0.0;
+    T ra[KT / 4], rb[KT / 4];
 
-    for (int64_t kk = k0; kk < k1; kk += KT) {
+    auto load = [&](int s) {
+        const int64_t kk = k0 + (int64_t)s * KT;
 #pragma unroll
         for (int pass = 0; pass < KT / 4; ++pass) {
-            const int r = pass * 4 + lrow;
-            const int64_t k = kk + r;
-            double va = 0.0, vb = 0.0;
-            if (k < k1) {
-                if (ga < p) va = (double)X[k * p + ga] - ma;
-                if (gb < p) vb = (double)X[k * p + gb] - mb;
-            }
-            As[r][lcol] = va;
-            Bs[r][lcol] = vb;
+            const int64_t k = kk + pass * 4 + lrow;
+            const bool kv = k < k1;
+            ra[pass] = (kv && ga < p) ? X[k * p + ga] : T(0);
+            rb[pass] = (kv && gb < p) ? X[k * p + gb] : T(0);
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int pass = 0; pass < KT / 4; ++pass) {
+            As[buf][pass * 4 + lrow][lcol] = ra[pass];
+            Bs[buf][pass * 4 + lrow][lcol] = rb[pass];
+        }
+    };
+
+    if (nslab > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int s = 0; s < nslab; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < nslab) load(s + 1);
+        const int64_t kk = k0 + (int64_t)s * KT;
 #pragma unroll
         for (int kq = 0; kq < KT / 4; ++kq) {
             const int kr = kq * 4 + (lane >> 4);
-            const int ci = lane & 15;
+            const bool kv = kk + kr < k1;
             double af[2], bf[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) af[i] = As[kr][wr + i * 16 + ci];
+            for (int i = 0; i < 2; ++i)
+                af[i] = kv ? (double)As[cur][kr][wr + i * 16 + ci] - ma[i] : 0.0;
 #pragma unroll
-            for (int j = 0; j < 2; ++j) bf[j] = Bs[kr][wc + j * 16 + ci];
+            for (int j = 0; j < 2; ++j)
+                bf[j] = kv ? (double)Bs[cur][kr][wc + j * 16 + ci] - mb[j] : 0.0;
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
+        if (s + 1 < nslab) store(cur ^ 1);
         __syncthreads();
     }
 
@@ -123,7 +151,7 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const T *__restrict__ X,
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int col = b0 + wc + j * 16 + (lane & 15);
+            const int col = b0 + wc + j * 16 + ci;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = a0 + wr + i * 16 + (lane >> 4) + 4 * v;

@@ -33,9 +33,17 @@ constexpr int STRIPES = 8;
 // queue lengths.
 constexpr int NUM_COUNTERS = 64;
 constexpr int CNT_QUEUE = 32;
-// per-block statistics partials: 4 kernel slots x MAX_PHASE_BLOCKS x 8 words
+// statistics of the level-l kernels: 64 striped accumulators of 8 words each
+// (block b adds into stripe b % 64), zeroed by scan_kernel, summed by
+// level_stats_kernel
 constexpr int BSTAT_WORDS = 8;
+constexpr int BSTAT_STRIPES = 64;
 enum BStat { BS_EXEC = 0, BS_USEFUL = 1, BS_RECHECK = 2, BS_FLIP = 3, BS_MAXRATIO = 4 };
+// row-blocked FAST kernel: one wave per block, a fixed LDS budget, and work
+// items of up to PAIRS_PER_ITEM ordered pairs of one row
+constexpr int RB_BLOCKS_PER_CU = 16;
+constexpr int RB_LDS_BYTES = 16384;
+constexpr int PAIRS_PER_ITEM = 4;
 
 // per-level statistics slots (uint64 each)
 enum StatSlot { ST_EXEC = 0, ST_TOTAL = 1, ST_RECHECK = 2, ST_FLIP = 3, ST_MAXRATIO = 4, ST_MAXDEG = 5, ST_USEFUL = 6, ST_SLOTS = 8 };
@@ -147,9 +155,13 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ cntB, int p,
                                                     int level, int32_t *__restrict__ rowoffA,
                                                     int32_t *__restrict__ rowoffB,
+                                                    int32_t *__restrict__ itemoffA,
+                                                    int32_t *__restrict__ itemoffB,
                                                     unsigned int *__restrict__ counters,
+                                                    unsigned long long *__restrict__ bstats,
                                                     unsigned long long *__restrict__ st) {
-    __shared__ int sA[1024], sB[1024];
+    __shared__ int sA[1024], sB[1024], sIA[1024], sIB[1024];
+    if (bstats && threadIdx.x < BSTAT_STRIPES * BSTAT_WORDS) bstats[threadIdx.x] = 0;
     __shared__ unsigned long long sT[1024];
     __shared__ int sM[1024];
     const int t = threadIdx.x;
@@ -157,41 +169,55 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int32_t *__restrict__ 
     const int r0 = t * per;
     int r1 = r0 + per;
     if (r1 > p) r1 = p;
-    int la = 0, lb = 0, md = 0;
+    int la = 0, lb = 0, lia = 0, lib = 0, md = 0;
     unsigned long long tt = 0;
     for (int r = r0; r < r1; ++r) {
         la += cntA[r];
         lb += cntB[r];
+        lia += (cntA[r] + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
+        lib += (cntB[r] + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
         const int d = deg[r];
         md = d > md ? d : md;
         tt += (unsigned long long)d * binom(d - 1, level);
     }
     sA[t] = la;
     sB[t] = lb;
+    sIA[t] = lia;
+    sIB[t] = lib;
     sT[t] = tt;
     sM[t] = md;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        int va = 0, vb = 0;
+        int va = 0, vb = 0, via = 0, vib = 0;
         if (t >= off) {
             va = sA[t - off];
             vb = sB[t - off];
+            via = sIA[t - off];
+            vib = sIB[t - off];
         }
         __syncthreads();
         sA[t] += va;
         sB[t] += vb;
+        sIA[t] += via;
+        sIB[t] += vib;
         __syncthreads();
     }
-    int ea = sA[t] - la, eb = sB[t] - lb;
+    int ea = sA[t] - la, eb = sB[t] - lb, eia = sIA[t] - lia, eib = sIB[t] - lib;
     for (int r = r0; r < r1; ++r) {
         rowoffA[r] = ea;
         rowoffB[r] = eb;
+        itemoffA[r] = eia;
+        itemoffB[r] = eib;
         ea += cntA[r];
         eb += cntB[r];
+        eia += (cntA[r] + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
+        eib += (cntB[r] + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
     }
     if (t == 1023) {
         rowoffA[p] = sA[1023];
         rowoffB[p] = sB[1023];
+        itemoffA[p] = sIA[1023];
+        itemoffB[p] = sIB[1023];
         for (int c = 0; c < NUM_COUNTERS; ++c) counters[c] = 0;
     }
     for (int off = 512; off >= 1; off >>= 1) {
@@ -215,18 +241,8 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int32_t *__restrict__ 
 // variance), which scales the rounding error of H.
 // ----------------------------------------------------------------------------
 template <int L>
-__device__ __forceinline__ float z32_test(const float *__restrict__ C, int p, int i, int j,
-                                          const int (&S)[L], float &dmin, float &rho) {
-    float c[L][L];
-    float ci[L], cj[L];
-    const float cij = C[(size_t)i * p + j];
-#pragma unroll
-    for (int a = 0; a < L; ++a) {
-        ci[a] = C[(size_t)i * p + S[a]];
-        cj[a] = C[(size_t)j * p + S[a]];
-#pragma unroll
-        for (int b = 0; b <= a; ++b) c[a][b] = C[(size_t)S[a] * p + S[b]];
-    }
+__device__ __forceinline__ float z32_from_values(float cij, const float (&ci)[L], const float (&cj)[L],
+                                                 float (&c)[L][L], float &dmin, float &rho) {
     float rinv[L];
     dmin = 3.0e38f;
 #pragma unroll
@@ -263,6 +279,33 @@ __device__ __forceinline__ float z32_test(const float *__restrict__ C, int p, in
     dmin = fminf(dmin, fminf(h00, h11));
     rho = h01 * rsqrtf(fabsf(h00) * fabsf(h11));
     return fabsf(atanhf(rho));
+}
+
+template <int L>
+__device__ __forceinline__ float z32_test(const float *__restrict__ C, int p, int i, int j,
+                                          const int (&S)[L], float &dmin, float &rho) {
+    float c[L][L];
+    float ci[L], cj[L];
+    const float cij = C[(size_t)i * p + j];
+#pragma unroll
+    for (int a = 0; a < L; ++a) {
+        ci[a] = C[(size_t)i * p + S[a]];
+        cj[a] = C[(size_t)j * p + S[a]];
+#pragma unroll
+        for (int b = 0; b <= a; ++b) c[a][b] = C[(size_t)S[a] * p + S[b]];
+    }
+    return z32_from_values<L>(cij, ci, cj, c, dmin, rho);
+}
+
+// error estimate of the FP32 Fisher-z: rounding of C32 and of the (L+2)-step
+// elimination, amplified by the smallest pivot and by dz/drho = 1/(1-rho^2)
+template <int L>
+__device__ __forceinline__ float z32_error(float dmin, float rho) {
+    return EPS32 * (float)(L + 2) * (1.0f + 1.0f / fmaxf(dmin, 1e-30f)) / fmaxf(1.0f - rho * rho, 1e-30f);
+}
+
+__device__ __forceinline__ bool z32_bad(float z, float dmin, float rho) {
+    return !(dmin > 1e-5f) || !(z <= 3.0e38f) || !(fabsf(rho) < 1.0f);
 }
 
 // ----------------------------------------------------------------------------
@@ -484,43 +527,267 @@ __global__ __launch_bounds__(256) void gauss_phase_kernel(
             const unsigned long long x = wstat[w2][threadIdx.x];
             v = threadIdx.x == BS_MAXRATIO ? (x > v ? x : v) : v + x;
         }
-        bstats[(size_t)blockIdx.x * BSTAT_WORDS + threadIdx.x] = v;
+        unsigned long long *dst = bstats + (size_t)(blockIdx.x % BSTAT_STRIPES) * BSTAT_WORDS + threadIdx.x;
+        if (v) {
+            if (threadIdx.x == BS_MAXRATIO)
+                atomicMax(dst, v);
+            else
+                atomicAdd(dst, v);
+        }
     }
 }
 
-// Sums the per-block partials of the four phase kernels of one level into its
-// statistics slots.  One block.
-__global__ __launch_bounds__(256) void level_stats_kernel(const unsigned long long *__restrict__ bstats,
-                                                          int fast_blocks, int slow_blocks,
-                                                          unsigned long long *__restrict__ st) {
-    __shared__ unsigned long long red[256][5];
-    unsigned long long v[5] = {0, 0, 0, 0, 0};
-    for (int slot = 0; slot < 4; ++slot) {
-        const int nb = (slot & 1) ? slow_blocks : fast_blocks;
-        const unsigned long long *b = bstats + (size_t)slot * MAX_PHASE_BLOCKS * BSTAT_WORDS;
-        for (int i = threadIdx.x; i < nb; i += 256) {
-            for (int k = 0; k < 4; ++k) v[k] += b[(size_t)i * BSTAT_WORDS + k];
-            const unsigned long long m = b[(size_t)i * BSTAT_WORDS + BS_MAXRATIO];
-            v[4] = m > v[4] ? m : v[4];
+// ----------------------------------------------------------------------------
+// level l >= 1, FAST phase, row-blocked.  One wave per block; a work item is
+// (row i, up to PAIRS_PER_ITEM ordered pairs (i, j) of this phase).  The wave
+// stages what every test of row i reads into LDS:
+//   mode FULL   (deg + 1)^2 floats: C[A, A] with A = adj(i) + [i], so every CI
+//               test reads LDS only;
+//   mode ROW    neighbour ids, C[i, adj(i)] and diag C[s, s]; C[j, S] and the
+//               off-diagonal C[S, S] come from global memory (L2/MALL);
+//   mode GLOBAL nothing staged (only for degrees beyond the LDS budget).
+// Lane groups of GS lanes walk a pair's subset ranks in order as in the
+// unblocked kernel; near-threshold tests go to the FP64 queue.
+// ----------------------------------------------------------------------------
+template <int L, int GS>
+__global__ __launch_bounds__(64) void gauss_rowblock_kernel(
+    const float *__restrict__ C32, int p, const int32_t *__restrict__ nbr,
+    const int32_t *__restrict__ deg, const int32_t *__restrict__ rowoff,
+    const int32_t *__restrict__ itemoff, int side, float thr32, float margin_k,
+    uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, int maxL, float *__restrict__ zmin,
+    SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount,
+    unsigned long long *__restrict__ bstats) {
+    extern __shared__ int lds_words[];
+    const int lane = threadIdx.x;
+    const int gbase = lane & ~(GS - 1);
+    const int gl = lane & (GS - 1);
+    const int grp = lane / GS;
+    constexpr int NG = 64 / GS;
+    const uint64_t gmask = (GS == 64) ? ~0ull : (((1ull << GS) - 1ull) << gbase);
+    const int nitems = itemoff[p];
+    constexpr int WORDS = RB_LDS_BYTES / 4;
+
+    unsigned long long n_exec = 0, n_useful = 0;
+
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        int lo = 0, hi = p;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (itemoff[mid] <= item) lo = mid; else hi = mid;
+        }
+        const int i = lo;
+        const int chunk = item - itemoff[i];
+        const int d = deg[i];
+        const int cnt = rowoff[i + 1] - rowoff[i];
+        const int q0 = chunk * PAIRS_PER_ITEM;
+        const int q1 = min(cnt, q0 + PAIRS_PER_ITEM);
+        const int jbase = side == 0 ? d - cnt : 0;
+        const int D = d + 1;
+        const int mode = (D * D + d <= WORDS) ? 0 : (3 * d <= WORDS ? 1 : 2);
+        const int32_t *gnb = nbr + (size_t)i * p;
+
+        __syncthreads();  // the previous item is done with LDS
+        int *snb = lds_words;
+        float *sf = reinterpret_cast<float *>(lds_words + d);
+        if (mode != 2) {
+            for (int k0 = 0; k0 < d; k0 += 64 * 4) {
+                int v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * 64 + lane;
+                    v[u] = k < d ? gnb[k] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * 64 + lane;
+                    if (k < d) snb[k] = v[u];
+                }
+            }
+        }
+        __syncthreads();
+        // Loads are issued in batches before their LDS stores so that each lane
+        // keeps several gathers in flight.
+        if (mode == 0) {
+            // sub[a * D + b] = C[A[a], A[b]], A = adj(i) + [i]; D <= 63 < 64
+            const bool colok = lane < D;
+            const int vb = lane < d ? snb[colok ? lane : 0] : i;
+            for (int a0 = 0; a0 < D; a0 += 8) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int a = a0 + u;
+                    const int va = a < d ? snb[a] : i;
+                    v[u] = (a < D && colok) ? C32[(size_t)va * p + vb] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int a = a0 + u;
+                    if (a < D && colok) sf[a * D + lane] = v[u];
+                }
+            }
+        } else if (mode == 1) {
+            for (int k0 = 0; k0 < d; k0 += 64 * 4) {
+                float v[4], w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * 64 + lane;
+                    const int vv = k < d ? snb[k] : 0;
+                    v[u] = k < d ? C32[(size_t)i * p + vv] : 0.0f;
+                    w[u] = k < d ? C32[(size_t)vv * p + vv] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * 64 + lane;
+                    if (k < d) {
+                        sf[k] = v[u];
+                        sf[d + k] = w[u];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        const int Nn = d - 1;
+        const uint64_t count = binom(Nn, L);
+        for (int q = q0 + grp; q < q1; q += NG) {
+            const int jpos = jbase + q;
+            const int j = mode == 2 ? gnb[jpos] : snb[jpos];
+            if (count == 0) continue;
+            if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
+            float zl = __builtin_inff();
+            for (uint64_t iter = 0;; ++iter) {
+                const uint64_t rank = iter * GS + gl;
+                const bool act = rank < count;
+                bool pass = false, near = false;
+                float zf = __builtin_inff();
+                int sp[L];
+                if (act) {
+                    int pos[L];
+                    unrank_lex<L>(Nn, rank, pos);
+#pragma unroll
+                    for (int c = 0; c < L; ++c) {
+                        int idx = pos[c] - 1;
+                        sp[c] = idx + (idx >= jpos);
+                    }
+                    float cij, ci[L], cj[L], cm[L][L];
+                    if (mode == 0) {
+                        cij = sf[d * D + jpos];
+#pragma unroll
+                        for (int a = 0; a < L; ++a) {
+                            ci[a] = sf[d * D + sp[a]];
+                            cj[a] = sf[jpos * D + sp[a]];
+#pragma unroll
+                            for (int b = 0; b <= a; ++b) cm[a][b] = sf[sp[a] * D + sp[b]];
+                        }
+                    } else if (mode == 1) {
+                        int S[L];
+#pragma unroll
+                        for (int a = 0; a < L; ++a) S[a] = snb[sp[a]];
+                        cij = sf[jpos];
+#pragma unroll
+                        for (int a = 0; a < L; ++a) {
+                            ci[a] = sf[sp[a]];
+                            cj[a] = C32[(size_t)j * p + S[a]];
+#pragma unroll
+                            for (int b = 0; b < a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
+                            cm[a][a] = sf[d + sp[a]];
+                        }
+                    } else {
+                        int S[L];
+#pragma unroll
+                        for (int a = 0; a < L; ++a) S[a] = gnb[sp[a]];
+                        cij = C32[(size_t)i * p + j];
+#pragma unroll
+                        for (int a = 0; a < L; ++a) {
+                            ci[a] = C32[(size_t)i * p + S[a]];
+                            cj[a] = C32[(size_t)j * p + S[a]];
+#pragma unroll
+                            for (int b = 0; b <= a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
+                        }
+                    }
+                    float dmin, rho;
+                    const float z = z32_from_values<L>(cij, ci, cj, cm, dmin, rho);
+                    near = z32_bad(z, dmin, rho) || fabsf(z - thr32) <= margin_k * z32_error<L>(dmin, rho);
+                    pass = z < thr32;
+                    zf = z;
+                }
+                if (__ballot(near) & gmask) {
+#pragma unroll
+                    for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
+                    if (gl == 0) {
+                        const unsigned int slot = atomicAdd(qcount, 1u);
+                        queue[slot] = SlowItem{rowoff[i] + q, (unsigned int)iter, zl, 0};
+                    }
+                    break;
+                }
+                if (act) ++n_exec;
+                const uint64_t pm = __ballot(pass) & gmask;
+                bool finish;
+                if (pm) {
+                    const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
+                    if (act && gl <= P) {
+                        ++n_useful;
+                        if (zf < zl) zl = zf;
+                    }
+                    if (gl == P) {
+                        Gm[(size_t)i * p + j] = 0;
+                        Gm[(size_t)j * p + i] = 0;
+#pragma unroll
+                        for (int c = 0; c < L; ++c) {
+                            const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
+                            sep[((size_t)i * p + j) * maxL + c] = v;
+                            sep[((size_t)j * p + i) * maxL + c] = v;
+                        }
+                    }
+                    finish = true;
+                } else {
+                    if (act) {
+                        ++n_useful;
+                        if (zf < zl) zl = zf;
+                    }
+                    finish = (iter + 1) * GS >= count;
+                }
+                if (finish) {
+#pragma unroll
+                    for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
+                    if (gl == 0) {
+                        const int a = i < j ? i : j, b = i < j ? j : i;
+                        float *zp = zmin + (size_t)a * p + b;
+                        if (zl < *zp) *zp = zl;
+                    }
+                    break;
+                }
+            }
         }
     }
-    for (int k = 0; k < 5; ++k) red[threadIdx.x][k] = v[k];
-    __syncthreads();
-    for (int off = 128; off >= 1; off >>= 1) {
-        if (threadIdx.x < off)
-            for (int k = 0; k < 5; ++k) {
-                const unsigned long long o = red[threadIdx.x + off][k];
-                red[threadIdx.x][k] = k == 4 ? (o > red[threadIdx.x][k] ? o : red[threadIdx.x][k])
-                                             : red[threadIdx.x][k] + o;
-            }
-        __syncthreads();
+    for (int off = 32; off >= 1; off >>= 1) {
+        n_exec += __shfl_xor(n_exec, off);
+        n_useful += __shfl_xor(n_useful, off);
     }
-    if (threadIdx.x == 0) {
-        st[ST_EXEC] = red[0][BS_EXEC];
-        st[ST_USEFUL] = red[0][BS_USEFUL];
-        st[ST_RECHECK] = red[0][BS_RECHECK];
-        st[ST_FLIP] = red[0][BS_FLIP];
-        st[ST_MAXRATIO] = red[0][BS_MAXRATIO];
+    if (lane == 0 && n_exec) {
+        unsigned long long *dst = bstats + (size_t)(blockIdx.x % BSTAT_STRIPES) * BSTAT_WORDS;
+        atomicAdd(dst + BS_EXEC, n_exec);
+        atomicAdd(dst + BS_USEFUL, n_useful);
+    }
+}
+
+// Sums the striped accumulators of one level into its statistics slots.
+__global__ __launch_bounds__(64) void level_stats_kernel(const unsigned long long *__restrict__ bstats,
+                                                         unsigned long long *__restrict__ st) {
+    const int t = threadIdx.x;
+    unsigned long long v[5];
+    for (int k = 0; k < 5; ++k) v[k] = bstats[(size_t)t * BSTAT_WORDS + k];
+    for (int off = 32; off >= 1; off >>= 1) {
+        for (int k = 0; k < 4; ++k) v[k] += __shfl_xor(v[k], off);
+        const unsigned long long o = __shfl_xor(v[4], off);
+        v[4] = o > v[4] ? o : v[4];
+    }
+    if (t == 0) {
+        st[ST_EXEC] = v[BS_EXEC];
+        st[ST_USEFUL] = v[BS_USEFUL];
+        st[ST_RECHECK] = v[BS_RECHECK];
+        st[ST_FLIP] = v[BS_FLIP];
+        st[ST_MAXRATIO] = v[BS_MAXRATIO];
     }
 }
 
@@ -555,7 +822,7 @@ __global__ void to_u16_kernel(const uint8_t *__restrict__ g, size_t count, uint1
 // counters: see NUM_COUNTERS; all reset by scan_kernel.  bstats: per-block
 // statistics partials of the four phase kernels of a level.
 struct GaussWs {
-    int32_t *nbr, *deg, *cntA, *cntB, *rowoffA, *rowoffB;
+    int32_t *nbr, *deg, *cntA, *cntB, *rowoffA, *rowoffB, *itemoffA, *itemoffB;
     unsigned int *counters;
     SlowItem *queueA, *queueB;
     unsigned long long *bstats;
@@ -569,10 +836,10 @@ size_t gauss_workspace_bytes(int p) {
     size_t off = 0;
     off = align_up(off + sizeof(int32_t) * (size_t)p * p, 256);
     off = align_up(off + sizeof(int32_t) * (size_t)p * 3, 256);
-    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 2, 256);
+    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 4, 256);
     off = align_up(off + 256, 256);
     off = align_up(off + sizeof(SlowItem) * queue_capacity(p) * 2, 256);
-    off = align_up(off + sizeof(unsigned long long) * 4 * MAX_PHASE_BLOCKS * BSTAT_WORDS, 256);
+    off = align_up(off + sizeof(unsigned long long) * BSTAT_STRIPES * BSTAT_WORDS, 256);
     return off;
 }
 
@@ -588,7 +855,9 @@ static GaussWs carve(void *ws, int p) {
     off = align_up(off + sizeof(int32_t) * (size_t)p * 3, 256);
     w.rowoffA = (int32_t *)(b + off);
     w.rowoffB = w.rowoffA + (p + 1);
-    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 2, 256);
+    w.itemoffA = w.rowoffB + (p + 1);
+    w.itemoffB = w.itemoffA + (p + 1);
+    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 4, 256);
     w.counters = (unsigned int *)(b + off);
     off = align_up(off + 256, 256);
     w.queueA = (SlowItem *)(b + off);
@@ -612,6 +881,10 @@ static int phase_grid() {
     return g > MAX_PHASE_BLOCKS ? MAX_PHASE_BLOCKS : (g < 4 * STRIPES ? 4 * STRIPES : g);
 }
 static int slow_grid() { return phase_grid() / 4; }
+static int rowblock_grid() {
+    int g = phase_grid() / PHASE_BLOCKS_PER_CU * RB_BLOCKS_PER_CU;
+    return g > MAX_PHASE_BLOCKS ? MAX_PHASE_BLOCKS : g;
+}
 
 int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, hipStream_t s) {
     if (p < 1 || max_level < 0 || !Gm || !zmin) return set_error(ERR_ARG, "gauss_init: bad arguments");
@@ -636,7 +909,7 @@ int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long 
     GPUCSL_LAUNCH("compact", compact_kernel, dim3((p + 3) / 4), dim3(256), 0, s, Gm, p, w.nbr, w.deg, w.cntA,
                        w.cntB);
     GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(1), dim3(1024), 0, s, w.deg, w.cntA, w.cntB, p, level,
-                       w.rowoffA, w.rowoffB, w.counters, stats);
+                       w.rowoffA, w.rowoffB, w.itemoffA, w.itemoffB, w.counters, w.bstats, stats);
     return check_hip(hipGetLastError(), "compact launch");
 }
 
@@ -657,11 +930,12 @@ static void launch_phase(const float *C32, const double *C64, int p, const Gauss
     unsigned int *qcount = w.counters + CNT_QUEUE + side;
     unsigned int *fast_ctr = w.counters + (2 * side) * STRIPES;
     unsigned int *slow_ctr = w.counters + (2 * side + 1) * STRIPES;
-    unsigned long long *fast_bs = w.bstats + (size_t)(2 * side) * MAX_PHASE_BLOCKS * BSTAT_WORDS;
-    unsigned long long *slow_bs = w.bstats + (size_t)(2 * side + 1) * MAX_PHASE_BLOCKS * BSTAT_WORDS;
-    GPUCSL_LAUNCH(kPhaseFastName[L], (gauss_phase_kernel<L, GS, false>), dim3(phase_grid()), dim3(256), 0, s, C32,
-                       C64, p, w.nbr, w.deg, rowoff, side, (float)thr, thr, margin_k, Gm, sep, maxL,
-                       zmin, fast_ctr, queue, qcount, fast_bs);
+    unsigned long long *fast_bs = w.bstats;
+    unsigned long long *slow_bs = w.bstats;
+    (void)fast_ctr;
+    GPUCSL_LAUNCH(kPhaseFastName[L], (gauss_rowblock_kernel<L, GS>), dim3(rowblock_grid()), dim3(64),
+                  RB_LDS_BYTES, s, C32, p, w.nbr, w.deg, rowoff, side == 0 ? w.itemoffA : w.itemoffB,
+                  side, (float)thr, margin_k, Gm, sep, maxL, zmin, queue, qcount, fast_bs);
     GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_phase_kernel<L, GS, true>), dim3(slow_grid()), dim3(256), 0, s, C32,
                        C64, p, w.nbr, w.deg, rowoff, side, (float)thr, thr, margin_k, Gm, sep, maxL,
                        zmin, slow_ctr, queue, qcount, slow_bs);
@@ -706,8 +980,7 @@ int gauss_level_launch(int level, const float *C32, const double *C64, int p, in
         }
         if (rc) return rc;
     }
-    GPUCSL_LAUNCH("level_stats", level_stats_kernel, dim3(1), dim3(256), 0, s, w.bstats, phase_grid(),
-                  slow_grid(), stats);
+    GPUCSL_LAUNCH("level_stats", level_stats_kernel, dim3(1), dim3(64), 0, s, w.bstats, stats);
     return check_hip(hipGetLastError(), "gauss_level launch");
 }
 
