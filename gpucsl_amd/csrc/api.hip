@@ -86,6 +86,19 @@ int gauss_level_launch(int level, const float *C32, const double *C64, int p, in
 int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats,
                    hipStream_t s);
 int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s);
+size_t disc_workspace_bytes(int p, int max_level, int max_dim);
+int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *sep, float *pmax, void *ws,
+                     hipStream_t s);
+int disc_level0_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                       int max_level, int max_dim, double alpha, uint8_t *Gm, float *pmax,
+                       unsigned long long *st, void *ws, size_t ws_bytes, hipStream_t s);
+int disc_level_launch(int level, const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                      int max_level, int max_dim, double alpha, uint8_t *Gm, int32_t *sep, float *pmax,
+                      unsigned long long *st, void *ws, size_t ws_bytes, hipStream_t s);
+int disc_single_test_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int vi, int vj,
+                            const int32_t *S, int l, int sequential, uint32_t *table_out, double *out,
+                            hipStream_t s);
+int lower_minus_one_launch(float *pmax, int p, hipStream_t s);
 int to_u16_launch(const uint8_t *Gm, int p, uint16_t *out, hipStream_t s);
 
 // test hook: lexicographic unranking
@@ -232,6 +245,74 @@ int gpucsl_compact(const uint8_t *G, int p, int32_t *Gc, void *ws, size_t ws_byt
     const int32_t *deg = (const int32_t *)((char *)ws + align_up(sizeof(int32_t) * (size_t)p * p, 256));
     hipLaunchKernelGGL(gc_layout_kernel, dim3((p + 255) / 256, p), dim3(256), 0, S(stream), nbr, deg, p, Gc);
     return check_hip(hipGetLastError(), "gc layout");
+}
+
+size_t gpucsl_disc_workspace_bytes(int p, int max_level, int max_dim) {
+    return disc_workspace_bytes(p, max_level, max_dim);
+}
+
+int gpucsl_disc_init(int p, int max_level, int max_dim, uint8_t *G, int32_t *sepsets, float *pmax,
+                     uint64_t *stats, void *ws, size_t ws_bytes, void *stream) {
+    if (ws_bytes < disc_workspace_bytes(p, max_level, max_dim))
+        return set_error(ERR_WORKSPACE, "disc_init: workspace too small");
+    int rc = disc_init_launch(p, max_level, max_dim, G, sepsets, pmax, ws, S(stream));
+    if (rc) return rc;
+    if (stats) launch_fill<uint64_t>(stats, 0ull, (size_t)(max_level + 1) * GPUCSL_STAT_SLOTS, S(stream));
+    return check_hip(hipGetLastError(), "disc_init stats");
+}
+
+int gpucsl_disc_level0(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p, int max_level,
+                       int max_dim, double alpha, uint8_t *G, float *pmax, uint64_t *stats_level0, void *ws,
+                       size_t ws_bytes, void *stream) {
+    return disc_level0_launch(data, n, stride, dims, p, max_level, max_dim, alpha, G, pmax,
+                              reinterpret_cast<unsigned long long *>(stats_level0), ws, ws_bytes, S(stream));
+}
+
+int gpucsl_disc_level(int level, const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                      int max_level, int max_dim, double alpha, uint8_t *G, int32_t *sepsets, float *pmax,
+                      uint64_t *stats_level, void *ws, size_t ws_bytes, void *stream) {
+    return disc_level_launch(level, data, n, stride, dims, p, max_level, max_dim, alpha, G, sepsets, pmax,
+                             reinterpret_cast<unsigned long long *>(stats_level), ws, ws_bytes, S(stream));
+}
+
+int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                         int max_level, int max_dim, double alpha, uint8_t *G, int32_t *sepsets, float *pmax,
+                         uint64_t *stats, void *ws, size_t ws_bytes, void *stream) {
+    if (!stats) return set_error(ERR_ARG, "disc_skeleton: null stats");
+    int rc = gpucsl_disc_init(p, max_level, max_dim, G, sepsets, pmax, stats, ws, ws_bytes, stream);
+    if (rc) return rc;
+    rc = gpucsl_disc_level0(data, n, stride, dims, p, max_level, max_dim, alpha, G, pmax, stats, ws, ws_bytes,
+                            stream);
+    if (rc) return rc;
+    const int built = max_level < GPUCSL_MAX_LEVEL_TEMPLATE ? max_level : GPUCSL_MAX_LEVEL_TEMPLATE;
+    for (int l = 1; l <= built; ++l) {
+        rc = gpucsl_disc_level(l, data, n, stride, dims, p, max_level, max_dim, alpha, G, sepsets, pmax,
+                               stats + (size_t)l * GPUCSL_STAT_SLOTS, ws, ws_bytes, stream);
+        if (rc) return rc;
+    }
+    if (max_level > GPUCSL_MAX_LEVEL_TEMPLATE) {
+        const int next = GPUCSL_MAX_LEVEL_TEMPLATE + 1;
+        unsigned long long *st = reinterpret_cast<unsigned long long *>(stats) + (size_t)next * GPUCSL_STAT_SLOTS;
+        rc = compact_launch(G, p, next, ws, st, S(stream));
+        if (rc) return rc;
+        unsigned long long maxdeg = 0;
+        rc = check_hip(hipMemcpyAsync(&maxdeg, st + 5, sizeof(maxdeg), hipMemcpyDeviceToHost, S(stream)),
+                       "read max degree");
+        if (rc) return rc;
+        rc = check_hip(hipStreamSynchronize(S(stream)), "sync");
+        if (rc) return rc;
+        if ((long long)maxdeg - 1 >= next)
+            return set_error(ERR_UNSUPPORTED, "level %d has work (max degree %llu) but only levels <= %d are built",
+                             next, maxdeg, GPUCSL_MAX_LEVEL_TEMPLATE);
+    }
+    return lower_minus_one_launch(pmax, p, S(stream));
+}
+
+int gpucsl_disc_ci_test(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int vi, int vj,
+                        const int32_t *sepset, int l, int sequential, uint32_t *table_out, double *out,
+                        void *stream) {
+    if (!data || !dims || !table_out || !out || (l > 0 && !sepset)) return set_error(ERR_ARG, "disc_ci_test: null pointer");
+    return disc_single_test_launch(data, n, stride, dims, vi, vj, sepset, l, sequential, table_out, out, S(stream));
 }
 
 int gpucsl_profile_begin(void) {

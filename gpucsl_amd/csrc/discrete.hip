@@ -1,0 +1,539 @@
+// Discrete (Pearson chi^2) PC-stable skeleton on gfx950.
+//
+// Replaces gpucsl/pc/cuda/discrete_ci.cu (discrete_ci_level_0 / _level_n) and
+// gpucsl/pc/cuda/helpers/discrete_helpers.cu, with the host level loop of
+// gpucsl/pc/discover_skeleton_discrete.py:80-112.
+//
+// Data layout in HBM: uint8 category codes, column-major with a padded column
+// stride (data[v * stride + k], stride a multiple of 16, padding ignored), so a
+// lane reads 16 observations of a variable with one 16-byte load.
+//
+// One 256-thread workgroup evaluates one CI test: it counts the contingency
+// table N[g][a][b] (g = stratum of S in mixed radix, first sepset variable least
+// significant; a = value of v_i; b = value of v_j) with LDS atomics into one
+// table replica per wave, folds the replicas, forms the integer marginals, sums
+// chi^2 in FP64 in a fixed order and evaluates Perlman's pchisq.  Counts are
+// exact integers.  Level 0 sums chi^2 in the reference's sequential (a, b)
+// order; levels >= 1 use 64 lane-strided partial sums and a halving tree (the
+// reference's order there is that of atomicAdd(double) and not defined).
+//
+// Every sepset of every pair adjacent at level start is tested, as in the
+// reference (its early exit is a `continue`, discrete_ci.cu:240-243), so pmax is
+// the max over that full enumeration.  The removing sepset is the first passing
+// test in pcalg order (key = side << 62 | rank, side 0 for the row of the
+// smaller endpoint), chosen with one atomicMin per passing test and written by
+// a writeback kernel after the level.
+#include "common.h"
+
+namespace gpucsl {
+
+// in gauss.hip
+int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats,
+                   hipStream_t s);
+size_t gauss_workspace_bytes(int p);
+void level_list_views(void *ws, int p, int32_t **nbr, int32_t **deg);
+
+constexpr int DT_THREADS = 256;
+constexpr int DT_LDS_BYTES = 32 * 1024;
+constexpr int DT_LDS_WORDS = DT_LDS_BYTES / 4;
+constexpr int DT_BLOCKS_PER_CU = 4;
+constexpr unsigned long long KEY_NONE = ~0ull;
+enum DStat { DS_EXEC = 0, DS_TOTAL = 1, DS_MAXDEG = 5, DS_USEFUL = 6 };
+
+// int[L] with a one-element stand-in for L = 0
+template <int L>
+using IArr = int[L > 0 ? L : 1];
+
+struct DiscWs {
+    unsigned long long *testoff;  // p + 1 prefix of tests per row
+    unsigned long long *keys;     // p * p removal keys, upper triangle used
+    uint32_t *gtab;               // global tables for tests too large for LDS
+    size_t gtab_words;            // words per workgroup slice
+    int grid;                     // workgroups of the test kernels
+};
+
+// ----------------------------------------------------------------------------
+// one CI test inside a workgroup
+// ----------------------------------------------------------------------------
+// Counts the table of (vi, vj | S) into `tab` (cells words, zeroed here) using
+// `replicas` copies at stride `cells`; returns after the replicas are folded
+// into copy 0.  All threads of the block must call it.
+template <int L>
+__device__ void count_table(const uint8_t *__restrict__ data, int64_t n, int64_t stride, int vi,
+                            int vj, const IArr<L> &S, int di, int dj, const IArr<L> &radix,
+                            uint32_t *tab, int cells, int replicas) {
+    const int tid = threadIdx.x;
+    for (int c = tid; c < cells * replicas; c += DT_THREADS) tab[c] = 0;
+    __syncthreads();
+    uint32_t *mine = tab + (size_t)((tid >> 6) % replicas) * cells;
+    const uint8_t *ci = data + (size_t)vi * stride;
+    const uint8_t *cj = data + (size_t)vj * stride;
+    const int64_t chunks = (n + 15) / 16;
+    for (int64_t ch = tid; ch < chunks; ch += DT_THREADS) {
+        const uint4 wa = *reinterpret_cast<const uint4 *>(ci + ch * 16);
+        const uint4 wb = *reinterpret_cast<const uint4 *>(cj + ch * 16);
+        uint4 ws[L > 0 ? L : 1];
+#pragma unroll
+        for (int c = 0; c < L; ++c) ws[c] = *reinterpret_cast<const uint4 *>(data + (size_t)S[c] * stride + ch * 16);
+        const int lim = (ch * 16 + 16 <= n) ? 16 : (int)(n - ch * 16);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if (q < lim) {
+                const int word = q >> 2, sh = (q & 3) * 8;
+                const unsigned a = ((&wa.x)[word] >> sh) & 0xffu;
+                const unsigned b = ((&wb.x)[word] >> sh) & 0xffu;
+                unsigned g = 0;
+#pragma unroll
+                for (int c = 0; c < L; ++c) g += (((&ws[c].x)[word] >> sh) & 0xffu) * (unsigned)radix[c];
+                atomicAdd(&mine[(g * di + a) * dj + b], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    if (replicas > 1) {
+        for (int c = tid; c < cells; c += DT_THREADS) {
+            uint32_t v = tab[c];
+            for (int r = 1; r < replicas; ++r) v += tab[(size_t)r * cells + c];
+            tab[c] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Integer marginals: Nis[g*di + a], Njs[g*dj + b], Ns[g]
+// (gpucsl/pc/cuda/helpers/discrete_helpers.cu:97-133).
+__device__ void marginals(const uint32_t *N, int dim_s, int di, int dj, uint32_t *Nis, uint32_t *Njs,
+                          uint32_t *Ns) {
+    const int tid = threadIdx.x;
+    for (int r = tid; r < dim_s * di; r += DT_THREADS) {
+        const int g = r / di, a = r - g * di;
+        uint32_t v = 0;
+        for (int b = 0; b < dj; ++b) v += N[(g * di + a) * dj + b];
+        Nis[r] = v;
+    }
+    for (int r = tid; r < dim_s * dj; r += DT_THREADS) {
+        const int g = r / dj, b = r - g * dj;
+        uint32_t v = 0;
+        for (int a = 0; a < di; ++a) v += N[(g * di + a) * dj + b];
+        Njs[r] = v;
+    }
+    for (int g = tid; g < dim_s; g += DT_THREADS) {
+        uint32_t v = 0;
+        for (int c = 0; c < di * dj; ++c) v += N[g * di * dj + c];
+        Ns[g] = v;
+    }
+    __syncthreads();
+}
+
+// Per-cell term (O - E)^2 / E with E = N_i+s * N_+js / N_++s, skipping
+// N_++s = 0 or E = 0 (discrete_helpers.cu:140-186).
+__device__ __forceinline__ double chi2_term(const uint32_t *N, const uint32_t *Nis, const uint32_t *Njs,
+                                            const uint32_t *Ns, int di, int dj, int c) {
+#pragma clang fp contract(off)
+    const int g = c / (di * dj), a = (c / dj) % di, b = c % dj;
+    const double e = Nis[g * di + a] * (double)Njs[g * dj + b] / Ns[g];
+    if (Ns[g] == 0 || e == 0.0) return 0.0;
+    const double d = N[c] - e;
+    return d * d / e;
+}
+
+// chi^2 of the table; returns the value in thread 0 (others: unspecified).
+__device__ double chi2_sum(const uint32_t *N, const uint32_t *Nis, const uint32_t *Njs, const uint32_t *Ns,
+                           int dim_s, int di, int dj, bool sequential) {
+#pragma clang fp contract(off)
+    const int cells = dim_s * di * dj;
+    double acc = 0.0;
+    if (sequential) {
+        if (threadIdx.x == 0)
+            for (int c = 0; c < cells; ++c) {
+                acc += chi2_term(N, Nis, Njs, Ns, di, dj, c);
+            }
+        return acc;
+    }
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (int c = lane; c < cells; c += 64) acc += chi2_term(N, Nis, Njs, Ns, di, dj, c);
+        for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off);
+    }
+    return acc;
+}
+
+// ----------------------------------------------------------------------------
+// level 0: gpucsl/pc/cuda/discrete_ci.cu:37-97
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(DT_THREADS) void disc_level0_kernel(
+    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int p,
+    double alpha, uint8_t *__restrict__ Gm, float *__restrict__ pmax, uint32_t *__restrict__ gtab,
+    size_t gtab_words, unsigned long long *__restrict__ st) {
+    extern __shared__ uint32_t lds[];
+    const long long pairs = (long long)p * (p - 1) / 2;
+    if (blockIdx.x == 0) {
+        for (int v = threadIdx.x; v < p; v += DT_THREADS) Gm[(size_t)v * p + v] = 0;
+        if (threadIdx.x == 0) {
+            st[DS_EXEC] = pairs;
+            st[DS_TOTAL] = pairs;
+            st[DS_USEFUL] = pairs;
+        }
+    }
+    const IArr<0> dummyS = {0};
+    const IArr<0> noradix = {1};
+    for (long long t = blockIdx.x; t < pairs; t += gridDim.x) {
+        // t -> (vi, vj) with vj < vi: vi(vi-1)/2 <= t < vi(vi+1)/2
+        int vi = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
+        while ((long long)vi * (vi - 1) / 2 > t) --vi;
+        while ((long long)(vi + 1) * vi / 2 <= t) ++vi;
+        const int vj = (int)(t - (long long)vi * (vi - 1) / 2);
+        const int di = dims[vi], dj = dims[vj];
+        const int cells = di * dj;
+        const int marg = di + dj + 1;
+        uint32_t *tab;
+        int replicas;
+        if (4 * cells + marg <= DT_LDS_WORDS) {
+            tab = lds;
+            replicas = 4;
+        } else if (cells + marg <= DT_LDS_WORDS) {
+            tab = lds;
+            replicas = 1;
+        } else {
+            tab = gtab + (size_t)blockIdx.x * gtab_words;
+            replicas = 1;
+        }
+        count_table<0>(data, n, stride, vi, vj, dummyS, di, dj, noradix, tab, cells, replicas);
+        uint32_t *Nis = tab + (size_t)replicas * cells, *Njs = Nis + di, *Ns = Njs + dj;
+        marginals(tab, 1, di, dj, Nis, Njs, Ns);
+        const double chi2 = chi2_sum(tab, Nis, Njs, Ns, 1, di, dj, true);
+        if (threadIdx.x == 0) {
+            const double pv = pchisq(chi2, (di - 1) * (dj - 1));
+            pmax[(size_t)vj * p + vi] = (float)pv;
+            const uint8_t keep = (pv >= alpha) ? 0 : 1;
+            Gm[(size_t)vi * p + vj] = keep;
+            Gm[(size_t)vj * p + vi] = keep;
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------
+// level l >= 1: gpucsl/pc/cuda/discrete_ci.cu:113-246
+// ----------------------------------------------------------------------------
+// testoff[r] = sum over rows < r of deg * C(deg - 1, L); also T_l, max degree.
+__global__ __launch_bounds__(1024) void disc_testoff_kernel(const int32_t *__restrict__ deg, int p, int L,
+                                                            unsigned long long *__restrict__ testoff,
+                                                            unsigned long long *__restrict__ st) {
+    __shared__ unsigned long long sT[1024];
+    __shared__ int sM[1024];
+    const int t = threadIdx.x;
+    const int per = (p + 1023) / 1024;
+    const int r0 = t * per;
+    const int r1 = min(p, r0 + per);
+    unsigned long long loc = 0;
+    int md = 0;
+    for (int r = r0; r < r1; ++r) {
+        const int d = deg[r];
+        loc += (unsigned long long)d * binom(d - 1, L);
+        md = max(md, d);
+    }
+    sT[t] = loc;
+    sM[t] = md;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        unsigned long long v = t >= off ? sT[t - off] : 0;
+        __syncthreads();
+        sT[t] += v;
+        __syncthreads();
+    }
+    unsigned long long e = sT[t] - loc;
+    for (int r = r0; r < r1; ++r) {
+        testoff[r] = e;
+        e += (unsigned long long)deg[r] * binom(deg[r] - 1, L);
+    }
+    if (t == 1023) {
+        testoff[p] = sT[1023];
+        st[DS_EXEC] = sT[1023];
+        st[DS_TOTAL] = sT[1023];
+        st[DS_USEFUL] = sT[1023];
+    }
+    __syncthreads();
+    for (int off = 512; off >= 1; off >>= 1) {
+        if (t < off) sM[t] = max(sM[t], sM[t + off]);
+        __syncthreads();
+    }
+    if (t == 0) st[DS_MAXDEG] = (unsigned long long)sM[0];
+}
+
+template <int L>
+__global__ __launch_bounds__(DT_THREADS) void disc_level_kernel(
+    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int p,
+    double alpha, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+    const unsigned long long *__restrict__ testoff, float *__restrict__ pmax,
+    unsigned long long *__restrict__ keys, uint32_t *__restrict__ gtab, size_t gtab_words) {
+    extern __shared__ uint32_t lds[];
+    const unsigned long long T = testoff[p];
+    for (unsigned long long t = blockIdx.x; t < T; t += gridDim.x) {
+        int lo = 0, hi = p;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (testoff[mid] <= t) lo = mid; else hi = mid;
+        }
+        const int i = lo;
+        const int d = deg[i];
+        const unsigned long long per_pair = binom(d - 1, L);
+        const unsigned long long o = t - testoff[i];
+        const int jpos = (int)(o / per_pair);
+        const unsigned long long rank = o - (unsigned long long)jpos * per_pair;
+        const int32_t *row = nbr + (size_t)i * p;
+        const int j = row[jpos];
+        int pos[L];
+        IArr<L> S, radix;
+        unrank_lex<L>(d - 1, rank, pos);
+        int dim_s = 1;
+#pragma unroll
+        for (int c = 0; c < L; ++c) {
+            const int idx = pos[c] - 1;
+            S[c] = row[idx + (idx >= jpos)];
+            radix[c] = dim_s;
+            dim_s *= dims[S[c]];
+        }
+        const int di = dims[i], dj = dims[j];
+        const int cells = dim_s * di * dj;
+        const int marg = dim_s * (di + dj + 1);
+        uint32_t *tab;
+        int replicas;
+        if (4 * cells + marg <= DT_LDS_WORDS) {
+            tab = lds;
+            replicas = 4;
+        } else if (cells + marg <= DT_LDS_WORDS) {
+            tab = lds;
+            replicas = 1;
+        } else {
+            tab = gtab + (size_t)blockIdx.x * gtab_words;
+            replicas = 1;
+        }
+        count_table<L>(data, n, stride, i, j, S, di, dj, radix, tab, cells, replicas);
+        uint32_t *Nis = tab + (size_t)replicas * cells, *Njs = Nis + dim_s * di, *Ns = Njs + dim_s * dj;
+        marginals(tab, dim_s, di, dj, Nis, Njs, Ns);
+        const double chi2 = chi2_sum(tab, Nis, Njs, Ns, dim_s, di, dj, false);
+        if (threadIdx.x == 0) {
+            const float pf = (float)pchisq(chi2, (di - 1) * (dj - 1) * dim_s);
+            const int a = i < j ? i : j, b = i < j ? j : i;
+            atomic_max_nonneg(&pmax[(size_t)a * p + b], pf);
+            if ((double)pf >= alpha) {
+                const unsigned long long key = ((unsigned long long)(i < j ? 0 : 1) << 62) | rank;
+                atomicMin(&keys[(size_t)a * p + b], key);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Removes the edges whose key was set, writes their sepsets in both
+// directions and resets the keys for the next level.
+template <int L>
+__global__ void disc_writeback_kernel(int p, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+                                      unsigned long long *__restrict__ keys, uint8_t *__restrict__ Gm,
+                                      int32_t *__restrict__ sep, int maxL) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (b >= p || b <= a) return;
+    const unsigned long long key = keys[(size_t)a * p + b];
+    if (key == KEY_NONE) return;
+    keys[(size_t)a * p + b] = KEY_NONE;
+    const int side = (int)(key >> 62);
+    const unsigned long long rank = key & ((1ull << 62) - 1);
+    const int i = side ? b : a, j = side ? a : b;
+    const int32_t *row = nbr + (size_t)i * p;
+    const int d = deg[i];
+    int lo = 0, hi = d;  // position of j in the ascending neighbour list
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (row[mid] < j) lo = mid + 1; else hi = mid;
+    }
+    const int jpos = lo;
+    int pos[L];
+    unrank_lex<L>(d - 1, rank, pos);
+    for (int c = 0; c < L; ++c) {
+        const int idx = pos[c] - 1;
+        const int v = row[idx + (idx >= jpos)];
+        sep[((size_t)i * p + j) * maxL + c] = v;
+        sep[((size_t)j * p + i) * maxL + c] = v;
+    }
+    Gm[(size_t)a * p + b] = 0;
+    Gm[(size_t)b * p + a] = 0;
+}
+
+// Single test for known-answer checks: table (cells words), chi^2 and p.
+template <int L>
+__global__ __launch_bounds__(DT_THREADS) void disc_single_test_kernel(
+    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int vi,
+    int vj, const int32_t *__restrict__ Sin, int sequential, uint32_t *__restrict__ table_out,
+    double *__restrict__ out) {
+    extern __shared__ uint32_t lds[];
+    IArr<L> S, radix;
+    int dim_s = 1;
+    for (int c = 0; c < L; ++c) {
+        S[c] = Sin[c];
+        radix[c] = dim_s;
+        dim_s *= dims[S[c]];
+    }
+    const int di = dims[vi], dj = dims[vj];
+    const int cells = dim_s * di * dj;
+    count_table<L>(data, n, stride, vi, vj, S, di, dj, radix, lds, cells, 1);
+    uint32_t *Nis = lds + cells, *Njs = Nis + dim_s * di, *Ns = Njs + dim_s * dj;
+    marginals(lds, dim_s, di, dj, Nis, Njs, Ns);
+    const double chi2 = chi2_sum(lds, Nis, Njs, Ns, dim_s, di, dj, sequential != 0);
+    for (int c = threadIdx.x; c < cells; c += DT_THREADS) table_out[c] = lds[c];
+    if (threadIdx.x == 0) {
+        out[0] = chi2;
+        out[1] = pchisq(chi2, (di - 1) * (dj - 1) * dim_s);
+        out[2] = (double)((di - 1) * (dj - 1) * dim_s);
+    }
+}
+
+// -1 on and below the diagonal (gpucsl/pc/helpers.py:96-101).
+__global__ void lower_minus_one_kernel(float *__restrict__ pmax, int p) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (b < p && b <= a) pmax[(size_t)a * p + b] = -1.0f;
+}
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+int lower_minus_one_launch(float *pmax, int p, hipStream_t s) {
+    if (!pmax || p < 1) return set_error(ERR_ARG, "pmax finalize: bad arguments");
+    GPUCSL_LAUNCH("pmax_lower", lower_minus_one_kernel, dim3((p + 255) / 256, p), dim3(256), 0, s, pmax, p);
+    return check_hip(hipGetLastError(), "pmax finalize launch");
+}
+
+static int disc_grid() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, nn = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&nn, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nn <= 0)
+            nn = 256;
+        cus = nn;
+    }
+    return cus * DT_BLOCKS_PER_CU;
+}
+
+// Largest table (+ marginals) one test can need, in words.
+static size_t max_table_words(int max_dim, int level) {
+    size_t cells = (size_t)max_dim * max_dim, ds = 1;
+    for (int t = 0; t < level; ++t) ds *= (size_t)max_dim;
+    cells *= ds;
+    return cells + ds * (2 * (size_t)max_dim + 1);
+}
+
+static size_t gtab_words_for(int max_dim, int max_level) {
+    size_t need = max_table_words(max_dim, max_level);
+    return need <= (size_t)DT_LDS_WORDS ? 0 : align_up(need, 64);
+}
+
+size_t disc_workspace_bytes(int p, int max_level, int max_dim) {
+    size_t off = align_up(gauss_workspace_bytes(p), 256);
+    off = align_up(off + sizeof(unsigned long long) * (size_t)(p + 1), 256);
+    off = align_up(off + sizeof(unsigned long long) * (size_t)p * p, 256);
+    off = align_up(off + sizeof(uint32_t) * gtab_words_for(max_dim, max_level) * (size_t)disc_grid(), 256);
+    return off;
+}
+
+static DiscWs disc_carve(void *ws, int p, int max_level, int max_dim) {
+    DiscWs w;
+    char *b = (char *)ws;
+    size_t off = align_up(gauss_workspace_bytes(p), 256);
+    w.testoff = (unsigned long long *)(b + off);
+    off = align_up(off + sizeof(unsigned long long) * (size_t)(p + 1), 256);
+    w.keys = (unsigned long long *)(b + off);
+    off = align_up(off + sizeof(unsigned long long) * (size_t)p * p, 256);
+    w.gtab = (uint32_t *)(b + off);
+    w.gtab_words = gtab_words_for(max_dim, max_level);
+    w.grid = disc_grid();
+    return w;
+}
+
+int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *sep, float *pmax, void *ws,
+                     hipStream_t s) {
+    if (p < 1 || max_level < 0 || !Gm || !pmax || !ws) return set_error(ERR_ARG, "disc_init: bad arguments");
+    launch_fill<uint8_t>(Gm, (uint8_t)1, (size_t)p * p, s);
+    if (max_level > 0 && sep) launch_fill<int32_t>(sep, -1, (size_t)p * p * max_level, s);
+    launch_fill<float>(pmax, 0.0f, (size_t)p * p, s);
+    DiscWs w = disc_carve(ws, p, max_level, max_dim);
+    launch_fill<unsigned long long>(w.keys, KEY_NONE, (size_t)p * p, s);
+    return check_hip(hipGetLastError(), "disc_init launch");
+}
+
+int disc_level0_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                       int max_level, int max_dim, double alpha, uint8_t *Gm, float *pmax,
+                       unsigned long long *st, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (!data || !dims || !Gm || !pmax || !st || !ws || p < 1 || n < 1)
+        return set_error(ERR_ARG, "disc_level0: bad arguments");
+    if (stride < (n + 15) / 16 * 16 || stride % 16 != 0)
+        return set_error(ERR_ARG, "disc_level0: stride must be a multiple of 16 and >= n rounded up to 16");
+    if (((uintptr_t)data & 15) != 0) return set_error(ERR_ARG, "disc_level0: data must be 16-byte aligned");
+    if (ws_bytes < disc_workspace_bytes(p, max_level, max_dim))
+        return set_error(ERR_WORKSPACE, "disc_level0: workspace too small");
+    if (max_table_words(max_dim, 0) > (size_t)DT_LDS_WORDS && gtab_words_for(max_dim, max_level) < max_table_words(max_dim, 0))
+        return set_error(ERR_WORKSPACE, "disc_level0: table workspace too small");
+    DiscWs w = disc_carve(ws, p, max_level, max_dim);
+    GPUCSL_LAUNCH("disc_level0", disc_level0_kernel, dim3(w.grid), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n,
+                  stride, dims, p, alpha, Gm, pmax, w.gtab, w.gtab_words, st);
+    return check_hip(hipGetLastError(), "disc_level0 launch");
+}
+
+static const char *const kDiscName[] = {"", "disc_level1", "disc_level2", "disc_level3", "disc_level4",
+                                        "disc_level5", "disc_level6", "disc_level7", "disc_level8"};
+
+template <int L>
+static void disc_level_launch_t(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                                int maxL, double alpha, uint8_t *Gm, int32_t *sep, float *pmax,
+                                const DiscWs &w, int32_t *nbr, int32_t *deg, hipStream_t s) {
+    GPUCSL_LAUNCH(kDiscName[L], disc_level_kernel<L>, dim3(w.grid), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n,
+                  stride, dims, p, alpha, nbr, deg, w.testoff, pmax, w.keys, w.gtab, w.gtab_words);
+    GPUCSL_LAUNCH("disc_writeback", disc_writeback_kernel<L>, dim3((p + 255) / 256, p), dim3(256), 0, s, p, nbr,
+                  deg, w.keys, Gm, sep, maxL);
+}
+
+int disc_level_launch(int level, const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                      int max_level, int max_dim, double alpha, uint8_t *Gm, int32_t *sep, float *pmax,
+                      unsigned long long *st, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (level < 1 || level > max_level) return set_error(ERR_ARG, "disc_level: level %d outside [1, %d]", level, max_level);
+    if (level > GPUCSL_MAX_LEVEL_TEMPLATE)
+        return set_error(ERR_UNSUPPORTED, "disc_level: levels above %d are not built", GPUCSL_MAX_LEVEL_TEMPLATE);
+    if (!data || !dims || !Gm || !sep || !pmax || !st || !ws) return set_error(ERR_ARG, "disc_level: null pointer");
+    if (stride < (n + 15) / 16 * 16 || stride % 16 != 0)
+        return set_error(ERR_ARG, "disc_level: stride must be a multiple of 16 and >= n rounded up to 16");
+    if (ws_bytes < disc_workspace_bytes(p, max_level, max_dim))
+        return set_error(ERR_WORKSPACE, "disc_level: workspace too small");
+    int rc = compact_launch(Gm, p, level, ws, nullptr, s);
+    if (rc) return rc;
+    int32_t *nbr, *deg;
+    level_list_views(ws, p, &nbr, &deg);
+    DiscWs w = disc_carve(ws, p, max_level, max_dim);
+    GPUCSL_LAUNCH("disc_testoff", disc_testoff_kernel, dim3(1), dim3(1024), 0, s, deg, p, level, w.testoff, st);
+    switch (level) {
+#define GPUCSL_D(LV)                                                                                   \
+    case LV:                                                                                           \
+        disc_level_launch_t<LV>(data, n, stride, dims, p, max_level, alpha, Gm, sep, pmax, w, nbr, deg, s); \
+        break;
+        GPUCSL_D(1) GPUCSL_D(2) GPUCSL_D(3) GPUCSL_D(4) GPUCSL_D(5) GPUCSL_D(6) GPUCSL_D(7) GPUCSL_D(8)
+#undef GPUCSL_D
+    }
+    return check_hip(hipGetLastError(), "disc_level launch");
+}
+
+int disc_single_test_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int vi, int vj,
+                            const int32_t *S, int l, int sequential, uint32_t *table_out, double *out,
+                            hipStream_t s) {
+    if (l < 0 || l > 4) return set_error(ERR_ARG, "disc_ci_test: level must be in 0..4");
+    switch (l) {
+    case 0: hipLaunchKernelGGL(disc_single_test_kernel<0>, dim3(1), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n, stride, dims, vi, vj, S, sequential, table_out, out); break;
+    case 1: hipLaunchKernelGGL(disc_single_test_kernel<1>, dim3(1), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n, stride, dims, vi, vj, S, sequential, table_out, out); break;
+    case 2: hipLaunchKernelGGL(disc_single_test_kernel<2>, dim3(1), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n, stride, dims, vi, vj, S, sequential, table_out, out); break;
+    case 3: hipLaunchKernelGGL(disc_single_test_kernel<3>, dim3(1), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n, stride, dims, vi, vj, S, sequential, table_out, out); break;
+    case 4: hipLaunchKernelGGL(disc_single_test_kernel<4>, dim3(1), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n, stride, dims, vi, vj, S, sequential, table_out, out); break;
+    }
+    return check_hip(hipGetLastError(), "disc_ci_test launch");
+}
+
+}  // namespace gpucsl

@@ -903,6 +903,13 @@ int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, 
     return check_hip(hipGetLastError(), "gauss_level0 launch");
 }
 
+// Views of the level-start neighbour lists inside a Gaussian/discrete workspace.
+void level_list_views(void *ws, int p, int32_t **nbr, int32_t **deg) {
+    GaussWs w = carve(ws, p);
+    *nbr = w.nbr;
+    *deg = w.deg;
+}
+
 int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats,
                    hipStream_t s) {
     GaussWs w = carve(ws, p);

@@ -1,0 +1,55 @@
+"""Discrete skeleton discovery, drop-in for gpucsl/pc/discover_skeleton_discrete.py."""
+from __future__ import annotations
+
+from timeit import default_timer as timer
+
+import numpy as np
+
+from gpucsl_amd.cli.cli_util import GpucslError
+from gpucsl_amd.pc.helpers import SkeletonResult, log, log_time, timed
+
+
+@timed
+def discover_skeleton_gpu_discrete(
+    skeleton: np.ndarray,
+    data: np.ndarray,
+    alpha: float,
+    max_level: int,
+    num_variables: int,
+    num_observations: int,
+    kernels=None,
+    memory_restriction: int = None,
+    is_debug: bool = False,
+    should_log: bool = False,
+) -> SkeletonResult:
+    """Signature and result of gpucsl/pc/discover_skeleton_discrete.py:24-128.
+
+    The data are ordinal-encoded on the host, uploaded once in a padded
+    column-major layout and every level runs on the GPU.  ``memory_restriction``
+    bounds the workspace in bytes: a RuntimeError is raised if it is smaller
+    than what the tests of max_level need (the reference raises in the same
+    situation, gpucsl/pc/kernel_management.py:434-442).
+    """
+    import torch
+
+    from gpucsl_amd.discrete import DiscreteEngine, ordinal_encode
+
+    if skeleton is not None and tuple(np.shape(skeleton)) != (num_variables, num_variables):
+        raise GpucslError(f"skeleton must be {num_variables}x{num_variables}")
+    codes, dims = ordinal_encode(data)
+    max_dim = int(dims.max()) if dims.size else 1
+    engine = DiscreteEngine(num_variables, num_observations, max_level, alpha, max_dim)
+    if memory_restriction is not None and engine.workspace_bytes() > memory_restriction:
+        raise RuntimeError(f"Out of memory for maximum level of {max_level}: the workspace needs "
+                           f"{engine.workspace_bytes()} bytes, the restriction is {memory_restriction}")
+    engine.set_codes(codes, dims)
+    torch.cuda.synchronize()
+    start = timer()
+    engine.run()
+    res = engine.results()
+    computation_time = timer() - start
+    log_time("Computation time", computation_time)
+    if should_log:
+        for level, st in enumerate(res.stats):
+            log(f"level {level}: {st}")
+    return SkeletonResult(res.skeleton, res.separation_sets, res.pmax, computation_time)

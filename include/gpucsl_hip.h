@@ -104,6 +104,48 @@ int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_le
 int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * discrete skeleton (Pearson chi^2)
+ * data: uint8 category codes, column-major with a padded column stride:
+ * data[v*stride + k], stride a multiple of 16 and >= n rounded up to 16, data
+ * 16-byte aligned; dims[v] = number of categories of v; max_dim = max dims.
+ * ------------------------------------------------------------------------- */
+size_t gpucsl_disc_workspace_bytes(int p, int max_level, int max_dim);
+
+/* G = 1, sepsets = -1, pmax = 0, stats = 0, removal keys cleared.  Replaces the
+ * buffer set-up of gpucsl/pc/discover_skeleton_discrete.py:41-55. */
+int gpucsl_disc_init(int p, int max_level, int max_dim, uint8_t *G, int32_t *sepsets, float *pmax,
+                     uint64_t *stats, void *ws, size_t ws_bytes, void *stream);
+
+/* Level 0: chi^2 test of every pair, removal on p >= alpha, pmax.  Replaces
+ * discrete_ci_level_0 (gpucsl/pc/cuda/discrete_ci.cu:37-97). */
+int gpucsl_disc_level0(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                       int max_level, int max_dim, double alpha, uint8_t *G, float *pmax,
+                       uint64_t *stats_level0, void *ws, size_t ws_bytes, void *stream);
+
+/* Level l >= 1: compaction, every test of every level-start adjacent pair,
+ * pmax, removal with the pcalg-order first sepset.  Replaces one iteration of
+ * the level loop of gpucsl/pc/discover_skeleton_discrete.py:80-112 (compact +
+ * discrete_ci_level_n, gpucsl/pc/cuda/discrete_ci.cu:185-246). */
+int gpucsl_disc_level(int level, const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                      int max_level, int max_dim, double alpha, uint8_t *G, int32_t *sepsets, float *pmax,
+                      uint64_t *stats_level, void *ws, size_t ws_bytes, void *stream);
+
+/* Levels 0..max_level and the pmax post-processing (-1 on and below the
+ * diagonal, gpucsl/pc/helpers.py:96-101), no host synchronisation.  Replaces
+ * gpucsl/pc/discover_skeleton_discrete.py:24-128 after ordinal encoding. */
+int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
+                         int max_level, int max_dim, double alpha, uint8_t *G, int32_t *sepsets, float *pmax,
+                         uint64_t *stats, void *ws, size_t ws_bytes, void *stream);
+
+/* One CI test, for known-answer checks: table_out gets the dims_s*di*dj counts
+ * N[g][a][b], out[0..2] = chi^2, p value, degrees of freedom.  sepset is a
+ * device array of l ids; sequential selects the level-0 summation order.
+ * (gpucsl/pc/cuda/helpers/discrete_helpers.cu:25-186, math_helpers.cu:128-175) */
+int gpucsl_disc_ci_test(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int vi, int vj,
+                        const int32_t *sepset, int l, int sequential, uint32_t *table_out, double *out,
+                        void *stream);
+
+/* ---------------------------------------------------------------------------
  * graph helpers
  * ------------------------------------------------------------------------- */
 /* Reference layout of compact (gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56):

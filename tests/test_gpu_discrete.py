@@ -1,0 +1,144 @@
+"""Discrete (Pearson chi^2) skeleton on the GPU against the FP64 oracle and the
+reference's known-answer vectors (GPU box only)."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gpucsl_amd import _native as nat
+from gpucsl_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _engine(codes, max_level=1, alpha=0.05):
+    from gpucsl_amd.discrete import DiscreteEngine
+
+    codes = np.asarray(codes, np.uint8)
+    n, p = codes.shape
+    dims = (codes.max(axis=0).astype(np.int64) + 1).astype(np.uint8)
+    eng = DiscreteEngine(p, n, max_level, alpha, int(dims.max()))
+    eng.set_codes(codes, dims)
+    return eng, dims
+
+
+def test_contingency_level0_golden(golden):
+    c = golden["contingency_level0"]
+    codes = np.stack([c["a"], c["b"]], axis=1)
+    eng, _ = _engine(codes)
+    table, chi2, p, df = eng.ci_test(0, 1, [])
+    assert table.reshape(7, 2).tolist() == c["expected"]
+    assert df == 6
+
+
+def test_contingency_level2_golden(golden):
+    c = golden["contingency_level2"]
+    eng, _ = _engine(np.array(c["data"]))
+    table, _, _, df = eng.ci_test(c["vi"], c["vj"], c["sepset"])
+    assert table.tolist() == c["expected"]
+    assert df == 4 * 4 * 25
+
+
+def test_chi2_golden_and_pchisq(golden):
+    """The reference's table of tests/test_cuda_discrete_helpers.py:128-172,
+    rebuilt as observations (stratum s, v_i = a, v_j = b)."""
+    c = golden["marginals_chi2"]
+    N = np.array(c["N"]).reshape(5, 5, 5)
+    rows = [(s, a, b) for s, a, b in itertools.product(range(5), repeat=3) for _ in range(N[s, a, b])]
+    codes = np.array(rows, np.uint8)
+    eng, _ = _engine(codes)
+    table, chi2, p, df = eng.ci_test(1, 2, [0])
+    assert table.tolist() == N.flatten().tolist()
+    assert abs(chi2 - c["chi2"]) <= c["tol"]
+    assert df == 80
+    assert p == pytest.approx(oracle.pchisq(chi2, df), rel=1e-14, abs=1e-300)
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
+def test_single_tests_match_oracle(l):
+    d = syn.discrete_bn(12, 3001, 2.0, seed=40 + l)
+    codes = d.data
+    eng, dims = _engine(codes)
+    colmajor = np.ascontiguousarray(codes.T)
+    rng = np.random.default_rng(l)
+    for _ in range(20):
+        vs = rng.choice(12, size=l + 2, replace=False)
+        vi, vj, S = int(vs[0]), int(vs[1]), [int(v) for v in sorted(vs[2:])]
+        table, chi2, p, df = eng.ci_test(vi, vj, S)
+        ochi2, odf, work = oracle.disc_chi2(colmajor, dims, vi, vj, S)
+        assert table.tolist() == work[: table.size].tolist()
+        assert df == odf
+        assert chi2 == ochi2 or abs(chi2 - ochi2) <= 1e-12 * abs(ochi2)
+        assert p == pytest.approx(oracle.pchisq(ochi2, odf), rel=1e-13, abs=1e-300)
+
+
+def _run(codes, alpha, L):
+    eng, dims = _engine(codes, L, alpha)
+    eng.run()
+    res = eng.results()
+    ref = oracle.disc_skeleton(np.ascontiguousarray(np.asarray(codes, np.uint8).T), dims, alpha, L)
+    return res, ref
+
+
+def _compare(res, ref, label):
+    same_skel = (res.skeleton == ref["skeleton"]).all()
+    same_sep = (res.separation_sets == ref["sepsets"]).all()
+    iu = np.triu_indices(res.skeleton.shape[0], 1)
+    dp = np.abs(res.pmax[iu].astype(np.float64) - ref["pmax"][iu])
+    exact = float((res.pmax[iu] == ref["pmax"][iu]).mean()) if iu[0].size else 1.0
+    say(f"[{label}] skeleton_equal={same_skel} sepsets_equal={same_sep} pmax_bitexact={exact:.5f} "
+        f"max|dpmax|={dp.max() if dp.size else 0:.3g} edges={int(res.skeleton.sum()) // 2}")
+    for l, st in enumerate(res.stats):
+        say(f"   level {l}: {st} oracle_exec={int(ref['executed'][l])}")
+    assert same_skel and same_sep
+    assert dp.max() <= 1e-6
+    assert (res.pmax[np.tril_indices(res.skeleton.shape[0])] == -1).all()
+    for l in range(len(res.stats)):
+        assert res.stats[l]["executed"] == ref["executed"][l]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_small_parity(seed):
+    d = syn.discrete_bn(30, 4000, 2.0, seed=seed)
+    res, ref = _run(d.data, 0.05, 3)
+    _compare(res, ref, f"disc seed {seed}")
+
+
+def test_c3_parity():
+    """Config C3: p=200, n=50000, cardinalities 2..5, degree 2, alpha=0.01, L=3 (seed 200)."""
+    d = syn.discrete_bn(200, 50000, 2.0, seed=200)
+    res, ref = _run(d.data, 0.01, 3)
+    _compare(res, ref, "C3")
+
+
+def test_constant_column_and_unaligned_n():
+    d = syn.discrete_bn(10, 1237, 2.0, seed=5)
+    codes = np.concatenate([d.data, np.zeros((1237, 1), np.uint8)], axis=1)
+    res, ref = _run(codes, 0.05, 2)
+    _compare(res, ref, "constant column")
+    assert res.skeleton[10].sum() == 0          # df = 0 -> p = 1 -> removed at level 0
+
+
+def test_dropin_api_and_encoding():
+    from gpucsl_amd.pc.discover_skeleton_discrete import discover_skeleton_gpu_discrete
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    d = syn.discrete_bn(15, 3000, 2.0, seed=8)
+    raw = d.data.astype(np.int64) * 7 + 3          # non-contiguous category labels
+    res, runtime = discover_skeleton_gpu_discrete(init_pc_graph(raw), raw, 0.05, 2, 15, 3000)
+    assert runtime > 0 and res.discover_skeleton_kernel_time > 0
+    dims = (d.data.max(axis=0) + 1).astype(np.uint8)
+    ref = oracle.disc_skeleton(np.ascontiguousarray(d.data.T), dims, 0.05, 2)
+    assert (res.skeleton == ref["skeleton"]).all()
+    assert (res.seperation_sets == ref["sepsets"]).all()
+    with pytest.raises(RuntimeError):
+        discover_skeleton_gpu_discrete(init_pc_graph(raw), raw, 0.05, 2, 15, 3000, memory_restriction=10)
