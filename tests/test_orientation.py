@@ -1,0 +1,98 @@
+"""Edge orientation (CPU): Meek rules R1-R3 and v-structures on the hand-built
+graphs of the reference's tests (tests/edge_orientation/test_orientation_rules.py:16-72,
+tests/edge_orientation/test_v_structures.py:37-116), plus CPDAG invariants."""
+import networkx as nx
+import numpy as np
+
+from gpucsl_amd.pc.edge_orientation.edge_orientation import orient_edges
+from gpucsl_amd.pc.edge_orientation.orientation_rules import apply_rules
+from gpucsl_amd.pc.edge_orientation.v_structures import orient_v_structure
+
+
+def sepsets(p, L, entries):
+    s = np.full((p, p, L), -1, np.int32)
+    for (a, b), vals in entries.items():
+        s[a, b, : len(vals)] = vals
+    return s
+
+
+def digraph(nodes, arcs):
+    g = nx.DiGraph()
+    g.add_nodes_from(nodes)
+    g.add_edges_from(arcs)
+    return g
+
+
+def test_rule1_chain_continues_direction():
+    # 1 -> 2 - 3 with 1, 3 non-adjacent: 2 -> 3
+    g = digraph([1, 2, 3], [(1, 2), (2, 3), (3, 2)])
+    apply_rules(g)
+    assert g.has_edge(1, 2) and g.has_edge(2, 3) and not g.has_edge(3, 2)
+
+
+def test_rule2_avoids_cycle():
+    # 1 -> 3 -> 2 (directed) and 1 - 2: 1 -> 2; 1 - 3 stays as given
+    g = digraph([1, 2, 3], [(1, 3), (3, 1), (1, 2), (2, 3)])
+    apply_rules(g)
+    assert not g.has_edge(3, 1) or g.has_edge(1, 3)
+    assert g.has_edge(1, 2) and g.has_edge(2, 3)
+
+
+def test_rule3_two_collider_parents():
+    g = digraph([1, 2, 3, 4], [(1, 2), (2, 1), (1, 3), (3, 1), (1, 4), (4, 1), (3, 2), (4, 2)])
+    apply_rules(g)
+    assert not g.has_edge(2, 1)
+    for e in [(1, 2), (1, 3), (3, 1), (1, 4), (4, 1), (3, 2), (4, 2)]:
+        assert g.has_edge(*e)
+
+
+def test_v_structure_respects_sepset():
+    edges = [(1, 2), (2, 1), (2, 3), (3, 2)]
+    g = digraph([1, 2, 3], edges)
+    orient_v_structure(g, sepsets(4, 2, {(1, 3): [2], (3, 1): [2]}))
+    assert all(g.has_edge(*e) for e in edges)             # 2 separates 1 and 3: no collider
+    orient_v_structure(g, sepsets(4, 2, {}))
+    assert g.has_edge(1, 2) and g.has_edge(3, 2)
+    assert not g.has_edge(2, 1) and not g.has_edge(2, 3)
+
+
+def test_v_structure_conflicting_triples_follow_visit_order():
+    # path 1 - 2 - 3 - 4 with empty sepsets: colliders at 2 and 3 conflict; the
+    # column-major visit order resolves it as 1 -> 2 -> 3 <- 4
+    g = digraph([1, 2, 3, 4], [(1, 2), (2, 1), (2, 3), (3, 2), (3, 4), (4, 3)])
+    orient_v_structure(g, sepsets(5, 2, {}))
+    assert g.has_edge(1, 2) and g.has_edge(2, 3) and g.has_edge(4, 3)
+    assert not g.has_edge(2, 1) and not g.has_edge(3, 2) and not g.has_edge(3, 4)
+
+
+def test_v_structure_partially_directed_input():
+    g = digraph([1, 2, 3], [(1, 2), (2, 1), (3, 2)])
+    orient_v_structure(g, sepsets(4, 2, {(1, 3): [2], (3, 1): [2]}))
+    assert g.has_edge(1, 2) and g.has_edge(2, 1) and g.has_edge(3, 2)
+    orient_v_structure(g, sepsets(4, 2, {}))
+    assert not g.has_edge(2, 1) and g.has_edge(1, 2)
+
+
+def test_orient_edges_collider_and_chain_cpdag():
+    # skeleton 0 - 2 - 1 (collider at 2, 0 and 1 separated by the empty set) and 2 - 3
+    skel = np.zeros((4, 4), np.uint16)
+    for a, b in [(0, 2), (1, 2), (2, 3)]:
+        skel[a, b] = skel[b, a] = 1
+    sep = sepsets(4, 2, {})
+    sep[0, 3, 0] = sep[3, 0, 0] = 2
+    sep[1, 3, 0] = sep[3, 1, 0] = 2
+    dag, runtime = orient_edges(nx.DiGraph(skel), sep)
+    assert runtime >= 0
+    assert set(dag.edges) == {(0, 2), (1, 2), (2, 3)}     # R1 orients 2 -> 3
+
+
+def test_orient_edges_is_a_pdag_of_the_skeleton():
+    rng = np.random.default_rng(0)
+    p = 12
+    skel = np.triu((rng.random((p, p)) < 0.3).astype(np.uint16), 1)
+    skel = skel + skel.T
+    sep = np.full((p, p, 2), -1, np.int32)
+    dag, _ = orient_edges(nx.DiGraph(skel), sep)
+    und = {frozenset(e) for e in dag.edges}
+    want = {frozenset((a, b)) for a in range(p) for b in range(p) if skel[a, b]}
+    assert und == want                                    # no edge added or lost
