@@ -31,7 +31,17 @@ SIGNATURES = {
     "gpucsl_corr_c64_to_c32": (_I, [_P, _I, _P, _P]),
     "gpucsl_gauss_workspace_bytes": (_SZ, [_I, _I]),
     "gpucsl_gauss_init": (_I, [_I, _I, _P, _P, _P, _P, _P]),
-    "gpucsl_gauss_level0": (_I, [_P, _P, _I, _D, _F, _P, _P, _P, _P]),
+    "gpucsl_gauss_level0": (_I, [_P, _P, _I, _D, _F, _P, _P, _P, _P, _SZ, _P]),
+    "gpucsl_gauss_job_workspace_bytes": (_SZ, [_I]),
+    "gpucsl_corr_k_splits": (_I, [_I64, _I]),
+    "gpucsl_corr_job_workspace_bytes": (_SZ, [_I, _I]),
+    "gpucsl_gauss_batch_workspace_bytes": (_SZ, [_P, _I]),
+    "gpucsl_gauss_batch_corr": (_I, [_P, _P, _I, _P]),
+    "gpucsl_gauss_batch_init": (_I, [_P, _P, _I, _P]),
+    "gpucsl_gauss_batch_level0": (_I, [_P, _P, _I, _F, _P, _P]),
+    "gpucsl_gauss_batch_level": (_I, [_I, _P, _P, _I, _F, _I, _P, _P, _SZ, _P]),
+    "gpucsl_gauss_batch_finish": (_I, [_P, _P, _I, _P]),
+    "gpucsl_gauss_batch_skeleton": (_I, [_P, _P, _I, _F, _I, _P, _P, _SZ, _P]),
     "gpucsl_gauss_level": (_I, [_I, _P, _P, _I, _I, _D, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_gauss_skeleton": (_I, [_P, _P, _I, _I, _P, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_pmax_from_zmin": (_I, [_P, _I, _I64, _P, _P]),

@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "../../include/gpucsl_hip.h"
-#include "common.h"
+#include "job.h"
 
 namespace gpucsl {
 
@@ -49,7 +49,7 @@ static hipEvent_t take() {
         return e;
     }
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     return e;
 }
 
@@ -58,13 +58,13 @@ bool enabled() { return g_on; }
 void record_start(hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_mu);
     t_start = take();
-    hipEventRecord(t_start, s);
+    (void)hipEventRecord(t_start, s);
 }
 
 void record_stop(const char *name, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_mu);
     hipEvent_t e = take();
-    hipEventRecord(e, s);
+    (void)hipEventRecord(e, s);
     g_recs.push_back(Rec{name, t_start, e});
     t_start = nullptr;
 }
@@ -72,20 +72,32 @@ void record_stop(const char *name, hipStream_t s) {
 
 // defined in corr.hip / gauss.hip
 size_t corr_workspace_bytes(int64_t n, int p);
+int corr_k_splits(int64_t n, int p);
 int corr_launch(const void *X, int dtype, int64_t n, int p, double *C64, float *C32, void *ws,
                 size_t ws_bytes, hipStream_t s);
 int c64_to_c32_launch(const double *C64, int p, float *C32, hipStream_t s);
+int corr_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStream_t s);
+int c64_to_c32_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStream_t s);
 size_t gauss_workspace_bytes(int p);
 int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, hipStream_t s);
-int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, float margin_k,
-                        uint8_t *Gm, float *zmin, unsigned long long *stats, hipStream_t s);
-int gauss_level_launch(int level, const float *C32, const double *C64, int p, int max_level,
-                       double thr, float margin_k, int group_size, uint8_t *Gm, int32_t *sep,
-                       float *zmin, unsigned long long *stats, void *ws, size_t ws_bytes,
-                       hipStream_t s);
-int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats,
-                   hipStream_t s);
+int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, float margin_k, uint8_t *Gm,
+                        float *zmin, unsigned long long *stats, void *ws, size_t ws_bytes, hipStream_t s);
+int gauss_level_launch(int level, const float *C32, const double *C64, int p, int max_level, double thr,
+                       float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
+                       unsigned long long *stats, void *ws, size_t ws_bytes, hipStream_t s);
+int gauss_skeleton_levels(const float *C32, const double *C64, int p, int max_level, const double *thr,
+                          float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
+                          unsigned long long *stats, void *ws, size_t ws_bytes, hipStream_t s);
+int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats, hipStream_t s);
+void level_list_views(void *ws, int p, int32_t **nbr, int32_t **deg);
 int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s);
+size_t gauss_batch_ws_bytes(const Job *jobs_host, int nj);
+int gauss_batch_init(const Job *jh, const Job *jd, int nj, hipStream_t s);
+int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, unsigned long long *st, hipStream_t s);
+int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, unsigned long long *st, hipStream_t s);
+int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float margin_k, int group_size,
+                      unsigned long long *st, void *bws, size_t bws_bytes, hipStream_t s);
+int gauss_batch_finish(const Job *jh, const Job *jd, int nj, hipStream_t s);
 size_t disc_workspace_bytes(int p, int max_level, int max_dim);
 int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *sep, float *pmax, void *ws,
                      hipStream_t s);
@@ -134,6 +146,26 @@ using namespace gpucsl;
 
 static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Levels above the built range only matter if some row still has enough
+// neighbours for them: compact once more, read the max degree (one host sync).
+template <typename CompactFn>
+static int check_unbuilt_levels(int max_level, uint64_t *stats, CompactFn compact, hipStream_t s) {
+    if (max_level <= GPUCSL_MAX_LEVEL_TEMPLATE) return OK;
+    const int next = GPUCSL_MAX_LEVEL_TEMPLATE + 1;
+    unsigned long long *st = reinterpret_cast<unsigned long long *>(stats) + (size_t)next * GPUCSL_STAT_SLOTS;
+    int rc = compact(next, st);
+    if (rc) return rc;
+    unsigned long long maxdeg = 0;
+    rc = check_hip(hipMemcpyAsync(&maxdeg, st + 5, sizeof(maxdeg), hipMemcpyDeviceToHost, s), "read max degree");
+    if (rc) return rc;
+    rc = check_hip(hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    if ((long long)maxdeg - 1 >= next)
+        return set_error(ERR_UNSUPPORTED, "level %d has work (max degree %llu) but only levels <= %d are built", next,
+                         maxdeg, GPUCSL_MAX_LEVEL_TEMPLATE);
+    return OK;
+}
+
 extern "C" {
 
 const char *gpucsl_last_error(void) { return g_err; }
@@ -175,6 +207,14 @@ size_t gpucsl_gauss_workspace_bytes(int p, int max_level) {
     return gauss_workspace_bytes(p);
 }
 
+size_t gpucsl_gauss_job_workspace_bytes(int p) { return p < 1 ? 0 : job_ws_bytes(p); }
+
+int gpucsl_corr_k_splits(int64_t n, int p) { return corr_k_splits(n, p); }
+
+size_t gpucsl_corr_job_workspace_bytes(int p, int k_splits) {
+    return p < 1 || k_splits < 1 ? 0 : corr_job_ws_bytes(p, k_splits);
+}
+
 int gpucsl_gauss_init(int p, int max_level, uint8_t *G, int32_t *sepsets, float *zmin,
                       uint64_t *stats, void *stream) {
     int rc = gauss_init_launch(p, max_level, G, sepsets, zmin, S(stream));
@@ -184,9 +224,9 @@ int gpucsl_gauss_init(int p, int max_level, uint8_t *G, int32_t *sepsets, float 
 }
 
 int gpucsl_gauss_level0(const float *C32, const double *C64, int p, double thr0, float margin,
-                        uint8_t *G, float *zmin, uint64_t *stats_level0, void *stream) {
+                        uint8_t *G, float *zmin, uint64_t *stats_level0, void *ws, size_t ws_bytes, void *stream) {
     return gauss_level0_launch(C32, C64, p, thr0, margin, G, zmin,
-                               reinterpret_cast<unsigned long long *>(stats_level0), S(stream));
+                               reinterpret_cast<unsigned long long *>(stats_level0), ws, ws_bytes, S(stream));
 }
 
 int gpucsl_gauss_level(int level, const float *C32, const double *C64, int p, int max_level,
@@ -202,34 +242,97 @@ int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_le
                           int32_t *sepsets, float *zmin, uint64_t *stats, void *ws, size_t ws_bytes,
                           void *stream) {
     if (!thr || !stats) return set_error(ERR_ARG, "gauss_skeleton: null thr or stats");
+    if (!C32 || !C64) return set_error(ERR_ARG, "gauss_skeleton: null correlation matrix");
     int rc = gpucsl_gauss_init(p, max_level, G, sepsets, zmin, stats, stream);
     if (rc) return rc;
-    rc = gpucsl_gauss_level0(C32, C64, p, thr[0], margin, G, zmin, stats, stream);
+    rc = gauss_skeleton_levels(C32, C64, p, max_level, thr, margin, group_size, G, sepsets, zmin,
+                               reinterpret_cast<unsigned long long *>(stats), ws, ws_bytes, S(stream));
     if (rc) return rc;
+    return check_unbuilt_levels(
+        max_level, stats,
+        [&](int lv, unsigned long long *st) { return compact_launch(G, p, lv, ws, st, S(stream)); }, S(stream));
+}
+
+// ---- batches of Gaussian jobs ----------------------------------------------
+static int check_batch(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj) {
+    if (nj < 1 || !jobs || !jobs_dev) return set_error(ERR_ARG, "batch: empty job table");
+    for (int b = 0; b < nj; ++b)
+        if (jobs[b].p < 1 || jobs[b].max_level < 0 || jobs[b].n < 4)
+            return set_error(ERR_ARG, "batch: job %d has p=%d max_level=%d n=%lld", b, jobs[b].p, jobs[b].max_level,
+                             (long long)jobs[b].n);
+    return OK;
+}
+
+size_t gpucsl_gauss_batch_workspace_bytes(const gpucsl_gauss_job *jobs, int nj) {
+    return nj < 1 || !jobs ? 0 : gauss_batch_ws_bytes(jobs, nj);
+}
+
+int gpucsl_gauss_batch_corr(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, void *stream) {
+    int rc = check_batch(jobs, jobs_dev, nj);
+    if (rc) return rc;
+    int with_x = 0;
+    for (int b = 0; b < nj; ++b) with_x += jobs[b].X != nullptr;
+    if (with_x == nj) return corr_batch_launch(jobs, jobs_dev, nj, S(stream));
+    if (with_x == 0) return c64_to_c32_batch_launch(jobs, jobs_dev, nj, S(stream));
+    return set_error(ERR_ARG, "batch_corr: either every job or no job has X");
+}
+
+int gpucsl_gauss_batch_init(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, void *stream) {
+    int rc = check_batch(jobs, jobs_dev, nj);
+    if (rc) return rc;
+    return gauss_batch_init(jobs, jobs_dev, nj, S(stream));
+}
+
+int gpucsl_gauss_batch_level0(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, float margin,
+                              uint64_t *stats_level0, void *stream) {
+    int rc = check_batch(jobs, jobs_dev, nj);
+    if (rc) return rc;
+    if (!stats_level0) return set_error(ERR_ARG, "batch_level0: null stats");
+    return gauss_batch_level0(jobs, jobs_dev, nj, margin, reinterpret_cast<unsigned long long *>(stats_level0),
+                              S(stream));
+}
+
+int gpucsl_gauss_batch_level(int level, const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj,
+                             float margin, int group_size, uint64_t *stats_level, void *ws, size_t ws_bytes,
+                             void *stream) {
+    int rc = check_batch(jobs, jobs_dev, nj);
+    if (rc) return rc;
+    return gauss_batch_level(level, jobs, jobs_dev, nj, margin, group_size,
+                             reinterpret_cast<unsigned long long *>(stats_level), ws, ws_bytes, S(stream));
+}
+
+int gpucsl_gauss_batch_finish(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, void *stream) {
+    int rc = check_batch(jobs, jobs_dev, nj);
+    if (rc) return rc;
+    return gauss_batch_finish(jobs, jobs_dev, nj, S(stream));
+}
+
+int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, float margin,
+                                int group_size, uint64_t *stats, void *ws, size_t ws_bytes, void *stream) {
+    int rc = check_batch(jobs, jobs_dev, nj);
+    if (rc) return rc;
+    if (!stats || !ws) return set_error(ERR_ARG, "batch_skeleton: null stats or workspace");
+    int max_level = 0;
+    for (int b = 0; b < nj; ++b) max_level = jobs[b].max_level > max_level ? jobs[b].max_level : max_level;
+    launch_fill<uint64_t>(stats, 0ull, (size_t)(max_level + 1) * GPUCSL_STAT_SLOTS, S(stream));
+    if ((rc = gpucsl_gauss_batch_corr(jobs, jobs_dev, nj, stream))) return rc;
+    if ((rc = gauss_batch_init(jobs, jobs_dev, nj, S(stream)))) return rc;
+    if ((rc = gauss_batch_level0(jobs, jobs_dev, nj, margin, reinterpret_cast<unsigned long long *>(stats),
+                                 S(stream))))
+        return rc;
     const int built = max_level < GPUCSL_MAX_LEVEL_TEMPLATE ? max_level : GPUCSL_MAX_LEVEL_TEMPLATE;
     for (int l = 1; l <= built; ++l) {
-        rc = gpucsl_gauss_level(l, C32, C64, p, max_level, thr[l], margin, group_size, G, sepsets, zmin,
-                                stats + (size_t)l * GPUCSL_STAT_SLOTS, ws, ws_bytes, stream);
+        rc = gauss_batch_level(l, jobs, jobs_dev, nj, margin, group_size,
+                               reinterpret_cast<unsigned long long *>(stats) + (size_t)l * GPUCSL_STAT_SLOTS, ws,
+                               ws_bytes, S(stream));
         if (rc) return rc;
     }
-    if (max_level > GPUCSL_MAX_LEVEL_TEMPLATE) {
-        // a level above the built range only matters if some row still has
-        // enough neighbours for it: compact once more and read the max degree.
-        const int next = GPUCSL_MAX_LEVEL_TEMPLATE + 1;
-        unsigned long long *st = reinterpret_cast<unsigned long long *>(stats) + (size_t)next * GPUCSL_STAT_SLOTS;
-        rc = compact_launch(G, p, next, ws, st, S(stream));
-        if (rc) return rc;
-        unsigned long long maxdeg = 0;
-        rc = check_hip(hipMemcpyAsync(&maxdeg, st + 5, sizeof(maxdeg), hipMemcpyDeviceToHost, S(stream)),
-                       "read max degree");
-        if (rc) return rc;
-        rc = check_hip(hipStreamSynchronize(S(stream)), "sync");
-        if (rc) return rc;
-        if ((long long)maxdeg - 1 >= next)
-            return set_error(ERR_UNSUPPORTED, "level %d has work (max degree %llu) but only levels <= %d are built",
-                             next, maxdeg, GPUCSL_MAX_LEVEL_TEMPLATE);
-    }
-    return OK;
+    rc = check_unbuilt_levels(
+        max_level, stats,
+        [&](int lv, unsigned long long *st) { return gauss_batch_compact(jobs, jobs_dev, nj, lv, st, S(stream)); },
+        S(stream));
+    if (rc) return rc;
+    return gauss_batch_finish(jobs, jobs_dev, nj, S(stream));
 }
 
 int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void *stream) {
@@ -241,8 +344,8 @@ int gpucsl_compact(const uint8_t *G, int p, int32_t *Gc, void *ws, size_t ws_byt
     if (ws_bytes < gauss_workspace_bytes(p)) return set_error(ERR_WORKSPACE, "compact: workspace too small");
     int rc = compact_launch(G, p, 1, ws, nullptr, S(stream));
     if (rc) return rc;
-    const int32_t *nbr = (const int32_t *)ws;
-    const int32_t *deg = (const int32_t *)((char *)ws + align_up(sizeof(int32_t) * (size_t)p * p, 256));
+    int32_t *nbr = nullptr, *deg = nullptr;
+    level_list_views(ws, p, &nbr, &deg);
     hipLaunchKernelGGL(gc_layout_kernel, dim3((p + 255) / 256, p), dim3(256), 0, S(stream), nbr, deg, p, Gc);
     return check_hip(hipGetLastError(), "gc layout");
 }
@@ -290,21 +393,10 @@ int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const u
                                stats + (size_t)l * GPUCSL_STAT_SLOTS, ws, ws_bytes, stream);
         if (rc) return rc;
     }
-    if (max_level > GPUCSL_MAX_LEVEL_TEMPLATE) {
-        const int next = GPUCSL_MAX_LEVEL_TEMPLATE + 1;
-        unsigned long long *st = reinterpret_cast<unsigned long long *>(stats) + (size_t)next * GPUCSL_STAT_SLOTS;
-        rc = compact_launch(G, p, next, ws, st, S(stream));
-        if (rc) return rc;
-        unsigned long long maxdeg = 0;
-        rc = check_hip(hipMemcpyAsync(&maxdeg, st + 5, sizeof(maxdeg), hipMemcpyDeviceToHost, S(stream)),
-                       "read max degree");
-        if (rc) return rc;
-        rc = check_hip(hipStreamSynchronize(S(stream)), "sync");
-        if (rc) return rc;
-        if ((long long)maxdeg - 1 >= next)
-            return set_error(ERR_UNSUPPORTED, "level %d has work (max degree %llu) but only levels <= %d are built",
-                             next, maxdeg, GPUCSL_MAX_LEVEL_TEMPLATE);
-    }
+    rc = check_unbuilt_levels(
+        max_level, stats,
+        [&](int lv, unsigned long long *st) { return compact_launch(G, p, lv, ws, st, S(stream)); }, S(stream));
+    if (rc) return rc;
     return lower_minus_one_launch(pmax, p, S(stream));
 }
 

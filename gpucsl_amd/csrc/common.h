@@ -373,7 +373,7 @@ inline void launch_fill(T *dst, T value, size_t count, hipStream_t s) {
     hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, dst, value, count);
 }
 
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+__host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // ----------------------------------------------------------------------------
 // optional per-launch event timing (enabled by gpucsl_profile_begin; never

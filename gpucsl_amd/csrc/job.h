@@ -1,0 +1,130 @@
+// Job tables and workspace layouts shared by the Gaussian kernels.
+#pragma once
+#include "../../include/gpucsl_hip.h"
+#include "common.h"
+
+namespace gpucsl {
+
+using Job = gpucsl_gauss_job;
+
+constexpr int CORR_ROW_SPLITS_MAX = 64;
+constexpr int JOB_DESC_BYTES = 512;  // room for one job descriptor at the head of a workspace
+static_assert(sizeof(Job) <= JOB_DESC_BYTES, "job descriptor too large");
+
+// FP64-recheck queue entry
+struct SlowItem {
+    int w;              // pair index in the phase numbering of the job
+    unsigned int iter;  // first iteration still to evaluate
+    float zl;           // zmin over the tests already counted
+    int job;            // index into the launch's job table
+};
+
+// per-job level workspace
+struct JobWs {
+    int32_t *nbr, *deg, *cntA, *cntB, *rowoffA, *rowoffB, *itemoffA, *itemoffB;
+};
+
+__host__ __device__ inline JobWs carve_job(void *ws, int p) {
+    JobWs w;
+    char *b = (char *)ws;
+    size_t off = 0;
+    w.nbr = (int32_t *)(b + off);
+    off = align_up(off + sizeof(int32_t) * (size_t)p * p, 256);
+    w.deg = (int32_t *)(b + off);
+    w.cntA = w.deg + p;
+    w.cntB = w.deg + 2 * p;
+    off = align_up(off + sizeof(int32_t) * (size_t)p * 3, 256);
+    w.rowoffA = (int32_t *)(b + off);
+    w.rowoffB = w.rowoffA + (p + 1);
+    w.itemoffA = w.rowoffB + (p + 1);
+    w.itemoffB = w.itemoffA + (p + 1);
+    return w;
+}
+
+inline size_t job_ws_bytes(int p) {
+    size_t off = 0;
+    off = align_up(off + sizeof(int32_t) * (size_t)p * p, 256);
+    off = align_up(off + sizeof(int32_t) * (size_t)p * 3, 256);
+    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 4, 256);
+    return off;
+}
+
+// per-job correlation workspace
+struct CorrWs {
+    double *partial, *mean, *sd, *gram;
+};
+
+__host__ __device__ inline CorrWs carve_corr(void *ws, int p) {
+    CorrWs c;
+    char *b = (char *)ws;
+    size_t off = 0;
+    c.partial = (double *)(b + off);
+    off = align_up(off + sizeof(double) * (size_t)CORR_ROW_SPLITS_MAX * p, 256);
+    c.mean = (double *)(b + off);
+    off = align_up(off + sizeof(double) * (size_t)p, 256);
+    c.sd = (double *)(b + off);
+    off = align_up(off + sizeof(double) * (size_t)p, 256);
+    c.gram = (double *)(b + off);
+    return c;
+}
+
+inline size_t corr_job_ws_bytes(int p, int k_splits) {
+    size_t off = 0;
+    off = align_up(off + sizeof(double) * (size_t)CORR_ROW_SPLITS_MAX * p, 256);
+    off = align_up(off + sizeof(double) * (size_t)p, 256);
+    off = align_up(off + sizeof(double) * (size_t)p, 256);
+    off = align_up(off + sizeof(double) * (size_t)k_splits * p * p, 256);
+    return off;
+}
+
+// per-launch (batch) workspace
+constexpr int NUM_COUNTERS = 64;
+constexpr int CNT_QUEUE = 32;
+constexpr int STRIPES = 8;
+constexpr int BSTAT_WORDS = 8;
+constexpr int BSTAT_STRIPES = 64;
+
+struct BatchWs {
+    unsigned int *counters;        // NUM_COUNTERS
+    unsigned long long *bstats;    // BSTAT_STRIPES * BSTAT_WORDS
+    int32_t *itembaseA, *itembaseB;  // nj + 1 each
+    SlowItem *queueA, *queueB;     // qcap each
+};
+
+inline size_t batch_ws_bytes(int nj, size_t qcap) {
+    size_t off = 0;
+    off = align_up(off + sizeof(unsigned int) * NUM_COUNTERS, 256);
+    off = align_up(off + sizeof(unsigned long long) * BSTAT_STRIPES * BSTAT_WORDS, 256);
+    off = align_up(off + sizeof(int32_t) * (size_t)(nj + 1) * 2, 256);
+    off = align_up(off + sizeof(SlowItem) * qcap * 2, 256);
+    return off;
+}
+
+inline BatchWs carve_batch(void *ws, int nj, size_t qcap) {
+    BatchWs w;
+    char *b = (char *)ws;
+    size_t off = 0;
+    w.counters = (unsigned int *)(b + off);
+    off = align_up(off + sizeof(unsigned int) * NUM_COUNTERS, 256);
+    w.bstats = (unsigned long long *)(b + off);
+    off = align_up(off + sizeof(unsigned long long) * BSTAT_STRIPES * BSTAT_WORDS, 256);
+    w.itembaseA = (int32_t *)(b + off);
+    w.itembaseB = w.itembaseA + (nj + 1);
+    off = align_up(off + sizeof(int32_t) * (size_t)(nj + 1) * 2, 256);
+    w.queueA = (SlowItem *)(b + off);
+    w.queueB = w.queueA + qcap;
+    return w;
+}
+
+// A phase has at most one queue entry per ordered pair of its side.
+inline size_t queue_capacity(const Job *jobs_host, int nj) {
+    size_t q = 1;
+    for (int b = 0; b < nj; ++b) q += (size_t)jobs_host[b].p * (jobs_host[b].p - 1) / 2;
+    return q;
+}
+
+// copies one job descriptor into device memory from a kernel argument, so a
+// single-dataset call stays stream-ordered and graph-capturable
+__global__ void write_job_kernel(Job *dst, Job src);
+
+}  // namespace gpucsl

@@ -122,7 +122,7 @@ class GaussianEngine:
                  self.zmin.data_ptr(), self.stats.data_ptr(), s)
         nat.call("gpucsl_gauss_level0", self.C32.data_ptr(), self.C64.data_ptr(), self.p,
                  float(self.thr[0]), self.margin, self.G.data_ptr(), self.zmin.data_ptr(),
-                 self.stats.data_ptr(), s)
+                 self.stats.data_ptr(), self.ws.data_ptr(), self.ws_bytes, s)
         if on_level:
             on_level(0)
         for l in range(1, self.max_level + 1):

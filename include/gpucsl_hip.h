@@ -67,6 +67,41 @@ int gpucsl_corr_c64_to_c32(const double *C64, int p, float *C32, void *stream);
 /* ---------------------------------------------------------------------------
  * Gaussian skeleton
  * ------------------------------------------------------------------------- */
+#define GPUCSL_MAX_LEVELS 9   /* thresholds for levels 0..8 */
+
+/* One Gaussian skeleton job (one dataset).  Every pointer is device memory
+ * owned by the caller: ws is a per-job level workspace of
+ * gpucsl_gauss_job_workspace_bytes(p) bytes and corr_ws a correlation
+ * workspace of gpucsl_corr_job_workspace_bytes(p, k_splits) bytes.  X (n x p
+ * row-major, x_dtype 0 = float32, 1 = float64) is only read by the correlation
+ * step and may be NULL when C64 is supplied.  The single-dataset entry points
+ * below build one of these internally; the batched entry points take an array
+ * of them, the same array in host memory and in device memory. */
+typedef struct gpucsl_gauss_job {
+    int32_t p;
+    int32_t max_level;
+    int64_t n;
+    int32_t x_dtype;
+    int32_t k_splits;   /* split-K factor of the correlation GEMM for this job */
+    const void *X;
+    float *C32;
+    double *C64;
+    uint8_t *G;
+    int32_t *sepsets;
+    float *zmin;
+    float *pmax;
+    uint16_t *skeleton; /* may be NULL; filled by the finish step */
+    void *ws;
+    void *corr_ws;
+    double thr[GPUCSL_MAX_LEVELS];
+} gpucsl_gauss_job;
+
+/* Per-job workspaces of the batched entry points, and the split-K factor the
+ * correlation GEMM uses for an n x p dataset. */
+size_t gpucsl_gauss_job_workspace_bytes(int p);
+int gpucsl_corr_k_splits(int64_t n, int p);
+size_t gpucsl_corr_job_workspace_bytes(int p, int k_splits);
+/* Workspace of the single-dataset entry points (level lists, work queues). */
 size_t gpucsl_gauss_workspace_bytes(int p, int max_level);
 
 /* G = 1, sepsets = -1, zmin = +inf, stats = 0.  Replaces the buffer set-up of
@@ -78,7 +113,8 @@ int gpucsl_gauss_init(int p, int max_level, uint8_t *G, int32_t *sepsets, float 
  * cleared.  Replaces gaussian_ci_level_0 (gpucsl/pc/cuda/gaussian_ci.cu:18-51).
  * margin scales the FP32 error estimate inside which the FP64 recheck runs. */
 int gpucsl_gauss_level0(const float *C32, const double *C64, int p, double thr0, float margin,
-                        uint8_t *G, float *zmin, uint64_t *stats_level0, void *stream);
+                        uint8_t *G, float *zmin, uint64_t *stats_level0, void *ws, size_t ws_bytes,
+                        void *stream);
 
 /* Level l >= 1: compaction, then every CI test with |S| = l in pcalg order.
  * Replaces one iteration of gaussian_ci_worker
@@ -97,6 +133,35 @@ int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_le
                           const double *thr, float margin, int group_size, uint8_t *G,
                           int32_t *sepsets, float *zmin, uint64_t *stats, void *ws, size_t ws_bytes,
                           void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Batches of Gaussian jobs.  Every call takes the job table twice: `jobs` in
+ * host memory (sizes and validation) and `jobs_dev`, the same bytes in device
+ * memory (read by the kernels).  All jobs of a batch advance level by level in
+ * shared launches: one launch per kernel and level for the whole batch.  A job
+ * whose max_level is below the level of a launch takes no part in it.  The
+ * launch workspace `ws` holds the cross-job work queues
+ * (gpucsl_gauss_batch_workspace_bytes).  Results of a job do not depend on the
+ * other jobs of its batch.  This replaces running gaussian_ci_worker
+ * (gpucsl/pc/discover_skeleton_gaussian.py:30-113) once per dataset.
+ * ------------------------------------------------------------------------- */
+size_t gpucsl_gauss_batch_workspace_bytes(const gpucsl_gauss_job *jobs, int nj);
+/* Correlation of every job: from X when every job has X, else C32 = (float)C64. */
+int gpucsl_gauss_batch_corr(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, void *stream);
+/* G = 1, sepsets = -1, zmin = +inf for every job. */
+int gpucsl_gauss_batch_init(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, void *stream);
+/* Level 0 of every job with thr[0]; stats summed over the jobs. */
+int gpucsl_gauss_batch_level0(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, float margin,
+                              uint64_t *stats_level0, void *stream);
+/* Level l >= 1 of every job with max_level >= l, thresholds thr[l]. */
+int gpucsl_gauss_batch_level(int level, const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj,
+                             float margin, int group_size, uint64_t *stats_level, void *ws, size_t ws_bytes,
+                             void *stream);
+/* pmax from zmin (when pmax is set) and the uint16 skeleton (when set). */
+int gpucsl_gauss_batch_finish(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, void *stream);
+/* corr + init + levels 0..max(max_level) + finish; stats: (max level + 1) x 8. */
+int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, float margin,
+                                int group_size, uint64_t *stats, void *ws, size_t ws_bytes, void *stream);
 
 /* pmax = 2 (1 - Phi(sqrt(n-3) |zmin|)), -1 on and below the diagonal.
  * Replaces transform_to_pmax_cupy + postprocess_pmax_cupy

@@ -1,0 +1,165 @@
+"""Batched Gaussian skeletons (many datasets in shared launches) on the GPU:
+every job of a batch equals the same job run alone through the single-dataset
+path (bit for bit) and the FP64 oracle (skeleton and separation sets); a
+captured graph replays to the eager result.  GPU box only."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gpucsl_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _jobs(specs):
+    from gpucsl_amd.batch import GaussJobState
+
+    jobs, data = [], []
+    for (p, n, L, alpha, seed) in specs:
+        X = syn.gaussian_sem(p, n, 3.0, seed=seed).data.astype(np.float32)
+        j = GaussJobState(n, p, L, alpha=alpha)
+        j.set_data(X)
+        jobs.append(j)
+        data.append(X)
+    return jobs, data
+
+
+def _plain_results(specs):
+    from gpucsl_amd.batch import run_plain
+
+    jobs, _ = _jobs(specs)
+    run_plain(jobs)
+    torch.cuda.synchronize()
+    return [j.results() for j in jobs]
+
+
+def _same(a, b, label):
+    ok = ((a.skeleton == b.skeleton).all() and (a.separation_sets == b.separation_sets).all()
+          and np.array_equal(a.zmin.view(np.uint32), b.zmin.view(np.uint32))
+          and np.array_equal(a.pmax.view(np.uint32), b.pmax.view(np.uint32)))
+    if not ok:
+        say(f"[{label}] differs: skel {(a.skeleton != b.skeleton).sum()} sep "
+            f"{(a.separation_sets != b.separation_sets).sum()}")
+    return ok
+
+
+SPECS = [(20, 300, 3, 0.01, 1), (57, 2000, 3, 0.01, 2), (80, 1500, 2, 0.05, 3), (33, 5000, 0, 0.01, 4),
+         (120, 3000, 3, 0.01, 5), (64, 999, 1, 0.01, 6), (200, 5000, 3, 0.01, 7), (45, 700, 3, 0.05, 8)]
+
+
+def test_batch_equals_alone_and_oracle():
+    from gpucsl_amd.batch import GaussianBatch
+
+    say(f"batch of {len(SPECS)} jobs: {SPECS}")
+    jobs, data = _jobs(SPECS)
+    b = GaussianBatch(jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    got = b.results()
+    alone = _plain_results(SPECS)
+    refs = [oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), n, alpha, L)
+            for (p, n, L, alpha, seed), X in zip(SPECS, data)]
+    for k, ((p, n, L, alpha, seed), ref) in enumerate(zip(SPECS, refs)):
+        assert _same(got[k], alone[k], f"job {k} batch vs alone")
+        assert (got[k].skeleton == ref["skeleton"]).all(), f"job {k} skeleton vs oracle"
+        assert (got[k].separation_sets == ref["sepsets"]).all(), f"job {k} sepsets vs oracle"
+        iu = np.triu_indices(p, 1)
+        dp = np.abs(got[k].pmax[iu].astype(np.float64) - ref["pmax"][iu]).max()
+        say(f"  job {k} p={p} n={n} L={L}: edges={int(got[k].skeleton.sum()) // 2} max|dpmax|={dp:.3g}")
+        assert dp <= 2e-5
+    st = b.batch_stats()
+    oracle_exec = [sum(int(ref["executed"][l]) if l <= L else 0 for (p, n, L, a, s), ref in zip(SPECS, refs))
+                   for l in range(4)]
+    say("  batch useful per level", [d["useful"] for d in st.levels], "oracle", oracle_exec)
+    assert [d["useful"] for d in st.levels] == oracle_exec
+
+
+def test_batch_membership_does_not_change_results():
+    """A job's result is the same bits in two different batches."""
+    from gpucsl_amd.batch import GaussianBatch
+
+    a_specs = SPECS[:3] + [(150, 4000, 3, 0.01, 11)]
+    b_specs = [(150, 4000, 3, 0.01, 11)] + SPECS[5:]
+    ja, _ = _jobs(a_specs)
+    jb, _ = _jobs(b_specs)
+    ba, bb = GaussianBatch(ja), GaussianBatch(jb)
+    ba.launch()
+    bb.launch()
+    torch.cuda.synchronize()
+    assert _same(ba.results()[3], bb.results()[0], "job in two batches")
+
+
+def test_graph_replay_matches_eager():
+    from gpucsl_amd.batch import GaussianBatch
+
+    jobs, _ = _jobs(SPECS)
+    b = GaussianBatch(jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    want = b.results()
+    want_stats = b.batch_stats().levels
+    b.capture()
+    for rep in range(3):
+        # eager work between replays
+        junk = torch.randn(1000, device="cuda").sum()
+        b.run()
+        torch.cuda.synchronize()
+        got = b.results()
+        for k in range(len(SPECS)):
+            assert _same(got[k], want[k], f"replay {rep} job {k}")
+        assert b.batch_stats().levels == want_stats
+        del junk
+
+
+def test_bootstrap_batch_c4_slice():
+    """Bootstrap resamples of the p=200, n=5000 dataset (seed 300, rows drawn
+    with seed 300+b) plus uneven datasets, as in the batched workload."""
+    from gpucsl_amd.batch import GaussJobState, GaussianBatch, run_plain
+
+    base = syn.gaussian_sem(200, 5000, 3.0, seed=300).data
+    jobs, plain, xs = [], [], []
+    for b in range(6):
+        X = syn.bootstrap(base, 300 + b).astype(np.float32)
+        xs.append((X, 5000))
+    rng = np.random.Generator(np.random.PCG64(4000))
+    for k in range(3):
+        p, n = int(rng.integers(50, 501)), int(rng.integers(1000, 20001))
+        xs.append((syn.gaussian_sem(p, n, 3.0, seed=400 + k).data.astype(np.float32), n))
+    for X, n in xs:
+        for lst in (jobs, plain):
+            j = GaussJobState(n, X.shape[1], 3, alpha=0.01)
+            j.set_data(X)
+            lst.append(j)
+    bt = GaussianBatch(jobs)
+    bt.launch()
+    run_plain(plain)
+    torch.cuda.synchronize()
+    for k, (a, c) in enumerate(zip(bt.results(), [j.results() for j in plain])):
+        assert _same(a, c, f"c4 job {k}")
+    for k in (0, 7):
+        X, n = xs[k]
+        ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), n, 0.01, 3)
+        r = bt.results()[k]
+        assert (r.skeleton == ref["skeleton"]).all() and (r.separation_sets == ref["sepsets"]).all()
+        say(f"  c4 job {k} p={X.shape[1]} n={n} matches oracle; edges={int(r.skeleton.sum()) // 2}")
+
+
+def test_mixed_inputs_rejected():
+    from gpucsl_amd.batch import GaussJobState, GaussianBatch
+    from gpucsl_amd.cli.cli_util import GpucslError
+
+    a = GaussJobState(500, 10, 1, alpha=0.01)
+    a.set_data(np.random.default_rng(0).standard_normal((500, 10)).astype(np.float32))
+    c = GaussJobState(500, 10, 1, alpha=0.01)
+    c.set_correlation(np.eye(10))
+    with pytest.raises(GpucslError):
+        GaussianBatch([a, c]).launch()

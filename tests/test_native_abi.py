@@ -49,3 +49,20 @@ def test_product_does_not_import_oracle():
     for f in (ROOT / "gpucsl_amd").rglob("*.py"):
         src = f.read_text()
         assert "import oracle" not in src and "from oracle" not in src, f
+
+
+def test_job_struct_layout_matches_header():
+    """The ctypes mirror of gpucsl_gauss_job has the C layout: 4+4+8+4+4 bytes
+    of scalars, ten pointers, nine doubles (x86-64 / gfx950 alignment)."""
+    import ctypes as ct
+
+    from gpucsl_amd.batch import GaussJob, MAX_LEVELS
+
+    assert MAX_LEVELS == 9
+    assert GaussJob.n.offset == 8 and GaussJob.X.offset == 24 and GaussJob.corr_ws.offset == 96
+    assert GaussJob.thr.offset == 104 and ct.sizeof(GaussJob) == 176
+    src = (ROOT / "include" / "gpucsl_hip.h").read_text()
+    body = src[src.index("typedef struct gpucsl_gauss_job"):src.index("} gpucsl_gauss_job;")]
+    names = [ln.split(";")[0].replace("*", " ").split()[-1].split("[")[0]
+             for ln in body.splitlines()[1:] if ";" in ln]
+    assert names == [f[0] for f in GaussJob._fields_]
