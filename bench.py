@@ -30,6 +30,7 @@ WORKLOADS = {
     "gaussian_p50_n2000": (50, 2000, 3.0, 0, 0.01, 3, np.float64),
 }
 BATCH_WORKLOADS = ("batched_bootstrap256_p200_n5000_plus64",)
+SERVING_WORKLOADS = ("serving_poisson_mix",)
 
 
 def log(*a):
@@ -232,13 +233,187 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def serving_requests(count: int, seed: int = 500):
+    """The request mix: 70% Gaussian (p ~ U[20, 2000], n ~ U[500, 50000],
+    degree 3) and 30% discrete (p ~ U[20, 300], cardinalities 2..5,
+    n ~ U[1000, 100000], degree 2), max level 3, alpha 0.01; request k's data
+    uses seed seed+1+k.  Data is generated on the device."""
+    from gpucsl_amd import synthetic as syn
+    from gpucsl_amd.serving import Request
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    reqs = []
+    for k in range(count):
+        if rng.random() < 0.7:
+            p, n = int(rng.integers(20, 2001)), int(rng.integers(500, 50001))
+            X = syn.gaussian_sem_device(p, n, 3.0, seed=seed + 1 + k)
+            reqs.append(Request("gaussian", n, p, 3, 0.01, X=X))
+        else:
+            p, n = int(rng.integers(20, 301)), int(rng.integers(1000, 100001))
+            codes, dims = syn.discrete_bn_device(p, n, 2.0, seed=seed + 1 + k)
+            reqs.append(Request("discrete", n, p, 3, 0.01, codes=codes, dims=dims))
+    return reqs
+
+
+def bench_serving(args):
+    """Closed loop (every request at t=0) for capacity, the same requests one
+    by one through the single-dataset path, then an open-loop Poisson stream at
+    --load x capacity for p50/p99 latency."""
+    import torch
+
+    from gpucsl_amd import _native as nat
+    from gpucsl_amd.serving import SkeletonServer, run_alone
+
+    torch.cuda.set_device(0)
+    log(f"[bench] workload {args.workload}: generating {args.requests} requests on the device")
+    t0 = time.perf_counter()
+    reqs = serving_requests(args.requests)
+    warm = serving_requests(6, seed=900)
+    torch.cuda.synchronize()
+    log(f"[bench] generated in {time.perf_counter() - t0:.1f}s; "
+        f"{sum(r.kind == 'gaussian' for r in reqs)} Gaussian, {sum(r.kind == 'discrete' for r in reqs)} discrete")
+
+    def serve(requests, arrivals=None, copy_results=False):
+        """Submit requests at their arrival times (all at once when arrivals is
+        None) and tick until every result is on the host.  Zero-copy results
+        are released as soon as they arrive (the caller has consumed them)."""
+        srv = SkeletonServer(host_budget=4 << 30, copy_results=copy_results)
+        srv.host_tick_s = 0.0
+        start = time.perf_counter()
+        done, i = [], 0
+        while len(done) < len(requests):
+            now = time.perf_counter() - start
+            while i < len(requests) and (arrivals is None or arrivals[i] <= now):
+                srv.submit(requests[i], arrival=start + (arrivals[i] if arrivals is not None else 0.0))
+                i += 1
+            if srv.idle and i < len(requests):
+                time.sleep(max(0.0, min(arrivals[i] - now, 0.002)))
+                continue
+            th = time.perf_counter()
+            got = srv.tick()
+            srv.host_tick_s += time.perf_counter() - th
+            for c in got:
+                if not copy_results:
+                    c.release()
+            done += got
+            if not got and not srv.launched:
+                nxt = arrivals[i] - (time.perf_counter() - start) if arrivals is not None and i < len(requests) \
+                    else 0.05
+                srv.wait(timeout=max(0.0, min(nxt, 0.05)))
+        torch.cuda.synchronize()
+        return srv, done, time.perf_counter() - start
+
+    log("[bench] warmup")
+    serve(warm)
+    serve(warm, copy_results=True)
+    for r in warm:
+        run_alone(r).results()
+    torch.cuda.synchronize()
+
+    log("[bench] plain path: each request alone, one after another (results left on the device)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    states = [run_alone(r) for r in reqs]
+    torch.cuda.synchronize()
+    plain_gpu_sec = time.perf_counter() - t0
+    alone_results = [st.results() for st in states]
+    del states
+    log("[bench] plain path with results copied to the host (the drop-in API's return)")
+    t0 = time.perf_counter()
+    for r in reqs:
+        run_alone(r).results()
+    torch.cuda.synchronize()
+    plain_host_sec = time.perf_counter() - t0
+
+    log("[bench] served, closed loop (all requests at t=0), results into pinned memory")
+    srv, done, makespan = serve(reqs)
+    useful = srv.useful_tests()
+    capacity = len(reqs) / makespan
+    log(f"[bench] closed loop: {makespan:.3f}s, {capacity:.2f} requests/s, {useful / makespan:.3g} tests/s, "
+        f"{srv.ticks} ticks, largest batch {srv.max_batch}")
+    log("[bench] served, closed loop, results copied to owned host arrays; checked against the plain path")
+    srv_c, done_c, makespan_c = serve(reqs, copy_results=True)
+    served = {c.rid: c for c in done_c}
+    identical = 0
+    for r, a in zip(reqs, alone_results):
+        c = served[r.rid].result
+        same = (a.skeleton == c.skeleton).all() and (a.separation_sets == c.separation_sets).all() and \
+            np.array_equal(a.pmax.view(np.uint32), c.pmax.view(np.uint32))
+        identical += bool(same)
+    del alone_results, served, done_c
+
+    rate = args.load * capacity
+    rng = np.random.Generator(np.random.PCG64(501))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+    log(f"[bench] open loop: Poisson arrivals at {rate:.2f} requests/s")
+    srv2, done2, span2 = serve(reqs, arrivals)
+    lat = np.array([c.latency for c in done2])
+    lat_g = np.array([c.latency for c in done2 if c.kind == "gaussian"])
+    lat_d = np.array([c.latency for c in done2 if c.kind == "discrete"])
+    useful2 = srv2.useful_tests()
+
+    with nat.KernelTimer() as kt:
+        for r in reqs:
+            run_alone(r)
+        torch.cuda.synchronize()
+    timings = kt.timings
+    dominant = max(timings, key=lambda k: timings[k][1])
+
+    out = {
+        "metric": METRIC,
+        "value": useful / makespan,
+        "unit": "CI tests/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": makespan * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {len(reqs)} requests, 70% Gaussian p~U[20,2000] "
+                               f"n~U[500,50000] degree 3, 30% discrete p~U[20,300] card 2-5 n~U[1000,100000] "
+                               f"degree 2, max_level 3, alpha 0.01, seed 500; HBM-footprint admission, "
+                               f"continuous batching by level",
+                   "parallelism": "single GPU"},
+        "closed_loop": {"makespan_s": makespan, "requests_per_s": capacity, "ci_tests_per_s": useful / makespan,
+                        "ticks": srv.ticks, "largest_gaussian_batch": srv.max_batch,
+                        "host_seconds_in_tick": srv.host_tick_s, "host_seconds_by_phase": srv.host_time,
+                        "results": "pinned host memory (zero-copy), released on arrival",
+                        "makespan_with_owned_host_copies_s": makespan_c},
+        "plain_path": {"seconds_results_on_device": plain_gpu_sec,
+                       "seconds_results_on_host": plain_host_sec,
+                       "ci_tests_per_s": useful / plain_gpu_sec,
+                       "served_over_plain": plain_gpu_sec / makespan,
+                       "served_over_plain_with_host_results": plain_host_sec / makespan_c,
+                       "results_identical": f"{identical}/{len(reqs)}"},
+        "open_loop": {"arrival_rate_per_s": rate, "load": args.load, "span_s": span2,
+                      "throughput_requests_per_s": len(reqs) / span2, "ci_tests_per_s": useful2 / span2,
+                      "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
+                      "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
+                      "gaussian_p50_ms": float(np.percentile(lat_g, 50) * 1e3) if lat_g.size else None,
+                      "gaussian_p99_ms": float(np.percentile(lat_g, 99) * 1e3) if lat_g.size else None,
+                      "discrete_p50_ms": float(np.percentile(lat_d, 50) * 1e3) if lat_d.size else None,
+                      "discrete_p99_ms": float(np.percentile(lat_d, 99) * 1e3) if lat_d.size else None},
+        "roofline": {"kernel": dominant, "bound": None, "achieved": None, "peak": None, "unit": None,
+                     "frac": None, "traffic": None,
+                     "note": "mixed workload; per-kernel rooflines are in the single-dataset lines"},
+        "cpu_baseline": None,
+        "plain_path_kernel_ms": {k: v[1] for k, v in sorted(timings.items(), key=lambda x: -x[1][1])[:14]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gaussian_p1000_n10000",
-                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS))
+                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVING_WORKLOADS))
+    ap.add_argument("--requests", type=int, default=96, help="serving workload: requests in the stream")
+    ap.add_argument("--load", type=float, default=0.8, help="serving workload: arrival rate / capacity")
     ap.add_argument("--bootstraps", type=int, default=256, help="batched workload: bootstrap resamples")
     ap.add_argument("--datasets", type=int, default=64, help="batched workload: uneven datasets")
     ap.add_argument("--profile-steps", type=int, default=5)
@@ -253,6 +428,8 @@ def main():
 
     if args.workload in BATCH_WORKLOADS:
         return bench_batched(args)
+    if args.workload in SERVING_WORKLOADS:
+        return bench_serving(args)
 
     import torch
 

@@ -44,6 +44,8 @@ SIGNATURES = {
     "gpucsl_gauss_batch_skeleton": (_I, [_P, _P, _I, _F, _I, _P, _P, _SZ, _P]),
     "gpucsl_gauss_level": (_I, [_I, _P, _P, _I, _I, _D, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_gauss_skeleton": (_I, [_P, _P, _I, _I, _P, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gpucsl_set_level_tuning": (_I, [_I, _P]),
+    "gpucsl_get_level_tuning": (_I, [_I, _P]),
     "gpucsl_pmax_from_zmin": (_I, [_P, _I, _I64, _P, _P]),
     "gpucsl_compact": (_I, [_P, _I, _P, _P, _SZ, _P]),
     "gpucsl_skeleton_to_u16": (_I, [_P, _I, _P, _P]),
@@ -52,6 +54,7 @@ SIGNATURES = {
     "gpucsl_disc_init": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_disc_level0": (_I, [_P, _I64, _I64, _P, _I, _I, _I, _D, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_disc_level": (_I, [_I, _P, _I64, _I64, _P, _I, _I, _I, _D, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gpucsl_disc_finish": (_I, [_P, _I, _P, _P, _P]),
     "gpucsl_disc_skeleton": (_I, [_P, _I64, _I64, _P, _I, _I, _I, _D, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_disc_ci_test": (_I, [_P, _I64, _I64, _P, _I, _I, _P, _I, _I, _P, _P, _P]),
     "gpucsl_profile_begin": (_I, []),
@@ -78,7 +81,21 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
+        _apply_tuning_table()
     return _lib
+
+
+TUNING_TABLE = Path(__file__).resolve().parent / "tuning.json"
+
+
+def _apply_tuning_table() -> None:
+    """Per-level launch shapes measured by gpucsl_amd.tune, when present."""
+    if TUNING_TABLE.exists():
+        import json
+
+        from gpucsl_amd.tune import apply_table
+
+        apply_table(json.loads(TUNING_TABLE.read_text()))
 
 
 def last_error() -> str:

@@ -92,9 +92,12 @@ int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long 
 void level_list_views(void *ws, int p, int32_t **nbr, int32_t **deg);
 int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s);
 size_t gauss_batch_ws_bytes(const Job *jobs_host, int nj);
+int set_level_tuning(int level, const gpucsl_level_tuning *t);
+int get_level_tuning(int level, gpucsl_level_tuning *t);
 int gauss_batch_init(const Job *jh, const Job *jd, int nj, hipStream_t s);
 int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, unsigned long long *st, hipStream_t s);
-int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, unsigned long long *st, hipStream_t s);
+int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
+                        hipStream_t s);
 int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float margin_k, int group_size,
                       unsigned long long *st, void *bws, size_t bws_bytes, hipStream_t s);
 int gauss_batch_finish(const Job *jh, const Job *jd, int nj, hipStream_t s);
@@ -329,11 +332,15 @@ int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss
     }
     rc = check_unbuilt_levels(
         max_level, stats,
-        [&](int lv, unsigned long long *st) { return gauss_batch_compact(jobs, jobs_dev, nj, lv, st, S(stream)); },
+        [&](int lv, unsigned long long *st) { return gauss_batch_compact(jobs, jobs_dev, nj, lv, group_size, st, S(stream)); },
         S(stream));
     if (rc) return rc;
     return gauss_batch_finish(jobs, jobs_dev, nj, S(stream));
 }
+
+int gpucsl_set_level_tuning(int level, const gpucsl_level_tuning *tuning) { return set_level_tuning(level, tuning); }
+
+int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning) { return get_level_tuning(level, tuning); }
 
 int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void *stream) {
     return pmax_launch(zmin, p, n, pmax, S(stream));
@@ -398,6 +405,13 @@ int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const u
         [&](int lv, unsigned long long *st) { return compact_launch(G, p, lv, ws, st, S(stream)); }, S(stream));
     if (rc) return rc;
     return lower_minus_one_launch(pmax, p, S(stream));
+}
+
+int gpucsl_disc_finish(const uint8_t *G, int p, float *pmax, uint16_t *skeleton, void *stream) {
+    if (!G || !pmax || p < 1) return set_error(ERR_ARG, "disc_finish: bad arguments");
+    int rc = lower_minus_one_launch(pmax, p, S(stream));
+    if (rc) return rc;
+    return skeleton ? to_u16_launch(G, p, skeleton, S(stream)) : OK;
 }
 
 int gpucsl_disc_ci_test(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int vi, int vj,

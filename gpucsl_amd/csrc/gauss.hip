@@ -30,12 +30,62 @@ namespace gpucsl {
 
 constexpr float EPS32 = 5.9604645e-8f;  // 2^-24
 constexpr int PHASE_BLOCKS_PER_CU = 8;
-constexpr int MAX_PHASE_BLOCKS = 4096;
-// row-blocked FAST kernel: one wave per block, a fixed LDS budget, and work
-// items of up to PAIRS_PER_ITEM ordered pairs of one row
-constexpr int RB_BLOCKS_PER_CU = 16;
-constexpr int RB_LDS_BYTES = 16384;
-constexpr int PAIRS_PER_ITEM = 4;
+// FAST kernel: 256-thread blocks with an LDS staging area of at most 64 KB
+constexpr int FB_MAX_LDS_BYTES = 65536;
+
+// Lanes that test one pair together, from the pair's subset count: gs_min
+// lanes (gs_min in {8, 16, 32, 64}) while that takes at most small_iters
+// iterations, a 64-lane wave while that takes at most wave_iters, else the
+// whole 256-thread block.
+struct Team {
+    int gs_min, small_max, wave_max;  // count limits of the gs_min and wave teams
+};
+
+__host__ __device__ __forceinline__ int team_lanes(uint64_t count, const Team &t) {
+    if (count <= (uint64_t)t.small_max) return t.gs_min;
+    if (count <= (uint64_t)t.wave_max) return 64;
+    return 256;
+}
+
+// per-level launch shapes of the FAST kernel (settable at run time, e.g. from
+// an autotuning table); index 0 is unused
+static gpucsl_level_tuning g_tuning[GPUCSL_MAX_LEVEL_TEMPLATE + 1];
+static bool g_tuning_init = false;
+
+static void tuning_defaults() {
+    if (g_tuning_init) return;
+    for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = gpucsl_level_tuning{8, 8, 4, 32768};
+    g_tuning_init = true;
+}
+
+static Team team_for(int level, int gs_min) {
+    tuning_defaults();
+    const gpucsl_level_tuning &t = g_tuning[level];
+    return Team{gs_min, gs_min * t.small_team_iters, 64 * t.wave_team_iters};
+}
+
+int set_level_tuning(int level, const gpucsl_level_tuning *t) {
+    tuning_defaults();
+    if (!t) return set_error(ERR_ARG, "tuning: null");
+    if (t->small_team_iters < 1 || t->wave_team_iters < 1 || t->fast_blocks_per_cu < 1 || t->fast_blocks_per_cu > 32 ||
+        t->fast_lds_bytes < 4096 || t->fast_lds_bytes > FB_MAX_LDS_BYTES || t->fast_lds_bytes % 1024)
+        return set_error(ERR_ARG, "tuning: a field is out of range");
+    if (level == 0) {
+        for (int l = 1; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = *t;
+    } else if (level >= 1 && level <= GPUCSL_MAX_LEVEL_TEMPLATE) {
+        g_tuning[level] = *t;
+    } else {
+        return set_error(ERR_ARG, "tuning: level %d outside 0..%d", level, GPUCSL_MAX_LEVEL_TEMPLATE);
+    }
+    return OK;
+}
+
+int get_level_tuning(int level, gpucsl_level_tuning *t) {
+    tuning_defaults();
+    if (!t || level < 1 || level > GPUCSL_MAX_LEVEL_TEMPLATE) return set_error(ERR_ARG, "tuning: bad level or null");
+    *t = g_tuning[level];
+    return OK;
+}
 
 enum BStat { BS_EXEC = 0, BS_USEFUL = 1, BS_RECHECK = 2, BS_FLIP = 3, BS_MAXRATIO = 4 };
 enum StatSlot { ST_EXEC = 0, ST_TOTAL = 1, ST_RECHECK = 2, ST_FLIP = 3, ST_MAXRATIO = 4, ST_MAXDEG = 5, ST_USEFUL = 6, ST_SLOTS = 8 };
@@ -167,7 +217,7 @@ __global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jo
 // Per job (one block each): exclusive scans of cntA/cntB into rowoffA/rowoffB
 // and of their work items into itemoffA/itemoffB; adds T_l and max degree to
 // the launch statistics.  A job whose max level is below `level` gets no work.
-__global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs, int level,
+__global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs, int level, Team team,
                                                     unsigned long long *__restrict__ st) {
     const Job &J = jobs[blockIdx.x];
     const int p = J.p;
@@ -187,12 +237,16 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         const int ca = active ? w.cntA[r] : 0, cb = active ? w.cntB[r] : 0;
         la += ca;
         lb += cb;
-        lia += (ca + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
-        lib += (cb + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
         const int d = w.deg[r];
+        const uint64_t count = active ? binom(d - 1, level) : 0;
+        if (count) {
+            const int ppi = 256 / team_lanes(count, team);
+            lia += (ca + ppi - 1) / ppi;
+            lib += (cb + ppi - 1) / ppi;
+        }
         if (active) {
             md = d > md ? d : md;
-            tt += (unsigned long long)d * binom(d - 1, level);
+            tt += (unsigned long long)d * count;
         }
     }
     sA[t] = la;
@@ -226,8 +280,12 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         w.itemoffB[r] = eib;
         ea += ca;
         eb += cb;
-        eia += (ca + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
-        eib += (cb + PAIRS_PER_ITEM - 1) / PAIRS_PER_ITEM;
+        const uint64_t count = active ? binom(w.deg[r] - 1, level) : 0;
+        if (count) {
+            const int ppi = 256 / team_lanes(count, team);
+            eia += (ca + ppi - 1) / ppi;
+            eib += (cb + ppi - 1) / ppi;
+        }
     }
     if (t == 1023) {
         w.rowoffA[p] = sA[1023];
@@ -293,6 +351,28 @@ __global__ __launch_bounds__(1024) void jobprefix_kernel(const Job *__restrict__
         baseA[nj] = sA[1023];
         baseB[nj] = sB[1023];
     }
+}
+
+// Work-item lists of both sides: one entry per item, {job, row, first pair},
+// at the job's base plus the row's item offset.  One thread per row.
+__global__ __launch_bounds__(256) void items_kernel(const Job *__restrict__ jobs, int level, Team team,
+                                                    const int32_t *__restrict__ baseA,
+                                                    const int32_t *__restrict__ baseB, WorkItem *__restrict__ itemsA,
+                                                    WorkItem *__restrict__ itemsB) {
+    const int b = blockIdx.y;
+    const Job &J = jobs[b];
+    const int p = J.p;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= p || level > J.max_level) return;
+    JobWs w = carve_job(J.ws, p);
+    const uint64_t count = binom(w.deg[r] - 1, level);
+    if (!count) return;
+    const int ppi = 256 / team_lanes(count, team);
+    const int na = w.itemoffA[r + 1] - w.itemoffA[r], nb = w.itemoffB[r + 1] - w.itemoffB[r];
+    WorkItem *oa = itemsA + baseA[b] + w.itemoffA[r];
+    WorkItem *ob = itemsB + baseB[b] + w.itemoffB[r];
+    for (int k = 0; k < na; ++k) oa[k] = WorkItem{b, r, k * ppi, 0};
+    for (int k = 0; k < nb; ++k) ob[k] = WorkItem{b, r, k * ppi, 0};
 }
 
 // ----------------------------------------------------------------------------
@@ -371,136 +451,189 @@ __device__ __forceinline__ bool z32_bad(float z, float dmin, float rho) {
 }
 
 // ----------------------------------------------------------------------------
-// level l >= 1, one phase.  GS lanes form a group that owns one ordered pair
-
-// ----------------------------------------------------------------------------
-// level l >= 1, FAST phase, row-blocked.  One wave per block; a work item is
-// (job, row i, up to PAIRS_PER_ITEM ordered pairs (i, j) of this phase).  The
-// wave stages what every test of row i reads into LDS:
-//   mode FULL   (deg + 1)^2 floats: C[A, A] with A = adj(i) + [i], so every CI
-//               test reads LDS only;
-//   mode ROW    neighbour ids, C[i, adj(i)] and diag C[s, s]; C[j, S] and the
-//               off-diagonal C[S, S] come from global memory (L2/MALL);
-//   mode GLOBAL nothing staged (only for degrees beyond the LDS budget).
-// Lane groups of GS lanes walk a pair's subset ranks in order; a test within
-// the error margin of the threshold sends the pair to the FP64 queue.
+// level l >= 1, FAST phase.  A block of 256 threads takes one work item: the
+// pairs q0.. of row i on this side (j > i on side A, j < i on side B).  The
+// pairs of a row share the subset candidates adj(i)\{j}, C(deg-1, l) per pair,
+// so the lanes that test one pair together (a "team") are sized by that count:
+// gs_min lanes for short enumerations (256/gs_min pairs per item), 64 lanes
+// (4 pairs per item) up to 512 subsets, the whole block (1 pair) beyond.  A
+// team walks the pair's subset ranks in increasing order and stops at the first
+// passing test, so the removing sepset is the first in pcalg order and zmin
+// covers exactly the tests up to it.  What the item's tests read is staged in
+// LDS by the whole block:
+//   FULL      (l >= 2) C[A, A] with A = adj(i) + [i]: every test reads LDS only;
+//   PAIRROWS  neighbour ids, C[i, adj(i)], diag C[s, s] and C[j, adj(i)] for the
+//             item's pairs; level 1 then reads LDS only, levels >= 2 read the
+//             off-diagonal C[S, S] from global memory (L2);
+//   GLOBAL    nothing staged (rows beyond the LDS budget).
+// A test within the FP32 error margin of the threshold sends the pair, from
+// the start of that iteration, to the FP64 queue.
 // gpucsl/pc/cuda/gaussian_ci.cu:66-120, set_helpers.cu:25-71,
 // graph_helpers.cu:81-133.
 // ----------------------------------------------------------------------------
-template <int L, int GS>
-__global__ __launch_bounds__(64) void gauss_rowblock_kernel(
-    const Job *__restrict__ jobs, int nj, const int32_t *__restrict__ itembase, int side, float margin_k,
-    SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount, unsigned long long *__restrict__ bstats) {
-    extern __shared__ int lds_words[];
-    const int lane = threadIdx.x;
-    const int gbase = lane & ~(GS - 1);
-    const int gl = lane & (GS - 1);
-    const int grp = lane / GS;
-    constexpr int NG = 64 / GS;
-    const uint64_t gmask = (GS == 64) ? ~0ull : (((1ull << GS) - 1ull) << gbase);
-    const int nitems = itembase[nj];
-    constexpr int WORDS = RB_LDS_BYTES / 4;
+template <int L>
+__device__ __forceinline__ void gather_test(int mode, const float *__restrict__ sf, const int *__restrict__ snb,
+                                            int d, int D, int jpos, int g, const float *__restrict__ C32, int p,
+                                            int i, int j, const int32_t *__restrict__ gnb, const int (&sp)[L],
+                                            float &cij, float (&ci)[L], float (&cj)[L], float (&cm)[L][L]) {
+    if (mode == 0) {
+        cij = sf[d * D + jpos];
+#pragma unroll
+        for (int a = 0; a < L; ++a) {
+            ci[a] = sf[d * D + sp[a]];
+            cj[a] = sf[jpos * D + sp[a]];
+#pragma unroll
+            for (int b = 0; b <= a; ++b) cm[a][b] = sf[sp[a] * D + sp[b]];
+        }
+    } else if (mode == 1) {
+        const float *rowI = sf, *diag = sf + d, *prow = sf + (size_t)(2 + g) * d;
+        cij = rowI[jpos];
+#pragma unroll
+        for (int a = 0; a < L; ++a) {
+            ci[a] = rowI[sp[a]];
+            cj[a] = prow[sp[a]];
+#pragma unroll
+            for (int b = 0; b < a; ++b) cm[a][b] = C32[(size_t)snb[sp[a]] * p + snb[sp[b]]];
+            cm[a][a] = diag[sp[a]];
+        }
+    } else {
+        int S[L];
+#pragma unroll
+        for (int a = 0; a < L; ++a) S[a] = gnb[sp[a]];
+        cij = C32[(size_t)i * p + j];
+#pragma unroll
+        for (int a = 0; a < L; ++a) {
+            ci[a] = C32[(size_t)i * p + S[a]];
+            cj[a] = C32[(size_t)j * p + S[a]];
+#pragma unroll
+            for (int b = 0; b <= a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
+        }
+    }
+}
 
+template <int L>
+__global__ __launch_bounds__(256) void gauss_level_fast_kernel(
+    const Job *__restrict__ jobs, int nj, const int32_t *__restrict__ itembase, const WorkItem *__restrict__ items,
+    int side, Team tm, int WORDS, float margin_k, SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount,
+    unsigned long long *__restrict__ bstats) {
+    extern __shared__ int lds_words[];
+    __shared__ unsigned long long s_pm[4], s_nm[4];
+    __shared__ float s_z[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nitems = itembase[nj];
     unsigned long long n_exec = 0, n_useful = 0;
 
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        const int jb = job_of(itembase, nj, item);
-        const Job &J = jobs[jb];
-        const int p = J.p;
+        const WorkItem it = items[item];
+        const Job &J = jobs[it.job];
+        const int p = J.p, i = it.row;
         JobWs jw = carve_job(J.ws, p);
-        const int32_t *__restrict__ itemoff = side == 0 ? jw.itemoffA : jw.itemoffB;
         const int32_t *__restrict__ rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
         const float *__restrict__ C32 = J.C32;
         uint8_t *__restrict__ Gm = J.G;
         const float thr32 = (float)J.thr[L];
-        const int local = item - itembase[jb];
-        int lo = 0, hi = p;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (itemoff[mid] <= local) lo = mid; else hi = mid;
-        }
-        const int i = lo;
-        const int chunk = local - itemoff[i];
         const int d = jw.deg[i];
         const int cnt = rowoff[i + 1] - rowoff[i];
-        const int q0 = chunk * PAIRS_PER_ITEM;
-        const int q1 = min(cnt, q0 + PAIRS_PER_ITEM);
+        const int Nn = d - 1;
+        const uint64_t count = binom_k<L>(Nn);
+        const int team = team_lanes(count, tm);
+        const int ppi = 256 / team;
+        const int q0 = it.q0;
+        const int q1 = min(cnt, q0 + ppi);
+        const int npairs = q1 - q0;
         const int jbase = side == 0 ? d - cnt : 0;
         const int D = d + 1;
-        const int mode = (D * D + d <= WORDS) ? 0 : (3 * d <= WORDS ? 1 : 2);
+        int mode;
+        if (L >= 2 && D * D + d <= WORDS)
+            mode = 0;
+        else if (d * (2 + npairs) + d <= WORDS)
+            mode = 1;
+        else
+            mode = 2;
         const int32_t *gnb = jw.nbr + (size_t)i * p;
-
-        __syncthreads();  // the previous item is done with LDS
         int *snb = lds_words;
         float *sf = reinterpret_cast<float *>(lds_words + d);
-        if (mode != 2) {
-            for (int k0 = 0; k0 < d; k0 += 64 * 4) {
-                int v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int k = k0 + u * 64 + lane;
-                    v[u] = k < d ? gnb[k] : 0;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int k = k0 + u * 64 + lane;
-                    if (k < d) snb[k] = v[u];
-                }
-            }
-        }
+
+        __syncthreads();  // the previous item is done with LDS
+        if (mode != 2)
+            for (int k = tid; k < d; k += 256) snb[k] = gnb[k];
         __syncthreads();
-        // Loads are issued in batches before their LDS stores so that each lane
-        // keeps several gathers in flight.
         if (mode == 0) {
-            // sub[a * D + b] = C[A[a], A[b]], A = adj(i) + [i]; D <= 63 < 64
-            const bool colok = lane < D;
-            const int vb = lane < d ? snb[lane] : i;
-            for (int a0 = 0; a0 < D; a0 += 8) {
-                float v[8];
+            // sub[a * D + b] = C[A[a], A[b]]: wave w stages rows a = w, w + 4, ...,
+            // four rows (eight loads per lane) per batch; D <= 90 < 128
+            for (int a0 = wave; a0 < D; a0 += 16) {
+                float v[4][2];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int a = a0 + u;
+                for (int u = 0; u < 4; ++u) {
+                    const int a = a0 + 4 * u;
                     const int va = a < d ? snb[a] : i;
-                    v[u] = (a < D && colok) ? C32[(size_t)va * p + vb] : 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const int b = c * 64 + lane;
+                        const int vb = b < d ? snb[b] : i;
+                        v[u][c] = (a < D && b < D) ? C32[(size_t)va * p + vb] : 0.0f;
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int a = a0 + u;
-                    if (a < D && colok) sf[a * D + lane] = v[u];
+                for (int u = 0; u < 4; ++u) {
+                    const int a = a0 + 4 * u;
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const int b = c * 64 + lane;
+                        if (a < D && b < D) sf[a * D + b] = v[u][c];
+                    }
                 }
             }
         } else if (mode == 1) {
-            for (int k0 = 0; k0 < d; k0 += 64 * 4) {
-                float v[4], w[4];
+            for (int k0 = 0; k0 < d; k0 += 256 * 2) {
+                float v[2], w[2];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int k = k0 + u * 64 + lane;
+                for (int u = 0; u < 2; ++u) {
+                    const int k = k0 + u * 256 + tid;
                     const int vv = k < d ? snb[k] : 0;
                     v[u] = k < d ? C32[(size_t)i * p + vv] : 0.0f;
                     w[u] = k < d ? C32[(size_t)vv * p + vv] : 0.0f;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int k = k0 + u * 64 + lane;
+                for (int u = 0; u < 2; ++u) {
+                    const int k = k0 + u * 256 + tid;
                     if (k < d) {
                         sf[k] = v[u];
                         sf[d + k] = w[u];
                     }
                 }
             }
+            const int total = npairs * d;
+            for (int t0 = 0; t0 < total; t0 += 256 * 4) {
+                float v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u * 256 + tid;
+                    float x = 0.0f;
+                    if (t < total) {
+                        const int g = t / d, k = t - g * d;
+                        const int j = snb[jbase + q0 + g];
+                        x = C32[(size_t)j * p + snb[k]];
+                    }
+                    v[u] = x;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u * 256 + tid;
+                    if (t < total) sf[2 * d + t] = v[u];
+                }
+            }
         }
         __syncthreads();
 
-        const int Nn = d - 1;
-        const uint64_t count = binom(Nn, L);
-        for (int q = q0 + grp; q < q1; q += NG) {
-            const int jpos = jbase + q;
+        if (team == 256) {
+            // ---- the whole block tests pair q0
+            const int jpos = jbase + q0;
             const int j = mode == 2 ? gnb[jpos] : snb[jpos];
-            if (count == 0) continue;
             if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
             float zl = __builtin_inff();
             for (uint64_t iter = 0;; ++iter) {
-                const uint64_t rank = iter * GS + gl;
+                const uint64_t rank = iter * 256 + tid;
                 const bool act = rank < count;
                 bool pass = false, near = false;
                 float zf = __builtin_inff();
@@ -510,116 +643,190 @@ __global__ __launch_bounds__(64) void gauss_rowblock_kernel(
                     unrank_lex<L>(Nn, rank, pos);
 #pragma unroll
                     for (int c = 0; c < L; ++c) {
-                        int idx = pos[c] - 1;
+                        const int idx = pos[c] - 1;
                         sp[c] = idx + (idx >= jpos);
                     }
                     float cij, ci[L], cj[L], cm[L][L];
-                    if (mode == 0) {
-                        cij = sf[d * D + jpos];
-#pragma unroll
-                        for (int a = 0; a < L; ++a) {
-                            ci[a] = sf[d * D + sp[a]];
-                            cj[a] = sf[jpos * D + sp[a]];
-#pragma unroll
-                            for (int b = 0; b <= a; ++b) cm[a][b] = sf[sp[a] * D + sp[b]];
-                        }
-                    } else if (mode == 1) {
-                        int S[L];
-#pragma unroll
-                        for (int a = 0; a < L; ++a) S[a] = snb[sp[a]];
-                        cij = sf[jpos];
-#pragma unroll
-                        for (int a = 0; a < L; ++a) {
-                            ci[a] = sf[sp[a]];
-                            cj[a] = C32[(size_t)j * p + S[a]];
-#pragma unroll
-                            for (int b = 0; b < a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
-                            cm[a][a] = sf[d + sp[a]];
-                        }
-                    } else {
-                        int S[L];
-#pragma unroll
-                        for (int a = 0; a < L; ++a) S[a] = gnb[sp[a]];
-                        cij = C32[(size_t)i * p + j];
-#pragma unroll
-                        for (int a = 0; a < L; ++a) {
-                            ci[a] = C32[(size_t)i * p + S[a]];
-                            cj[a] = C32[(size_t)j * p + S[a]];
-#pragma unroll
-                            for (int b = 0; b <= a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
-                        }
-                    }
+                    gather_test<L>(mode, sf, snb, d, D, jpos, 0, C32, p, i, j, gnb, sp, cij, ci, cj, cm);
                     float dmin, rho;
                     const float z = z32_from_values<L>(cij, ci, cj, cm, dmin, rho);
                     near = z32_bad(z, dmin, rho) || fabsf(z - thr32) <= margin_k * z32_error<L>(dmin, rho);
                     pass = z < thr32;
                     zf = z;
                 }
-                if (__ballot(near) & gmask) {
-#pragma unroll
-                    for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
-                    if (gl == 0) {
-                        const unsigned int slot = atomicAdd(qcount, 1u);
-                        queue[slot] = SlowItem{rowoff[i] + q, (unsigned int)iter, zl, jb};
-                    }
-                    break;
+                const unsigned long long pm = __ballot(pass), nm = __ballot(near);
+                if (lane == 0) {
+                    s_pm[wave] = pm;
+                    s_nm[wave] = nm;
                 }
-                if (act) ++n_exec;
-                const uint64_t pm = __ballot(pass) & gmask;
-                bool finish;
-                if (pm) {
-                    const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
-                    if (act && gl <= P) {
-                        ++n_useful;
-                        if (zf < zl) zl = zf;
-                    }
-                    if (gl == P) {
-                        Gm[(size_t)i * p + j] = 0;
-                        Gm[(size_t)j * p + i] = 0;
-                        int32_t *sep = J.sepsets;
-                        const int maxL = J.max_level;
+                __syncthreads();
+                const bool any_near = (s_nm[0] | s_nm[1] | s_nm[2] | s_nm[3]) != 0;
+                int P = -1;
 #pragma unroll
-                        for (int c = 0; c < L; ++c) {
-                            const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
-                            sep[((size_t)i * p + j) * maxL + c] = v;
-                            sep[((size_t)j * p + i) * maxL + c] = v;
-                        }
-                    }
+                for (int w2 = 0; w2 < 4; ++w2)
+                    if (P < 0 && s_pm[w2]) P = w2 * 64 + __ffsll(s_pm[w2]) - 1;
+                bool finish;
+                bool write_zmin = false;
+                if (any_near) {
                     finish = true;
                 } else {
-                    if (act) {
-                        ++n_useful;
-                        if (zf < zl) zl = zf;
+                    if (act) ++n_exec;
+                    if (P >= 0) {
+                        if (act && tid <= P) {
+                            ++n_useful;
+                            zl = fminf(zl, zf);
+                        }
+                        if (tid == P) {
+                            Gm[(size_t)i * p + j] = 0;
+                            Gm[(size_t)j * p + i] = 0;
+                            int32_t *sep = J.sepsets;
+                            const int maxL = J.max_level;
+#pragma unroll
+                            for (int c = 0; c < L; ++c) {
+                                const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
+                                sep[((size_t)i * p + j) * maxL + c] = v;
+                                sep[((size_t)j * p + i) * maxL + c] = v;
+                            }
+                        }
+                        finish = true;
+                    } else {
+                        if (act) {
+                            ++n_useful;
+                            zl = fminf(zl, zf);
+                        }
+                        finish = (iter + 1) * 256 >= count;
                     }
-                    finish = (iter + 1) * GS >= count;
+                    write_zmin = finish;
                 }
                 if (finish) {
-#pragma unroll
-                    for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
-                    if (gl == 0) {
-                        const int a = i < j ? i : j, b = i < j ? j : i;
-                        float *zp = J.zmin + (size_t)a * p + b;
-                        if (zl < *zp) *zp = zl;
+                    float m = zl;
+                    for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+                    if (lane == 0) s_z[wave] = m;
+                    __syncthreads();
+                    if (tid == 0) {
+                        const float zb = fminf(fminf(s_z[0], s_z[1]), fminf(s_z[2], s_z[3]));
+                        if (any_near) {
+                            const unsigned int slot = atomicAdd(qcount, 1u);
+                            queue[slot] = SlowItem{iter * 256, rowoff[i] + q0, it.job, zb, 0};
+                        } else if (write_zmin) {
+                            const int a = i < j ? i : j, b = i < j ? j : i;
+                            float *zp = J.zmin + (size_t)a * p + b;
+                            if (zb < *zp) *zp = zb;
+                        }
                     }
-                    break;
                 }
+                __syncthreads();  // s_pm / s_nm / s_z are reused
+                if (finish) break;
+            }
+            continue;
+        }
+
+        // ---- teams of `team` lanes inside a wave, one pair each
+        const int g = tid / team;
+        const int q = q0 + g;
+        if (q >= q1) continue;
+        const int gbase = lane & ~(team - 1);
+        const int gl = lane & (team - 1);
+        const uint64_t gmask = team == 64 ? ~0ull : (((1ull << team) - 1ull) << gbase);
+        const int jpos = jbase + q;
+        const int j = mode == 2 ? gnb[jpos] : snb[jpos];
+        if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
+        float zl = __builtin_inff();
+        for (uint64_t iter = 0;; ++iter) {
+            const uint64_t rank = iter * team + gl;
+            const bool act = rank < count;
+            bool pass = false, near = false;
+            float zf = __builtin_inff();
+            int sp[L];
+            if (act) {
+                int pos[L];
+                unrank_lex<L>(Nn, rank, pos);
+#pragma unroll
+                for (int c = 0; c < L; ++c) {
+                    const int idx = pos[c] - 1;
+                    sp[c] = idx + (idx >= jpos);
+                }
+                float cij, ci[L], cj[L], cm[L][L];
+                gather_test<L>(mode, sf, snb, d, D, jpos, g, C32, p, i, j, gnb, sp, cij, ci, cj, cm);
+                float dmin, rho;
+                const float z = z32_from_values<L>(cij, ci, cj, cm, dmin, rho);
+                near = z32_bad(z, dmin, rho) || fabsf(z - thr32) <= margin_k * z32_error<L>(dmin, rho);
+                pass = z < thr32;
+                zf = z;
+            }
+            if (__ballot(near) & gmask) {
+                for (int off = team / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
+                if (gl == 0) {
+                    const unsigned int slot = atomicAdd(qcount, 1u);
+                    queue[slot] = SlowItem{iter * team, rowoff[i] + q, it.job, zl, 0};
+                }
+                break;
+            }
+            if (act) ++n_exec;
+            const uint64_t pm = __ballot(pass) & gmask;
+            bool finish;
+            if (pm) {
+                const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
+                if (act && gl <= P) {
+                    ++n_useful;
+                    zl = fminf(zl, zf);
+                }
+                if (gl == P) {
+                    Gm[(size_t)i * p + j] = 0;
+                    Gm[(size_t)j * p + i] = 0;
+                    int32_t *sep = J.sepsets;
+                    const int maxL = J.max_level;
+#pragma unroll
+                    for (int c = 0; c < L; ++c) {
+                        const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
+                        sep[((size_t)i * p + j) * maxL + c] = v;
+                        sep[((size_t)j * p + i) * maxL + c] = v;
+                    }
+                }
+                finish = true;
+            } else {
+                if (act) {
+                    ++n_useful;
+                    zl = fminf(zl, zf);
+                }
+                finish = (iter + 1) * team >= count;
+            }
+            if (finish) {
+                for (int off = team / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
+                if (gl == 0) {
+                    const int a = i < j ? i : j, b = i < j ? j : i;
+                    float *zp = J.zmin + (size_t)a * p + b;
+                    if (zl < *zp) *zp = zl;
+                }
+                break;
             }
         }
     }
+    // block totals -> one statistics stripe
     for (int off = 32; off >= 1; off >>= 1) {
         n_exec += __shfl_xor(n_exec, off);
         n_useful += __shfl_xor(n_useful, off);
     }
-    if (lane == 0 && n_exec) {
-        unsigned long long *dst = bstats + (size_t)(blockIdx.x % BSTAT_STRIPES) * BSTAT_WORDS;
-        atomicAdd(dst + BS_EXEC, n_exec);
-        atomicAdd(dst + BS_USEFUL, n_useful);
+    __syncthreads();
+    if (lane == 0) {
+        s_pm[wave] = n_exec;
+        s_nm[wave] = n_useful;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned long long e = s_pm[0] + s_pm[1] + s_pm[2] + s_pm[3];
+        const unsigned long long u = s_nm[0] + s_nm[1] + s_nm[2] + s_nm[3];
+        if (e) {
+            unsigned long long *dst = bstats + (size_t)(blockIdx.x % BSTAT_STRIPES) * BSTAT_WORDS;
+            atomicAdd(dst + BS_EXEC, e);
+            atomicAdd(dst + BS_USEFUL, u);
+        }
     }
 }
 
 // ----------------------------------------------------------------------------
 // level l >= 1, SLOW phase: finishes the pairs the FAST kernel queued, from the
-// queued iteration on, rechecking near-threshold FP32 tests in FP64 with the
+// queued subset rank on, rechecking near-threshold FP32 tests in FP64 with the
 // reference's operation order.  Work: a static share per lane group, then 8
 // striped counters chosen by block index (no single hot atomic).
 // ----------------------------------------------------------------------------
@@ -651,7 +858,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
     const int32_t *gnb = nullptr;
     double thr = 0.0;
     float thr32 = 0.0f;
-    uint64_t count = 0, iter = 0;
+    uint64_t count = 0, next = 0;
     float zl = __builtin_inff();
     bool have = false, done = false;
     unsigned long long n_exec = 0, n_re = 0, n_flip = 0, n_useful = 0;
@@ -695,7 +902,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 j = gnb[jpos];
                 Nn = d - 1;
                 count = binom(Nn, L);
-                iter = qi.iter;
+                next = qi.start;
                 zl = gl == 0 ? qi.zl : __builtin_inff();
                 thr = J->thr[L];
                 thr32 = (float)thr;
@@ -705,7 +912,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
         if (__ballot(!done) == 0) break;
         if (!have) continue;
 
-        const uint64_t rank = iter * GS + gl;
+        const uint64_t rank = next + gl;
         const bool act = rank < count;
         bool pass = false;
         float zf = __builtin_inff();
@@ -761,8 +968,8 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 ++n_useful;
                 if (zf < zl) zl = zf;
             }
-            ++iter;
-            finish = iter * GS >= count;
+            next += GS;
+            finish = next >= count;
         }
         if (finish) {
 #pragma unroll
@@ -879,13 +1086,13 @@ static int device_cus() {
     return cus;
 }
 static int slow_grid() { return device_cus() * PHASE_BLOCKS_PER_CU / 4; }
-static int rowblock_grid() {
-    const int g = device_cus() * RB_BLOCKS_PER_CU;
-    return g > MAX_PHASE_BLOCKS ? MAX_PHASE_BLOCKS : g;
+static int fast_grid(int level) {
+    tuning_defaults();
+    return device_cus() * g_tuning[level].fast_blocks_per_cu;
 }
 
 size_t gauss_batch_ws_bytes(const Job *jobs_host, int nj) {
-    return batch_ws_bytes(nj, queue_capacity(jobs_host, nj));
+    return batch_ws_bytes(nj, queue_capacity(jobs_host, nj), item_capacity(jobs_host, nj));
 }
 
 static int max_p(const Job *jh, int nj) {
@@ -920,10 +1127,13 @@ int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, uns
     return check_hip(hipGetLastError(), "gauss_level0 launch");
 }
 
-int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, unsigned long long *st, hipStream_t s) {
+int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
+                        hipStream_t s) {
     const int mp = max_p(jh, nj);
     GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd);
-    GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, st);
+    const Team team = team_for(level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
+                               gs_min);
+    GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, team, st);
     return check_hip(hipGetLastError(), "compact launch");
 }
 
@@ -934,29 +1144,19 @@ static const char *const kPhaseSlowName[] = {
     "", "gauss_level1_slow", "gauss_level2_slow", "gauss_level3_slow", "gauss_level4_slow",
     "gauss_level5_slow", "gauss_level6_slow", "gauss_level7_slow", "gauss_level8_slow"};
 
-template <int L, int GS>
-static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, float margin_k, hipStream_t s) {
+template <int L>
+static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int gs_min, float margin_k,
+                         hipStream_t s) {
     const int32_t *base = side == 0 ? w.itembaseA : w.itembaseB;
+    const WorkItem *items = side == 0 ? w.itemsA : w.itemsB;
     SlowItem *queue = side == 0 ? w.queueA : w.queueB;
     unsigned int *qcount = w.counters + CNT_QUEUE + side;
     unsigned int *slow_ctr = w.counters + (2 * side + 1) * STRIPES;
-    GPUCSL_LAUNCH(kPhaseFastName[L], (gauss_rowblock_kernel<L, GS>), dim3(rowblock_grid()), dim3(64), RB_LDS_BYTES,
-                  s, jd, nj, base, side, margin_k, queue, qcount, w.bstats);
-    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, GS>), dim3(slow_grid()), dim3(256), 0, s, jd, side,
+    const int lds = g_tuning[L].fast_lds_bytes;
+    GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj, base,
+                  items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
+    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 16>), dim3(slow_grid()), dim3(256), 0, s, jd, side,
                   margin_k, slow_ctr, queue, qcount, w.bstats);
-}
-
-template <int L>
-static int phase_dispatch_gs(int gs, const Job *jd, int nj, const BatchWs &w, int side, float margin_k,
-                             hipStream_t s) {
-    switch (gs) {
-    case 8: launch_phase<L, 8>(jd, nj, w, side, margin_k, s); break;
-    case 16: launch_phase<L, 16>(jd, nj, w, side, margin_k, s); break;
-    case 32: launch_phase<L, 32>(jd, nj, w, side, margin_k, s); break;
-    case 64: launch_phase<L, 64>(jd, nj, w, side, margin_k, s); break;
-    default: return set_error(ERR_ARG, "group size must be 8, 16, 32 or 64 (got %d)", gs);
-    }
-    return OK;
 }
 
 int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float margin_k, int group_size,
@@ -964,28 +1164,32 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
     if (level < 1) return set_error(ERR_ARG, "gauss_level: level %d < 1", level);
     if (level > GPUCSL_MAX_LEVEL_TEMPLATE)
         return set_error(ERR_UNSUPPORTED, "gauss_level: levels above %d are not built", GPUCSL_MAX_LEVEL_TEMPLATE);
+    if (group_size != 8 && group_size != 16 && group_size != 32 && group_size != 64)
+        return set_error(ERR_ARG, "group size must be 8, 16, 32 or 64 (got %d)", group_size);
     if (!st || !bws) return set_error(ERR_ARG, "gauss_level: null stats or workspace");
-    const size_t qcap = queue_capacity(jh, nj);
-    if (bws_bytes < batch_ws_bytes(nj, qcap)) return set_error(ERR_WORKSPACE, "gauss_level: workspace too small");
+    const size_t qcap = queue_capacity(jh, nj), icap = item_capacity(jh, nj);
+    if (bws_bytes < batch_ws_bytes(nj, qcap, icap)) return set_error(ERR_WORKSPACE, "gauss_level: workspace too small");
     for (int b = 0; b < nj; ++b)
         if (!jh[b].C32 || !jh[b].C64 || !jh[b].G || !jh[b].zmin || !jh[b].ws || (jh[b].max_level > 0 && !jh[b].sepsets))
             return set_error(ERR_ARG, "gauss_level: job %d has a null buffer", b);
-    int rc = gauss_batch_compact(jh, jd, nj, level, st, s);
+    int rc = gauss_batch_compact(jh, jd, nj, level, group_size, st, s);
     if (rc) return rc;
-    BatchWs w = carve_batch(bws, nj, qcap);
+    BatchWs w = carve_batch(bws, nj, qcap, icap);
     GPUCSL_LAUNCH("job_prefix", jobprefix_kernel, dim3(1), dim3(1024), 0, s, jd, nj, w.itembaseA, w.itembaseB,
                   w.counters, w.bstats);
+    const int mp = max_p(jh, nj);
+    GPUCSL_LAUNCH("work_items", items_kernel, dim3((mp + 255) / 256, nj), dim3(256), 0, s, jd, level,
+                  team_for(level, group_size), w.itembaseA, w.itembaseB, w.itemsA, w.itemsB);
     for (int side = 0; side < 2; ++side) {
         switch (level) {
-#define GPUCSL_CASE(LV)                                                               \
-    case LV:                                                                          \
-        rc = phase_dispatch_gs<LV>(group_size, jd, nj, w, side, margin_k, s);         \
+#define GPUCSL_CASE(LV)                                                     \
+    case LV:                                                                \
+        launch_phase<LV>(jd, nj, w, side, group_size, margin_k, s);         \
         break;
             GPUCSL_CASE(1) GPUCSL_CASE(2) GPUCSL_CASE(3) GPUCSL_CASE(4)
             GPUCSL_CASE(5) GPUCSL_CASE(6) GPUCSL_CASE(7) GPUCSL_CASE(8)
 #undef GPUCSL_CASE
         }
-        if (rc) return rc;
     }
     GPUCSL_LAUNCH("level_stats", level_stats_kernel, dim3(1), dim3(64), 0, s, w.bstats, st);
     return check_hip(hipGetLastError(), "gauss_level launch");
@@ -1003,7 +1207,8 @@ int gauss_batch_finish(const Job *jh, const Job *jd, int nj, hipStream_t s) {
 size_t gauss_workspace_bytes(int p) {
     Job j{};
     j.p = p;
-    return JOB_DESC_BYTES + align_up(batch_ws_bytes(1, queue_capacity(&j, 1)), 256) + job_ws_bytes(p);
+    return JOB_DESC_BYTES + align_up(batch_ws_bytes(1, queue_capacity(&j, 1), item_capacity(&j, 1)), 256) +
+           job_ws_bytes(p);
 }
 
 struct SingleWs {
@@ -1019,7 +1224,7 @@ static SingleWs carve_single(void *ws, int p) {
     SingleWs w;
     w.desc = (Job *)ws;
     w.bws = (char *)ws + JOB_DESC_BYTES;
-    w.bws_bytes = align_up(batch_ws_bytes(1, queue_capacity(&j, 1)), 256);
+    w.bws_bytes = align_up(batch_ws_bytes(1, queue_capacity(&j, 1), item_capacity(&j, 1)), 256);
     w.jws = (char *)w.bws + w.bws_bytes;
     return w;
 }
@@ -1104,7 +1309,7 @@ int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long 
     Job j = single_job(p, level, nullptr, nullptr, const_cast<uint8_t *>(Gm), nullptr, nullptr, nullptr, 0, ws);
     int rc = single_write_desc(j, ws, s);
     if (rc) return rc;
-    return gauss_batch_compact(&j, carve_single(ws, p).desc, 1, level, stats, s);
+    return gauss_batch_compact(&j, carve_single(ws, p).desc, 1, level, 16, stats, s);
 }
 
 // Views of the level-start neighbour lists inside a single-dataset workspace.

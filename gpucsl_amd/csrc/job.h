@@ -13,10 +13,16 @@ static_assert(sizeof(Job) <= JOB_DESC_BYTES, "job descriptor too large");
 
 // FP64-recheck queue entry
 struct SlowItem {
+    uint64_t start;     // first subset rank still to evaluate
     int w;              // pair index in the phase numbering of the job
-    unsigned int iter;  // first iteration still to evaluate
-    float zl;           // zmin over the tests already counted
     int job;            // index into the launch's job table
+    float zl;           // zmin over the tests already counted
+    int pad;
+};
+
+// level work item: `pairs` consecutive pairs of one row, from pair q0
+struct WorkItem {
+    int job, row, q0, pad;
 };
 
 // per-job level workspace
@@ -89,18 +95,22 @@ struct BatchWs {
     unsigned long long *bstats;    // BSTAT_STRIPES * BSTAT_WORDS
     int32_t *itembaseA, *itembaseB;  // nj + 1 each
     SlowItem *queueA, *queueB;     // qcap each
+    WorkItem *itemsA, *itemsB;     // qcap + sum of p each
 };
 
-inline size_t batch_ws_bytes(int nj, size_t qcap) {
+// A phase has at most one queue entry per ordered pair of its side, and at
+// most one work item per pair plus one per row.
+inline size_t batch_ws_bytes(int nj, size_t qcap, size_t icap) {
     size_t off = 0;
     off = align_up(off + sizeof(unsigned int) * NUM_COUNTERS, 256);
     off = align_up(off + sizeof(unsigned long long) * BSTAT_STRIPES * BSTAT_WORDS, 256);
     off = align_up(off + sizeof(int32_t) * (size_t)(nj + 1) * 2, 256);
     off = align_up(off + sizeof(SlowItem) * qcap * 2, 256);
+    off = align_up(off + sizeof(WorkItem) * icap * 2, 256);
     return off;
 }
 
-inline BatchWs carve_batch(void *ws, int nj, size_t qcap) {
+inline BatchWs carve_batch(void *ws, int nj, size_t qcap, size_t icap) {
     BatchWs w;
     char *b = (char *)ws;
     size_t off = 0;
@@ -113,13 +123,21 @@ inline BatchWs carve_batch(void *ws, int nj, size_t qcap) {
     off = align_up(off + sizeof(int32_t) * (size_t)(nj + 1) * 2, 256);
     w.queueA = (SlowItem *)(b + off);
     w.queueB = w.queueA + qcap;
+    off = align_up(off + sizeof(SlowItem) * qcap * 2, 256);
+    w.itemsA = (WorkItem *)(b + off);
+    w.itemsB = w.itemsA + icap;
     return w;
 }
 
-// A phase has at most one queue entry per ordered pair of its side.
 inline size_t queue_capacity(const Job *jobs_host, int nj) {
     size_t q = 1;
     for (int b = 0; b < nj; ++b) q += (size_t)jobs_host[b].p * (jobs_host[b].p - 1) / 2;
+    return q;
+}
+
+inline size_t item_capacity(const Job *jobs_host, int nj) {
+    size_t q = queue_capacity(jobs_host, nj);
+    for (int b = 0; b < nj; ++b) q += (size_t)jobs_host[b].p;
     return q;
 }
 

@@ -42,10 +42,26 @@ class DiscreteResult:
     stats: list
 
 
+def disc_layout(n: int, p: int, max_level: int, max_dim: int):
+    """Byte offsets of one discrete job's buffers in a single arena, and its size."""
+    stride = (n + COLUMN_ALIGN - 1) // COLUMN_ALIGN * COLUMN_ALIGN
+    L = max(max_level, 1)
+    pp = p * p
+    sizes = [("data", p * stride), ("dims", p), ("G", pp), ("sep", 4 * pp * L), ("pmax", 4 * pp),
+             ("skel16", 2 * pp), ("stats", 8 * (max_level + 1) * nat.STAT_SLOTS),
+             ("ws", int(nat.load().gpucsl_disc_workspace_bytes(p, max_level, max(max_dim, 1))))]
+    off, out = 0, {}
+    for name, sz in sizes:
+        out[name] = (off, sz)
+        off = (off + sz + 255) // 256 * 256
+    return out, off
+
+
 class DiscreteEngine:
-    def __init__(self, p: int, n: int, max_level: int, alpha: float, max_dim: int):
+    def __init__(self, p: int, n: int, max_level: int, alpha: float, max_dim: int, arena=None):
+        """arena: optional uint8 device tensor of at least disc_layout(...)[1]
+        bytes that holds every buffer of the job (the serving pool)."""
         torch = nat.require_gpu()
-        lib = nat.load()
         if p < 2:
             raise GpucslError("need at least two variables")
         self.torch = torch
@@ -55,15 +71,28 @@ class DiscreteEngine:
         self.device = dev
         self.stride = (self.n + COLUMN_ALIGN - 1) // COLUMN_ALIGN * COLUMN_ALIGN
         L = max(self.max_level, 1)
-        self.data = torch.zeros((p, self.stride), dtype=torch.uint8, device=dev)
-        self.dims = torch.empty((p,), dtype=torch.uint8, device=dev)
-        self.G = torch.empty((p, p), dtype=torch.uint8, device=dev)
-        self.sep = torch.empty((p, p, L), dtype=torch.int32, device=dev)
-        self.pmax = torch.empty((p, p), dtype=torch.float32, device=dev)
-        self.skel16 = torch.empty((p, p), dtype=torch.int16, device=dev)
-        self.stats = torch.zeros(((self.max_level + 1) * nat.STAT_SLOTS,), dtype=torch.int64, device=dev)
-        self.ws_bytes = int(lib.gpucsl_disc_workspace_bytes(p, self.max_level, self.max_dim))
-        self.ws = torch.empty((self.ws_bytes,), dtype=torch.uint8, device=dev)
+        layout, nbytes = disc_layout(self.n, self.p, self.max_level, self.max_dim)
+        if arena is None:
+            arena = torch.empty((nbytes,), dtype=torch.uint8, device=dev)
+        elif arena.numel() < nbytes:
+            raise GpucslError("arena too small for the discrete job")
+        self.arena, self.nbytes = arena, nbytes
+
+        def view(name, dtype, shape):
+            off, sz = layout[name]
+            return arena[off:off + sz].view(dtype).view(*shape)
+
+        self.data = view("data", torch.uint8, (p, self.stride))
+        self.data.zero_()
+        self.dims = view("dims", torch.uint8, (p,))
+        self.G = view("G", torch.uint8, (p, p))
+        self.sep = view("sep", torch.int32, (p, p, L))
+        self.pmax = view("pmax", torch.float32, (p, p))
+        self.skel16 = view("skel16", torch.int16, (p, p))
+        self.stats = view("stats", torch.int64, ((self.max_level + 1) * nat.STAT_SLOTS,))
+        self.stats.zero_()
+        self.ws_bytes = layout["ws"][1]
+        self.ws = view("ws", torch.uint8, (self.ws_bytes,))
 
     def workspace_bytes(self) -> int:
         return self.ws_bytes
@@ -80,6 +109,35 @@ class DiscreteEngine:
         host[:, : self.n] = codes.T
         self.data.copy_(torch.from_numpy(host))
         self.dims.copy_(torch.from_numpy(np.ascontiguousarray(dims, np.uint8)))
+
+    def set_codes_device(self, colmajor, dims) -> None:
+        """Device input: colmajor (p, >= n) uint8 codes, one row per variable
+        (only the first n columns are read), dims (p,) category counts."""
+        torch = self.torch
+        if colmajor.dtype != torch.uint8 or colmajor.shape[0] != self.p or colmajor.shape[1] < self.n:
+            raise GpucslError(f"codes must be uint8 ({self.p}, >= {self.n})")
+        self.data[:, : self.n].copy_(colmajor[:, : self.n], non_blocking=True)
+        self.dims.copy_(dims.to(torch.uint8), non_blocking=True)
+
+    # ---- level by level (the serving loop interleaves these with other jobs)
+    def start(self) -> None:
+        """Buffer set-up and level 0."""
+        s = nat.stream_handle()
+        nat.call("gpucsl_disc_init", self.p, self.max_level, self.max_dim, self.G.data_ptr(), self.sep.data_ptr(),
+                 self.pmax.data_ptr(), self.stats.data_ptr(), self.ws.data_ptr(), self.ws_bytes, s)
+        nat.call("gpucsl_disc_level0", self.data.data_ptr(), self.n, self.stride, self.dims.data_ptr(), self.p,
+                 self.max_level, self.max_dim, self.alpha, self.G.data_ptr(), self.pmax.data_ptr(),
+                 self.stats.data_ptr(), self.ws.data_ptr(), self.ws_bytes, s)
+
+    def level(self, l: int) -> None:
+        st = self.stats.data_ptr() + 8 * nat.STAT_SLOTS * l
+        nat.call("gpucsl_disc_level", l, self.data.data_ptr(), self.n, self.stride, self.dims.data_ptr(), self.p,
+                 self.max_level, self.max_dim, self.alpha, self.G.data_ptr(), self.sep.data_ptr(),
+                 self.pmax.data_ptr(), st, self.ws.data_ptr(), self.ws_bytes, nat.stream_handle())
+
+    def finish(self) -> None:
+        nat.call("gpucsl_disc_finish", self.G.data_ptr(), self.p, self.pmax.data_ptr(), self.skel16.data_ptr(),
+                 nat.stream_handle())
 
     def run(self) -> None:
         nat.call("gpucsl_disc_skeleton", self.data.data_ptr(), self.n, self.stride, self.dims.data_ptr(),

@@ -79,3 +79,69 @@ def bootstrap(data: np.ndarray, seed: int) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed))
     rows = rng.integers(0, data.shape[0], size=data.shape[0])
     return data[rows]
+
+
+# ---------------------------------------------------------------------------
+# Device generators for large request streams.  The DAG, coefficients and
+# conditional-probability tables come from numpy (as above); the n x p random
+# draws come from a seeded torch generator on the GPU, so a 2000 x 50000
+# dataset takes milliseconds instead of seconds.  The values differ from the
+# numpy generators' for the same seed.
+# ---------------------------------------------------------------------------
+def gaussian_sem_device(p: int, n: int, degree: float = 3.0, seed: int = 0, device="cuda"):
+    """Float32 (n, p) tensor on the device: X = E (I - B)^-1 for the linear SEM
+    on a random ER DAG; (I - B) is unit upper triangular in causal order."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    A, order = random_dag(p, degree, rng)
+    B = np.where(A, rng.uniform(0.5, 1.0, (p, p)) * rng.choice([-1.0, 1.0], (p, p)), 0.0)
+    Bo = B[np.ix_(order, order)]                    # strictly upper triangular in causal order
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    E = torch.randn((n, p), generator=g, device=device, dtype=torch.float64)
+    I = torch.eye(p, dtype=torch.float64, device=device)
+    Minv = torch.linalg.solve_triangular(I - torch.from_numpy(Bo).to(device), I, upper=True)   # p x p
+    Xo = E @ Minv
+    X = torch.empty_like(Xo)
+    X[:, torch.from_numpy(order).to(device)] = Xo
+    return X.to(torch.float32)
+
+
+def discrete_bn_device(p: int, n: int, degree: float = 2.0, seed: int = 0, card_low: int = 2, card_high: int = 5,
+                       device="cuda", row_align: int = 64):
+    """Ancestral sample of a random discrete BN on the device.  Returns
+    (codes (p, stride) uint8 with one row per variable, stride = n rounded up
+    to row_align, zero padded; dims (p,) uint8)."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    A, order = random_dag(p, degree, rng)
+    card = rng.integers(card_low, card_high + 1, size=p)
+    stride = (n + row_align - 1) // row_align * row_align
+    codes = torch.zeros((p, stride), dtype=torch.uint8, device=device)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    for k in order:
+        parents = np.flatnonzero(A[:, k])
+        cfg = torch.zeros(n, dtype=torch.int64, device=device)
+        radix = 1
+        for pa in parents:
+            cfg += codes[pa, :n].to(torch.int64) * radix
+            radix *= int(card[pa])
+        cdf = torch.from_numpy(np.cumsum(rng.dirichlet(np.ones(int(card[k])), size=radix), axis=1)).to(device)
+        u = torch.rand(n, generator=g, device=device, dtype=torch.float64)
+        vals = (u[:, None] > cdf[cfg]).sum(dim=1).clamp_(max=int(card[k]) - 1)
+        codes[k, :n] = vals.to(torch.uint8)
+    # ordinal encoding (codes of the categories that occur, in sorted order),
+    # as the discrete skeleton's host preparation does
+    body = codes[:, :n].to(torch.int64)
+    bins = 8
+    seen = torch.zeros(p * bins, dtype=torch.int64, device=device)
+    seen.index_add_(0, (body + torch.arange(p, device=device)[:, None] * bins).reshape(-1),
+                    torch.ones(p * n, dtype=torch.int64, device=device))
+    present = (seen.view(p, bins) > 0).to(torch.int64)
+    remap = present.cumsum(dim=1) - 1
+    codes[:, :n] = torch.gather(remap, 1, body).to(torch.uint8)
+    dims = present.sum(dim=1).to(torch.uint8)
+    return codes, dims

@@ -163,6 +163,22 @@ int gpucsl_gauss_batch_finish(const gpucsl_gauss_job *jobs, const gpucsl_gauss_j
 int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss_job *jobs_dev, int nj, float margin,
                                 int group_size, uint64_t *stats, void *ws, size_t ws_bytes, void *stream);
 
+/* Launch shapes of the level-l CI-test kernel (no counterpart in the
+ * reference, whose kernel_management.py:61-65 fixes threads per pair).  The
+ * lanes that test one pair together are group_size while C(deg-1, l) <=
+ * group_size * small_team_iters, 64 while it is <= 64 * wave_team_iters, else
+ * 256; fast_blocks_per_cu and fast_lds_bytes (4096..65536, a multiple of 1024)
+ * size the grid and its LDS staging.  Results do not depend on these values.
+ * level 0 sets every level.  Defaults: {8, 8, 4, 32768}. */
+typedef struct gpucsl_level_tuning {
+    int32_t small_team_iters;
+    int32_t wave_team_iters;
+    int32_t fast_blocks_per_cu;
+    int32_t fast_lds_bytes;
+} gpucsl_level_tuning;
+int gpucsl_set_level_tuning(int level, const gpucsl_level_tuning *tuning);
+int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning);
+
 /* pmax = 2 (1 - Phi(sqrt(n-3) |zmin|)), -1 on and below the diagonal.
  * Replaces transform_to_pmax_cupy + postprocess_pmax_cupy
  * (gpucsl/pc/helpers.py:62-82). */
@@ -201,6 +217,10 @@ int gpucsl_disc_level(int level, const uint8_t *data, int64_t n, int64_t stride,
 int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
                          int max_level, int max_dim, double alpha, uint8_t *G, int32_t *sepsets, float *pmax,
                          uint64_t *stats, void *ws, size_t ws_bytes, void *stream);
+/* After level-by-level calls: pmax = -1 on and below the diagonal
+ * (postprocess_pmax, gpucsl/pc/helpers.py:96-101) and, when skeleton is not
+ * NULL, the uint16 skeleton (the dtype of gpucsl's SkeletonResult). */
+int gpucsl_disc_finish(const uint8_t *G, int p, float *pmax, uint16_t *skeleton, void *stream);
 
 /* One CI test, for known-answer checks: table_out gets the dims_s*di*dj counts
  * N[g][a][b], out[0..2] = chi^2, p value, degrees of freedom.  sepset is a

@@ -1,0 +1,128 @@
+"""Serving loop on the GPU: requests admitted by HBM footprint while others are
+in flight give the same bits as the same request alone through the
+single-dataset path, for Gaussian and discrete requests.  GPU box only."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gpucsl_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _requests(seed, n_gauss, n_disc, small=True):
+    from gpucsl_amd.serving import Request
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    reqs = []
+    kinds = ["gaussian"] * n_gauss + ["discrete"] * n_disc
+    rng.shuffle(kinds)
+    for k, kind in enumerate(kinds):
+        if kind == "gaussian":
+            p, n = int(rng.integers(20, 151)), int(rng.integers(500, 3001))
+            X = syn.gaussian_sem_device(p, n, 3.0, seed=seed * 100 + k)
+            reqs.append(Request("gaussian", n, p, max_level=int(rng.integers(1, 4)), alpha=0.01, X=X))
+        else:
+            p, n = int(rng.integers(10, 41)), int(rng.integers(1000, 5001))
+            codes, dims = syn.discrete_bn_device(p, n, 2.0, seed=seed * 100 + k)
+            reqs.append(Request("discrete", n, p, max_level=3, alpha=0.01, codes=codes, dims=dims))
+    torch.cuda.synchronize()
+    return reqs
+
+
+def _alone(req):
+    from gpucsl_amd.serving import run_alone
+
+    st = run_alone(req)
+    torch.cuda.synchronize()
+    return st.results()
+
+
+def _same(served, alone, kind):
+    ok = (served.skeleton == alone.skeleton).all() and (served.separation_sets == alone.separation_sets).all()
+    ok = ok and np.array_equal(served.pmax.view(np.uint32), alone.pmax.view(np.uint32))
+    if kind == "gaussian":
+        ok = ok and np.array_equal(served.zmin.view(np.uint32), alone.zmin.view(np.uint32))
+    return ok
+
+
+def test_served_equals_alone_under_continuous_batching():
+    from gpucsl_amd.serving import SkeletonServer
+
+    reqs = _requests(7, 10, 5)
+    srv = SkeletonServer()
+    done = []
+    # staggered arrivals: a few requests, a few ticks, more requests
+    for k, r in enumerate(reqs):
+        srv.submit(r)
+        if k % 3 == 2:
+            done += srv.tick()
+            done += srv.tick()
+    done += srv.drain()
+    assert len(done) == len(reqs) and srv.idle
+    say(f"served {len(done)} requests in {srv.ticks} ticks, largest Gaussian batch {srv.max_batch}")
+    assert srv.max_batch >= 2
+    by_id = {c.rid: c for c in done}
+    for r in reqs:
+        c = by_id[r.rid]
+        assert c.latency > 0
+        assert _same(c.result, _alone(r), r.kind), f"request {r.rid} ({r.kind} p={r.p} n={r.n}) differs"
+
+
+def test_served_matches_oracle_small():
+    """Served Gaussian and discrete results against the FP64 oracle."""
+    from gpucsl_amd.serving import Request, SkeletonServer
+
+    d = syn.gaussian_sem(40, 2000, 3.0, seed=3)
+    X = d.data.astype(np.float32)
+    b = syn.discrete_bn(15, 3000, 2.0, seed=4)
+    dims = (b.data.max(axis=0) + 1).astype(np.uint8)
+    codes = torch.from_numpy(np.ascontiguousarray(b.data.T)).cuda()
+    srv = SkeletonServer()
+    srv.submit(Request("gaussian", 2000, 40, 3, 0.01, X=torch.from_numpy(X).cuda()))
+    srv.submit(Request("discrete", 3000, 15, 2, 0.05, codes=codes, dims=torch.from_numpy(dims).cuda()))
+    out = {c.kind: c for c in srv.drain()}
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), 2000, 0.01, 3)
+    g = out["gaussian"].result
+    assert (g.skeleton == ref["skeleton"]).all() and (g.separation_sets == ref["sepsets"]).all()
+    rd = oracle.disc_skeleton(np.ascontiguousarray(b.data.T), dims, 0.05, 2)
+    dd = out["discrete"].result
+    assert (dd.skeleton == rd["skeleton"]).all() and (dd.separation_sets == rd["sepsets"]).all()
+    for l, st in enumerate(dd.stats):
+        assert st["executed"] == rd["executed"][l]
+
+
+def test_admission_by_footprint_serialises_and_rejects():
+    from gpucsl_amd.cli.cli_util import GpucslError
+    from gpucsl_amd.serving import Request, SkeletonServer, request_footprint
+
+    reqs = _requests(11, 4, 2)
+    biggest = max(request_footprint(r) for r in reqs)
+    srv = SkeletonServer(hbm_budget=biggest)      # room for about one job at a time
+    for r in reqs:
+        srv.submit(r)
+    peak = 0
+    done = []
+    while not srv.idle:
+        got = srv.tick()
+        done += got
+        peak = max(peak, srv.used)
+        assert srv.used <= srv.budget
+        if not got and not srv.launched:
+            srv.wait()
+    assert len(done) == len(reqs)
+    for c in done:
+        r = next(x for x in reqs if x.rid == c.rid)
+        assert _same(c.result, _alone(r), r.kind)
+    big = Request("gaussian", 100, 2000, 3, 0.01, X=torch.zeros((100, 2000), device="cuda"))
+    with pytest.raises(GpucslError):
+        srv.submit(big)

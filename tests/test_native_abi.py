@@ -66,3 +66,29 @@ def test_job_struct_layout_matches_header():
     names = [ln.split(";")[0].replace("*", " ").split()[-1].split("[")[0]
              for ln in body.splitlines()[1:] if ";" in ln]
     assert names == [f[0] for f in GaussJob._fields_]
+
+
+def test_level_tuning_roundtrip_and_validation():
+    """Launch shapes are host state of the library: settable per level or for
+    every level, range-checked, readable back (no GPU needed)."""
+    from gpucsl_amd import _native as nat
+    from gpucsl_amd.cli.cli_util import GpucslError
+    from gpucsl_amd.tune import get_level, set_level
+
+    nat.load()
+    before = {lv: get_level(lv) for lv in range(1, 9)}
+    try:
+        shape = dict(small_team_iters=2, wave_team_iters=4, fast_blocks_per_cu=8, fast_lds_bytes=16384)
+        set_level(3, shape)
+        assert get_level(3) == shape
+        set_level(0, dict(shape, fast_blocks_per_cu=2))
+        assert all(get_level(lv)["fast_blocks_per_cu"] == 2 for lv in range(1, 9))
+        for bad in (dict(shape, fast_lds_bytes=70000), dict(shape, small_team_iters=0),
+                    dict(shape, fast_lds_bytes=5000)):
+            with pytest.raises(GpucslError):
+                set_level(1, bad)
+        with pytest.raises(GpucslError):
+            set_level(9, shape)
+    finally:
+        for lv, sh in before.items():
+            set_level(lv, sh)
