@@ -39,12 +39,22 @@ constexpr int FB_MAX_LDS_BYTES = 65536;
 // whole 256-thread block.
 struct Team {
     int gs_min, small_max, wave_max;  // count limits of the gs_min and wave teams
+    int item_tests;                   // subset tests a work item aims at
 };
 
 __host__ __device__ __forceinline__ int team_lanes(uint64_t count, const Team &t) {
     if (count <= (uint64_t)t.small_max) return t.gs_min;
     if (count <= (uint64_t)t.wave_max) return 64;
     return 256;
+}
+
+// Pairs per work item: enough for every team of the block, and about
+// item_tests subset tests in all.
+__host__ __device__ __forceinline__ int items_pairs(uint64_t count, const Team &t) {
+    const int teams = 256 / team_lanes(count, t);
+    const uint64_t want = count ? ((uint64_t)t.item_tests + count - 1) / count : 1;
+    const int w = want > (1u << 20) ? (1 << 20) : (int)want;
+    return w > teams ? w : teams;
 }
 
 // per-level launch shapes of the FAST kernel (settable at run time, e.g. from
@@ -54,21 +64,22 @@ static bool g_tuning_init = false;
 
 static void tuning_defaults() {
     if (g_tuning_init) return;
-    for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = gpucsl_level_tuning{8, 8, 4, 32768};
+    for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = gpucsl_level_tuning{8, 8, 4, 32768, 4096};
     g_tuning_init = true;
 }
 
 static Team team_for(int level, int gs_min) {
     tuning_defaults();
     const gpucsl_level_tuning &t = g_tuning[level];
-    return Team{gs_min, gs_min * t.small_team_iters, 64 * t.wave_team_iters};
+    return Team{gs_min, gs_min * t.small_team_iters, 64 * t.wave_team_iters, t.item_tests};
 }
 
 int set_level_tuning(int level, const gpucsl_level_tuning *t) {
     tuning_defaults();
     if (!t) return set_error(ERR_ARG, "tuning: null");
     if (t->small_team_iters < 1 || t->wave_team_iters < 1 || t->fast_blocks_per_cu < 1 || t->fast_blocks_per_cu > 32 ||
-        t->fast_lds_bytes < 4096 || t->fast_lds_bytes > FB_MAX_LDS_BYTES || t->fast_lds_bytes % 1024)
+        t->fast_lds_bytes < 4096 || t->fast_lds_bytes > FB_MAX_LDS_BYTES || t->fast_lds_bytes % 1024 ||
+        t->item_tests < 64 || t->item_tests > (1 << 24))
         return set_error(ERR_ARG, "tuning: a field is out of range");
     if (level == 0) {
         for (int l = 1; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = *t;
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         const int d = w.deg[r];
         const uint64_t count = active ? binom(d - 1, level) : 0;
         if (count) {
-            const int ppi = 256 / team_lanes(count, team);
+            const int ppi = items_pairs(count, team);
             lia += (ca + ppi - 1) / ppi;
             lib += (cb + ppi - 1) / ppi;
         }
@@ -282,7 +293,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         eb += cb;
         const uint64_t count = active ? binom(w.deg[r] - 1, level) : 0;
         if (count) {
-            const int ppi = 256 / team_lanes(count, team);
+            const int ppi = items_pairs(count, team);
             eia += (ca + ppi - 1) / ppi;
             eib += (cb + ppi - 1) / ppi;
         }
@@ -367,7 +378,7 @@ __global__ __launch_bounds__(256) void items_kernel(const Job *__restrict__ jobs
     JobWs w = carve_job(J.ws, p);
     const uint64_t count = binom(w.deg[r] - 1, level);
     if (!count) return;
-    const int ppi = 256 / team_lanes(count, team);
+    const int ppi = items_pairs(count, team);
     const int na = w.itemoffA[r + 1] - w.itemoffA[r], nb = w.itemoffB[r + 1] - w.itemoffB[r];
     WorkItem *oa = itemsA + baseA[b] + w.itemoffA[r];
     WorkItem *ob = itemsB + baseB[b] + w.itemoffB[r];
@@ -383,8 +394,8 @@ __global__ __launch_bounds__(256) void items_kernel(const Job *__restrict__ jobs
 // variance), which scales the rounding error of H.
 // ----------------------------------------------------------------------------
 template <int L>
-__device__ __forceinline__ float z32_from_values(float cij, const float (&ci)[L], const float (&cj)[L],
-                                                 float (&c)[L][L], float &dmin, float &rho) {
+__device__ __forceinline__ void rho_from_values(float cij, const float (&ci)[L], const float (&cj)[L],
+                                                float (&c)[L][L], float &dmin, float &rho) {
     float rinv[L];
     dmin = 3.0e38f;
 #pragma unroll
@@ -420,8 +431,26 @@ __device__ __forceinline__ float z32_from_values(float cij, const float (&ci)[L]
     }
     dmin = fminf(dmin, fminf(h00, h11));
     rho = h01 * rsqrtf(fabsf(h00) * fabsf(h11));
+}
+
+template <int L>
+__device__ __forceinline__ float z32_from_values(float cij, const float (&ci)[L], const float (&cj)[L],
+                                                 float (&c)[L][L], float &dmin, float &rho) {
+    rho_from_values<L>(cij, ci, cj, c, dmin, rho);
     return fabsf(atanhf(rho));
 }
+
+// error bound of the FP32 partial correlation: rounding of C32 and of the
+// (L+2)-step elimination, amplified by the smallest pivot
+template <int L>
+__device__ __forceinline__ float rho_error(float dmin) {
+    return EPS32 * (float)(L + 2) * (1.0f + __frcp_rn(fmaxf(dmin, 1e-30f)));
+}
+
+__device__ __forceinline__ bool rho_bad(float dmin, float ar) { return !(dmin > 1e-5f) || !(ar < 1.0f); }
+
+// Fisher z of a non-negative |rho| (+inf for none counted)
+__device__ __forceinline__ float z_of_rho(float ar) { return ar < 1.0f ? atanhf(ar) : __builtin_inff(); }
 
 template <int L>
 __device__ __forceinline__ float z32_test(const float *__restrict__ C, int p, int i, int j,
@@ -451,31 +480,32 @@ __device__ __forceinline__ bool z32_bad(float z, float dmin, float rho) {
 }
 
 // ----------------------------------------------------------------------------
-// level l >= 1, FAST phase.  A block of 256 threads takes one work item: the
-// pairs q0.. of row i on this side (j > i on side A, j < i on side B).  The
+// level l >= 1, FAST phase.  A block of 256 threads takes one work item: pairs
+// q0..q1-1 of row i on this side (j > i on side A, j < i on side B).  The
 // pairs of a row share the subset candidates adj(i)\{j}, C(deg-1, l) per pair,
 // so the lanes that test one pair together (a "team") are sized by that count:
-// gs_min lanes for short enumerations (256/gs_min pairs per item), 64 lanes
-// (4 pairs per item) up to 512 subsets, the whole block (1 pair) beyond.  A
-// team walks the pair's subset ranks in increasing order and stops at the first
-// passing test, so the removing sepset is the first in pcalg order and zmin
-// covers exactly the tests up to it.  What the item's tests read is staged in
-// LDS by the whole block:
-//   FULL      (l >= 2) C[A, A] with A = adj(i) + [i]: every test reads LDS only;
-//   PAIRROWS  neighbour ids, C[i, adj(i)], diag C[s, s] and C[j, adj(i)] for the
-//             item's pairs; level 1 then reads LDS only, levels >= 2 read the
-//             off-diagonal C[S, S] from global memory (L2);
-//   GLOBAL    nothing staged (rows beyond the LDS budget).
-// A test within the FP32 error margin of the threshold sends the pair, from
-// the start of that iteration, to the FP64 queue.
+// teams of tm.gs_min lanes or of one wave pull the item's pairs one by one from
+// an LDS counter; beyond that the whole block works through the pairs in turn.
+// A team walks the pair's subset ranks in increasing order and stops at the
+// first passing test, so the removing sepset is the first in pcalg order and
+// zmin covers exactly the tests up to it.  What the tests read is staged in LDS
+// by the whole block:
+//   FULL  (l >= 2) C[A, A] with A = adj(i) + [i]: the tests read LDS only;
+//   ROW   neighbour ids, C[i, adj(i)] and diag C[s, s]; C[j, S] and the
+//         off-diagonal C[S, S] come from global memory (L2);
+//   none  (rows beyond the LDS budget) everything from global memory.
+// Decisions are taken on the partial correlation rho itself: a test passes
+// when |rho| < tanh(threshold), and one within margin * (its FP32 error bound)
+// of that value sends the pair, from the start of that iteration, to the FP64
+// queue.  The smallest |rho| a team counts becomes zmin = atanh(min |rho|).
 // gpucsl/pc/cuda/gaussian_ci.cu:66-120, set_helpers.cu:25-71,
 // graph_helpers.cu:81-133.
 // ----------------------------------------------------------------------------
 template <int L>
 __device__ __forceinline__ void gather_test(int mode, const float *__restrict__ sf, const int *__restrict__ snb,
-                                            int d, int D, int jpos, int g, const float *__restrict__ C32, int p,
-                                            int i, int j, const int32_t *__restrict__ gnb, const int (&sp)[L],
-                                            float &cij, float (&ci)[L], float (&cj)[L], float (&cm)[L][L]) {
+                                            int d, int D, int jpos, const float *__restrict__ C32, int p, int i, int j,
+                                            const int32_t *__restrict__ gnb, const int (&sp)[L], float &cij,
+                                            float (&ci)[L], float (&cj)[L], float (&cm)[L][L]) {
     if (mode == 0) {
         cij = sf[d * D + jpos];
 #pragma unroll
@@ -486,14 +516,18 @@ __device__ __forceinline__ void gather_test(int mode, const float *__restrict__ 
             for (int b = 0; b <= a; ++b) cm[a][b] = sf[sp[a] * D + sp[b]];
         }
     } else if (mode == 1) {
-        const float *rowI = sf, *diag = sf + d, *prow = sf + (size_t)(2 + g) * d;
+        const float *rowI = sf, *diag = sf + d;
+        const float *rowJ = C32 + (size_t)j * p;
         cij = rowI[jpos];
+        int S[L];
+#pragma unroll
+        for (int a = 0; a < L; ++a) S[a] = snb[sp[a]];
 #pragma unroll
         for (int a = 0; a < L; ++a) {
             ci[a] = rowI[sp[a]];
-            cj[a] = prow[sp[a]];
+            cj[a] = rowJ[S[a]];
 #pragma unroll
-            for (int b = 0; b < a; ++b) cm[a][b] = C32[(size_t)snb[sp[a]] * p + snb[sp[b]]];
+            for (int b = 0; b < a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
             cm[a][a] = diag[sp[a]];
         }
     } else {
@@ -511,6 +545,44 @@ __device__ __forceinline__ void gather_test(int mode, const float *__restrict__ 
     }
 }
 
+// One FP32 test in rho space: |rho|, and whether it is within the margin of
+// the threshold rthr (or degenerate).
+template <int L>
+__device__ __forceinline__ float rho_test(int mode, const float *__restrict__ sf, const int *__restrict__ snb, int d,
+                                          int D, int jpos, const float *__restrict__ C32, int p, int i, int j,
+                                          const int32_t *__restrict__ gnb, const int (&sp)[L], float rthr,
+                                          float margin_k, bool &near) {
+    float cij, ci[L], cj[L], cm[L][L];
+    gather_test<L>(mode, sf, snb, d, D, jpos, C32, p, i, j, gnb, sp, cij, ci, cj, cm);
+    float dmin, rho;
+    rho_from_values<L>(cij, ci, cj, cm, dmin, rho);
+    const float ar = fabsf(rho);
+    near = rho_bad(dmin, ar) || fabsf(ar - rthr) <= margin_k * rho_error<L>(dmin);
+    return ar;
+}
+
+template <int L>
+__device__ __forceinline__ void write_sepset(const Job &J, int p, int i, int j, int mode,
+                                             const int *__restrict__ snb, const int32_t *__restrict__ gnb,
+                                             const int (&sp)[L]) {
+    J.G[(size_t)i * p + j] = 0;
+    J.G[(size_t)j * p + i] = 0;
+    int32_t *sep = J.sepsets;
+    const int maxL = J.max_level;
+#pragma unroll
+    for (int c = 0; c < L; ++c) {
+        const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
+        sep[((size_t)i * p + j) * maxL + c] = v;
+        sep[((size_t)j * p + i) * maxL + c] = v;
+    }
+}
+
+__device__ __forceinline__ void update_zmin(const Job &J, int p, int i, int j, float z) {
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    float *zp = J.zmin + (size_t)a * p + b;
+    if (z < *zp) *zp = z;
+}
+
 template <int L>
 __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
     const Job *__restrict__ jobs, int nj, const int32_t *__restrict__ itembase, const WorkItem *__restrict__ items,
@@ -518,7 +590,8 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
     unsigned long long *__restrict__ bstats) {
     extern __shared__ int lds_words[];
     __shared__ unsigned long long s_pm[4], s_nm[4];
-    __shared__ float s_z[4];
+    __shared__ float s_r[4];
+    __shared__ int s_next;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nitems = itembase[nj];
     unsigned long long n_exec = 0, n_useful = 0;
@@ -530,31 +603,24 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         JobWs jw = carve_job(J.ws, p);
         const int32_t *__restrict__ rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
         const float *__restrict__ C32 = J.C32;
-        uint8_t *__restrict__ Gm = J.G;
-        const float thr32 = (float)J.thr[L];
+        const uint8_t *__restrict__ Gm = J.G;
+        const float rthr = (float)tanh(J.thr[L]);
         const int d = jw.deg[i];
         const int cnt = rowoff[i + 1] - rowoff[i];
         const int Nn = d - 1;
         const uint64_t count = binom_k<L>(Nn);
         const int team = team_lanes(count, tm);
-        const int ppi = 256 / team;
         const int q0 = it.q0;
-        const int q1 = min(cnt, q0 + ppi);
-        const int npairs = q1 - q0;
+        const int q1 = min(cnt, q0 + items_pairs(count, tm));
         const int jbase = side == 0 ? d - cnt : 0;
         const int D = d + 1;
-        int mode;
-        if (L >= 2 && D * D + d <= WORDS)
-            mode = 0;
-        else if (d * (2 + npairs) + d <= WORDS)
-            mode = 1;
-        else
-            mode = 2;
+        const int mode = (L >= 2 && D * D + d <= WORDS) ? 0 : (3 * d <= WORDS ? 1 : 2);
         const int32_t *gnb = jw.nbr + (size_t)i * p;
         int *snb = lds_words;
         float *sf = reinterpret_cast<float *>(lds_words + d);
 
         __syncthreads();  // the previous item is done with LDS
+        if (tid == 0) s_next = q0;
         if (mode != 2)
             for (int k = tid; k < d; k += 256) snb[k] = gnb[k];
         __syncthreads();
@@ -603,40 +669,104 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
                     }
                 }
             }
-            const int total = npairs * d;
-            for (int t0 = 0; t0 < total; t0 += 256 * 4) {
-                float v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int t = t0 + u * 256 + tid;
-                    float x = 0.0f;
-                    if (t < total) {
-                        const int g = t / d, k = t - g * d;
-                        const int j = snb[jbase + q0 + g];
-                        x = C32[(size_t)j * p + snb[k]];
-                    }
-                    v[u] = x;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int t = t0 + u * 256 + tid;
-                    if (t < total) sf[2 * d + t] = v[u];
-                }
-            }
         }
         __syncthreads();
 
         if (team == 256) {
-            // ---- the whole block tests pair q0
-            const int jpos = jbase + q0;
+            // ---- the whole block works through the item's pairs in turn
+            for (int q = q0; q < q1; ++q) {
+                const int jpos = jbase + q;
+                const int j = mode == 2 ? gnb[jpos] : snb[jpos];
+                if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
+                float rl = __builtin_inff();  // min |rho| over the tests counted so far
+                for (uint64_t iter = 0;; ++iter) {
+                    const uint64_t rank = iter * 256 + tid;
+                    const bool act = rank < count;
+                    bool pass = false, near = false;
+                    float ar = __builtin_inff();
+                    int sp[L];
+                    if (act) {
+                        int pos[L];
+                        unrank_lex<L>(Nn, rank, pos);
+#pragma unroll
+                        for (int c = 0; c < L; ++c) {
+                            const int idx = pos[c] - 1;
+                            sp[c] = idx + (idx >= jpos);
+                        }
+                        ar = rho_test<L>(mode, sf, snb, d, D, jpos, C32, p, i, j, gnb, sp, rthr, margin_k, near);
+                        pass = ar < rthr;
+                    }
+                    const unsigned long long pm = __ballot(pass), nm = __ballot(near);
+                    if (lane == 0) {
+                        s_pm[wave] = pm;
+                        s_nm[wave] = nm;
+                    }
+                    __syncthreads();
+                    const bool any_near = (s_nm[0] | s_nm[1] | s_nm[2] | s_nm[3]) != 0;
+                    int P = -1;
+#pragma unroll
+                    for (int w2 = 0; w2 < 4; ++w2)
+                        if (P < 0 && s_pm[w2]) P = w2 * 64 + __ffsll(s_pm[w2]) - 1;
+                    bool finish;
+                    if (any_near) {
+                        finish = true;
+                    } else {
+                        if (act) ++n_exec;
+                        if (P >= 0) {
+                            if (act && tid <= P) {
+                                ++n_useful;
+                                rl = fminf(rl, ar);
+                            }
+                            if (tid == P) write_sepset<L>(J, p, i, j, mode, snb, gnb, sp);
+                            finish = true;
+                        } else {
+                            if (act) {
+                                ++n_useful;
+                                rl = fminf(rl, ar);
+                            }
+                            finish = (iter + 1) * 256 >= count;
+                        }
+                    }
+                    if (finish) {
+                        float m = rl;
+                        for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+                        if (lane == 0) s_r[wave] = m;
+                        __syncthreads();
+                        if (tid == 0) {
+                            const float zb = z_of_rho(fminf(fminf(s_r[0], s_r[1]), fminf(s_r[2], s_r[3])));
+                            if (any_near) {
+                                const unsigned int slot = atomicAdd(qcount, 1u);
+                                queue[slot] = SlowItem{iter * 256, rowoff[i] + q, it.job, zb, 0};
+                            } else {
+                                update_zmin(J, p, i, j, zb);
+                            }
+                        }
+                    }
+                    __syncthreads();  // s_pm / s_nm / s_r are reused
+                    if (finish) break;
+                }
+            }
+            continue;
+        }
+
+        // ---- teams of `team` lanes inside a wave pull pairs from s_next
+        const int gbase = lane & ~(team - 1);
+        const int gl = lane & (team - 1);
+        const uint64_t gmask = team == 64 ? ~0ull : (((1ull << team) - 1ull) << gbase);
+        while (true) {
+            int q = 0;
+            if (gl == 0) q = atomicAdd(&s_next, 1);
+            q = __shfl(q, gbase);
+            if (q >= q1) break;
+            const int jpos = jbase + q;
             const int j = mode == 2 ? gnb[jpos] : snb[jpos];
             if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
-            float zl = __builtin_inff();
+            float rl = __builtin_inff();
             for (uint64_t iter = 0;; ++iter) {
-                const uint64_t rank = iter * 256 + tid;
+                const uint64_t rank = iter * team + gl;
                 const bool act = rank < count;
                 bool pass = false, near = false;
-                float zf = __builtin_inff();
+                float ar = __builtin_inff();
                 int sp[L];
                 if (act) {
                     int pos[L];
@@ -646,159 +776,40 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
                         const int idx = pos[c] - 1;
                         sp[c] = idx + (idx >= jpos);
                     }
-                    float cij, ci[L], cj[L], cm[L][L];
-                    gather_test<L>(mode, sf, snb, d, D, jpos, 0, C32, p, i, j, gnb, sp, cij, ci, cj, cm);
-                    float dmin, rho;
-                    const float z = z32_from_values<L>(cij, ci, cj, cm, dmin, rho);
-                    near = z32_bad(z, dmin, rho) || fabsf(z - thr32) <= margin_k * z32_error<L>(dmin, rho);
-                    pass = z < thr32;
-                    zf = z;
+                    ar = rho_test<L>(mode, sf, snb, d, D, jpos, C32, p, i, j, gnb, sp, rthr, margin_k, near);
+                    pass = ar < rthr;
                 }
-                const unsigned long long pm = __ballot(pass), nm = __ballot(near);
-                if (lane == 0) {
-                    s_pm[wave] = pm;
-                    s_nm[wave] = nm;
+                if (__ballot(near) & gmask) {
+                    for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
+                    if (gl == 0) {
+                        const unsigned int slot = atomicAdd(qcount, 1u);
+                        queue[slot] = SlowItem{iter * team, rowoff[i] + q, it.job, z_of_rho(rl), 0};
+                    }
+                    break;
                 }
-                __syncthreads();
-                const bool any_near = (s_nm[0] | s_nm[1] | s_nm[2] | s_nm[3]) != 0;
-                int P = -1;
-#pragma unroll
-                for (int w2 = 0; w2 < 4; ++w2)
-                    if (P < 0 && s_pm[w2]) P = w2 * 64 + __ffsll(s_pm[w2]) - 1;
+                if (act) ++n_exec;
+                const uint64_t pm = __ballot(pass) & gmask;
                 bool finish;
-                bool write_zmin = false;
-                if (any_near) {
+                if (pm) {
+                    const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
+                    if (act && gl <= P) {
+                        ++n_useful;
+                        rl = fminf(rl, ar);
+                    }
+                    if (gl == P) write_sepset<L>(J, p, i, j, mode, snb, gnb, sp);
                     finish = true;
                 } else {
-                    if (act) ++n_exec;
-                    if (P >= 0) {
-                        if (act && tid <= P) {
-                            ++n_useful;
-                            zl = fminf(zl, zf);
-                        }
-                        if (tid == P) {
-                            Gm[(size_t)i * p + j] = 0;
-                            Gm[(size_t)j * p + i] = 0;
-                            int32_t *sep = J.sepsets;
-                            const int maxL = J.max_level;
-#pragma unroll
-                            for (int c = 0; c < L; ++c) {
-                                const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
-                                sep[((size_t)i * p + j) * maxL + c] = v;
-                                sep[((size_t)j * p + i) * maxL + c] = v;
-                            }
-                        }
-                        finish = true;
-                    } else {
-                        if (act) {
-                            ++n_useful;
-                            zl = fminf(zl, zf);
-                        }
-                        finish = (iter + 1) * 256 >= count;
+                    if (act) {
+                        ++n_useful;
+                        rl = fminf(rl, ar);
                     }
-                    write_zmin = finish;
+                    finish = (iter + 1) * team >= count;
                 }
                 if (finish) {
-                    float m = zl;
-                    for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-                    if (lane == 0) s_z[wave] = m;
-                    __syncthreads();
-                    if (tid == 0) {
-                        const float zb = fminf(fminf(s_z[0], s_z[1]), fminf(s_z[2], s_z[3]));
-                        if (any_near) {
-                            const unsigned int slot = atomicAdd(qcount, 1u);
-                            queue[slot] = SlowItem{iter * 256, rowoff[i] + q0, it.job, zb, 0};
-                        } else if (write_zmin) {
-                            const int a = i < j ? i : j, b = i < j ? j : i;
-                            float *zp = J.zmin + (size_t)a * p + b;
-                            if (zb < *zp) *zp = zb;
-                        }
-                    }
+                    for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
+                    if (gl == 0) update_zmin(J, p, i, j, z_of_rho(rl));
+                    break;
                 }
-                __syncthreads();  // s_pm / s_nm / s_z are reused
-                if (finish) break;
-            }
-            continue;
-        }
-
-        // ---- teams of `team` lanes inside a wave, one pair each
-        const int g = tid / team;
-        const int q = q0 + g;
-        if (q >= q1) continue;
-        const int gbase = lane & ~(team - 1);
-        const int gl = lane & (team - 1);
-        const uint64_t gmask = team == 64 ? ~0ull : (((1ull << team) - 1ull) << gbase);
-        const int jpos = jbase + q;
-        const int j = mode == 2 ? gnb[jpos] : snb[jpos];
-        if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
-        float zl = __builtin_inff();
-        for (uint64_t iter = 0;; ++iter) {
-            const uint64_t rank = iter * team + gl;
-            const bool act = rank < count;
-            bool pass = false, near = false;
-            float zf = __builtin_inff();
-            int sp[L];
-            if (act) {
-                int pos[L];
-                unrank_lex<L>(Nn, rank, pos);
-#pragma unroll
-                for (int c = 0; c < L; ++c) {
-                    const int idx = pos[c] - 1;
-                    sp[c] = idx + (idx >= jpos);
-                }
-                float cij, ci[L], cj[L], cm[L][L];
-                gather_test<L>(mode, sf, snb, d, D, jpos, g, C32, p, i, j, gnb, sp, cij, ci, cj, cm);
-                float dmin, rho;
-                const float z = z32_from_values<L>(cij, ci, cj, cm, dmin, rho);
-                near = z32_bad(z, dmin, rho) || fabsf(z - thr32) <= margin_k * z32_error<L>(dmin, rho);
-                pass = z < thr32;
-                zf = z;
-            }
-            if (__ballot(near) & gmask) {
-                for (int off = team / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
-                if (gl == 0) {
-                    const unsigned int slot = atomicAdd(qcount, 1u);
-                    queue[slot] = SlowItem{iter * team, rowoff[i] + q, it.job, zl, 0};
-                }
-                break;
-            }
-            if (act) ++n_exec;
-            const uint64_t pm = __ballot(pass) & gmask;
-            bool finish;
-            if (pm) {
-                const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
-                if (act && gl <= P) {
-                    ++n_useful;
-                    zl = fminf(zl, zf);
-                }
-                if (gl == P) {
-                    Gm[(size_t)i * p + j] = 0;
-                    Gm[(size_t)j * p + i] = 0;
-                    int32_t *sep = J.sepsets;
-                    const int maxL = J.max_level;
-#pragma unroll
-                    for (int c = 0; c < L; ++c) {
-                        const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
-                        sep[((size_t)i * p + j) * maxL + c] = v;
-                        sep[((size_t)j * p + i) * maxL + c] = v;
-                    }
-                }
-                finish = true;
-            } else {
-                if (act) {
-                    ++n_useful;
-                    zl = fminf(zl, zf);
-                }
-                finish = (iter + 1) * team >= count;
-            }
-            if (finish) {
-                for (int off = team / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
-                if (gl == 0) {
-                    const int a = i < j ? i : j, b = i < j ? j : i;
-                    float *zp = J.zmin + (size_t)a * p + b;
-                    if (zl < *zp) *zp = zl;
-                }
-                break;
             }
         }
     }
@@ -857,7 +868,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
     int p = 0, i = 0, j = 0, jpos = 0, Nn = 0;
     const int32_t *gnb = nullptr;
     double thr = 0.0;
-    float thr32 = 0.0f;
+    float rthr = 0.0f;
     uint64_t count = 0, next = 0;
     float zl = __builtin_inff();
     bool have = false, done = false;
@@ -905,7 +916,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 next = qi.start;
                 zl = gl == 0 ? qi.zl : __builtin_inff();
                 thr = J->thr[L];
-                thr32 = (float)thr;
+                rthr = (float)tanh(thr);
                 have = count > 0;
             }
         }
@@ -926,21 +937,22 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 S[c] = gnb[idx + (idx >= jpos)];
             }
             float dmin, rho;
-            const float z = z32_test<L>(J->C32, p, i, j, S, dmin, rho);
-            const float err = z32_error<L>(dmin, rho);
-            const bool bad = z32_bad(z, dmin, rho);
-            if (bad || fabsf(z - thr32) <= margin_k * err) {
+            z32_test<L>(J->C32, p, i, j, S, dmin, rho);
+            const float ar = fabsf(rho);
+            const float err = rho_error<L>(dmin);
+            const bool bad = rho_bad(dmin, ar);
+            if (bad || fabsf(ar - rthr) <= margin_k * err) {
                 const double z64 = ref64::gauss_z<L>(J->C64, p, i, j, S);
                 pass = z64 < thr;
                 zf = (float)z64;
                 ++n_re;
                 if (!bad) {
-                    n_flip += (pass != (z < thr32));
-                    mr = fmaxf(mr, fabsf(z - zf) / err);
+                    n_flip += (pass != (ar < rthr));
+                    mr = fmaxf(mr, (float)(fabs((double)ar - tanh(z64)) / (double)err));
                 }
             } else {
-                pass = z < thr32;
-                zf = z;
+                pass = ar < rthr;
+                zf = z_of_rho(ar);
             }
             ++n_exec;
         }

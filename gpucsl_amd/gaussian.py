@@ -135,6 +135,30 @@ class GaussianEngine:
                 on_level(l)
         self.finish()
 
+    def capture(self, X=None) -> None:
+        """Capture set_data(X) (when X is given) + run() into one HIP graph;
+        replay() then repeats the whole skeleton with one launch.  Buffers
+        and X stay bound to the graph."""
+        torch = self.torch
+        if X is not None:
+            self.set_data(X)
+        self.run()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            g.capture_begin()
+            if X is not None:
+                self.set_data(X)
+            self.run()
+            g.capture_end()
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph = g
+
+    def replay(self) -> None:
+        self._graph.replay()
+
     def finish(self) -> None:
         s = nat.stream_handle()
         nat.call("gpucsl_pmax_from_zmin", self.zmin.data_ptr(), self.p, self.n, self.pmax.data_ptr(), s)

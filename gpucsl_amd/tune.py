@@ -29,8 +29,9 @@ SPACE = {
     "wave_team_iters": (2, 4, 8),
     "fast_blocks_per_cu": (2, 4, 8),
     "fast_lds_bytes": (16384, 32768, 65536),
+    "item_tests": (1024, 4096, 16384),
 }
-FIELDS = ("small_team_iters", "wave_team_iters", "fast_blocks_per_cu", "fast_lds_bytes")
+FIELDS = ("small_team_iters", "wave_team_iters", "fast_blocks_per_cu", "fast_lds_bytes", "item_tests")
 
 
 class LevelTuning(ct.Structure):
@@ -49,8 +50,11 @@ def get_level(level: int) -> dict:
 
 
 def apply_table(table: dict) -> None:
+    """Apply a tuning table; fields a table lacks keep their current values."""
     for lv, shape in table.get("levels", {}).items():
-        set_level(int(lv), shape)
+        cur = get_level(int(lv))
+        cur.update({f: shape[f] for f in FIELDS if f in shape})
+        set_level(int(lv), cur)
 
 
 def _level_ms(timings: dict, level: int, reps: int) -> float:

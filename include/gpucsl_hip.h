@@ -168,13 +168,15 @@ int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss
  * lanes that test one pair together are group_size while C(deg-1, l) <=
  * group_size * small_team_iters, 64 while it is <= 64 * wave_team_iters, else
  * 256; fast_blocks_per_cu and fast_lds_bytes (4096..65536, a multiple of 1024)
- * size the grid and its LDS staging.  Results do not depend on these values.
- * level 0 sets every level.  Defaults: {8, 8, 4, 32768}. */
+ * size the grid and its LDS staging; a work item holds about item_tests
+ * (64..2^24) subset tests.  Results do not depend on these values.  level 0
+ * sets every level.  Defaults: {8, 8, 4, 32768, 4096}. */
 typedef struct gpucsl_level_tuning {
     int32_t small_team_iters;
     int32_t wave_team_iters;
     int32_t fast_blocks_per_cu;
     int32_t fast_lds_bytes;
+    int32_t item_tests;
 } gpucsl_level_tuning;
 int gpucsl_set_level_tuning(int level, const gpucsl_level_tuning *tuning);
 int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning);

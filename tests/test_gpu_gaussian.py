@@ -220,3 +220,28 @@ def test_dropin_api_shapes():
     ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), 1000, 0.05, 3)
     assert (res.skeleton == ref["skeleton"]).all()
     assert (res.seperation_sets == ref["sepsets"]).all()
+
+
+def test_single_dataset_graph_replay_matches_eager():
+    """The whole single-dataset skeleton captured as one HIP graph: every
+    replay, with other GPU work between replays, equals the eager run."""
+    from gpucsl_amd.gaussian import GaussianEngine
+    from gpucsl_amd.pc.helpers import get_gaussian_thresholds
+
+    d = syn.gaussian_sem(120, 3000, 3.0, seed=77).data.astype(np.float32)
+    X = torch.from_numpy(d).cuda()
+    eng = GaussianEngine(120, 3000, 3, get_gaussian_thresholds(d, 0.01))
+    eng.set_data(X)
+    eng.run()
+    torch.cuda.synchronize()
+    want = eng.results()
+    eng.capture(X)
+    for rep in range(3):
+        junk = (torch.randn(4096, device="cuda") * 2).sum()
+        eng.replay()
+        torch.cuda.synchronize()
+        got = eng.results()
+        assert (got.skeleton == want.skeleton).all() and (got.separation_sets == want.separation_sets).all()
+        assert np.array_equal(got.pmax.view(np.uint32), want.pmax.view(np.uint32))
+        assert [s["useful"] for s in got.stats] == [s["useful"] for s in want.stats]
+        del junk
