@@ -79,7 +79,8 @@ int c64_to_c32_launch(const double *C64, int p, float *C32, hipStream_t s);
 int corr_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStream_t s);
 int c64_to_c32_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStream_t s);
 size_t gauss_workspace_bytes(int p);
-int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, hipStream_t s);
+int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, unsigned long long *st, int nst,
+                      hipStream_t s);
 int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, float margin_k, uint8_t *Gm,
                         float *zmin, unsigned long long *stats, void *ws, size_t ws_bytes, hipStream_t s);
 int gauss_level_launch(int level, const float *C32, const double *C64, int p, int max_level, double thr,
@@ -220,10 +221,8 @@ size_t gpucsl_corr_job_workspace_bytes(int p, int k_splits) {
 
 int gpucsl_gauss_init(int p, int max_level, uint8_t *G, int32_t *sepsets, float *zmin,
                       uint64_t *stats, void *stream) {
-    int rc = gauss_init_launch(p, max_level, G, sepsets, zmin, S(stream));
-    if (rc) return rc;
-    if (stats) launch_fill<uint64_t>(stats, 0ull, (size_t)(max_level + 1) * GPUCSL_STAT_SLOTS, S(stream));
-    return check_hip(hipGetLastError(), "gauss_init stats");
+    return gauss_init_launch(p, max_level, G, sepsets, zmin, reinterpret_cast<unsigned long long *>(stats),
+                             stats ? (max_level + 1) * GPUCSL_STAT_SLOTS : 0, S(stream));
 }
 
 int gpucsl_gauss_level0(const float *C32, const double *C64, int p, double thr0, float margin,

@@ -23,32 +23,49 @@ __host__ __device__ inline int upper_tiles(int p) {
     return t * (t + 1) / 2;
 }
 
-// partial[s][a] = sum of X[k][a] over rows k of split s (sequential in k).
+// Row splits of the column sums: a function of the job's own n, so a job's
+// means are the same bits alone and in any batch.
+__host__ __device__ inline int corr_row_splits(int64_t n) {
+    int64_t rs = (n + 511) / 512;
+    if (rs > CORR_ROW_SPLITS_MAX) rs = CORR_ROW_SPLITS_MAX;
+    return rs < 1 ? 1 : (int)rs;
+}
+
+// partial[s][a] = sum of X[k][a] over the rows k of split s: 64 columns x 4
+// row groups per block (group g takes rows g, g+4, ... of the split), the four
+// group sums added in order.
 template <typename T>
-__global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ jobs, int row_splits) {
+__global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ jobs) {
     const Job &J = jobs[blockIdx.z];
     const int p = J.p;
     const int64_t n = J.n;
-    const int a = blockIdx.x * 256 + threadIdx.x;
+    const int rs = corr_row_splits(n);
     const int s = blockIdx.y;
-    if (a >= p) return;
+    if (s >= rs || (int)blockIdx.x * 64 >= p) return;
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int a = blockIdx.x * 64 + c;
     const T *X = (const T *)J.X;
-    const int64_t per = (n + row_splits - 1) / row_splits;
+    const int64_t per = (n + rs - 1) / rs;
     const int64_t r0 = s * per;
     const int64_t r1 = r0 + per < n ? r0 + per : n;
     double acc = 0.0;
-    for (int64_t k = r0; k < r1; ++k) acc += (double)X[k * p + a];
-    carve_corr(J.corr_ws, p).partial[(size_t)s * p + a] = acc;
-}
-
-__global__ void mean_kernel(const Job *__restrict__ jobs, int row_splits) {
-    const Job &J = jobs[blockIdx.z];
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= J.p) return;
-    CorrWs c = carve_corr(J.corr_ws, J.p);
-    double acc = 0.0;
-    for (int s = 0; s < row_splits; ++s) acc += c.partial[(size_t)s * J.p + a];
-    c.mean[a] = acc / (double)J.n;
+    if (a < p) {
+        int64_t k = r0 + g;
+        for (; k + 12 < r1; k += 16) {
+            const double x0 = (double)X[k * p + a], x1 = (double)X[(k + 4) * p + a];
+            const double x2 = (double)X[(k + 8) * p + a], x3 = (double)X[(k + 12) * p + a];
+            acc += x0;
+            acc += x1;
+            acc += x2;
+            acc += x3;
+        }
+        for (; k < r1; k += 4) acc += (double)X[k * p + a];
+    }
+    __shared__ double part[4][64];
+    part[g][c] = acc;
+    __syncthreads();
+    if (g == 0 && a < p)
+        carve_corr(J.corr_ws, p).partial[(size_t)s * p + a] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
 }
 
 // gram[s] (p x p, only tiles with ta <= tb written) = Zs^T Zs over k-range s,
@@ -68,7 +85,6 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
     if ((int)blockIdx.x >= upper_tiles(p) || (int)blockIdx.y >= J.k_splits) return;
     const T *__restrict__ X = (const T *)J.X;
     CorrWs cw = carve_corr(J.corr_ws, p);
-    const double *__restrict__ mean = cw.mean;
 
     // upper-triangle tile id -> (ta, tb), ta <= tb, row-major over the triangle
     const int tiles = (p + CT - 1) / CT;
@@ -100,12 +116,35 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
+    // column means of the two tile edges: mean[a] = (sum of the row-split
+    // partial sums, in split order) / n
+    __shared__ double s_mean[2][CT];
+    if (tid < 2 * CT) {
+        const int col = (tid < CT ? a0 : b0) + (tid & (CT - 1));
+        double m = 0.0;
+        if (col < p) {
+            const int rs = corr_row_splits(n);
+            const double *pc = cw.partial + col;
+            int q = 0;
+            for (; q + 4 <= rs; q += 4) {
+                const double x0 = pc[(size_t)q * p], x1 = pc[(size_t)(q + 1) * p];
+                const double x2 = pc[(size_t)(q + 2) * p], x3 = pc[(size_t)(q + 3) * p];
+                m += x0;
+                m += x1;
+                m += x2;
+                m += x3;
+            }
+            for (; q < rs; ++q) m += pc[(size_t)q * p];
+            m = m / (double)n;
+        }
+        s_mean[tid >> 6][tid & (CT - 1)] = m;
+    }
+    __syncthreads();
     double ma[2], mb[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const int a = a0 + wr + i * 16 + ci, b = b0 + wc + i * 16 + ci;
-        ma[i] = a < p ? mean[a] : 0.0;
-        mb[i] = b < p ? mean[b] : 0.0;
+        ma[i] = s_mean[0][wr + i * 16 + ci];
+        mb[i] = s_mean[1][wc + i * 16 + ci];
     }
 
     // loader: 256 threads cover KT rows x 64 cols as KT/4 passes of 4 rows
@@ -178,38 +217,80 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
         }
 }
 
-// sd[a] = sqrt(G_aa / (n-1)) with G_aa summed over splits in order.
-__global__ void sd_kernel(const Job *__restrict__ jobs) {
+// C64 and C32 for a quarter (16 rows x 64 columns) of one upper tile
+// (ta <= tb) of 64 x 64 and for its mirror: G_ab = sum of the split-K partial
+// Gram matrices in split order, sd[a] = sqrt(G_aa / (n-1)), and
+// C[a][b] = C[b][a] = clip((G_ab / (n-1)) / sd_lo / sd_hi, -1, 1) with
+// lo = min(a, b): numpy corrcoef's operation order for its upper triangle,
+// mirrored so that C is exactly symmetric; C32 = (float)C64.
+constexpr int NQ = 4;          // row quarters per tile
+constexpr int QR = CT / NQ;    // rows per quarter
+__global__ __launch_bounds__(256) void normalize_tile_kernel(const Job *__restrict__ jobs) {
     const Job &J = jobs[blockIdx.z];
     const int p = J.p;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= p) return;
-    CorrWs c = carve_corr(J.corr_ws, p);
-    double g = 0.0;
-    for (int s = 0; s < J.k_splits; ++s) g += c.gram[(size_t)s * p * p + (size_t)a * p + a];
-    c.sd[a] = sqrt(g / (double)(J.n - 1));
-}
-
-// C64[a][b] = clip((G_ab / (n-1)) / sd_lo / sd_hi, -1, 1) with lo = min(a, b):
-// numpy corrcoef's operation order for its upper triangle, mirrored so that C is
-// exactly symmetric; C32 = (float)C64.
-__global__ __launch_bounds__(256) void normalize_kernel(const Job *__restrict__ jobs) {
-    const Job &J = jobs[blockIdx.z];
-    const int p = J.p;
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    const int a = blockIdx.y;
-    if (b >= p || a >= p) return;
-    CorrWs cw = carve_corr(J.corr_ws, p);
-    const int lo = a < b ? a : b, hi = a < b ? b : a;
-    double g = 0.0;
-    for (int s = 0; s < J.k_splits; ++s) g += cw.gram[(size_t)s * p * p + (size_t)lo * p + hi];
-    double c = g / (double)(J.n - 1);
-    c = c / cw.sd[lo];
-    c = c / cw.sd[hi];
-    if (c > 1.0) c = 1.0;
-    if (c < -1.0) c = -1.0;
-    if (J.C64) J.C64[(size_t)a * p + b] = c;
-    J.C32[(size_t)a * p + b] = (float)c;
+    const int quarter = blockIdx.x % NQ;
+    if ((int)(blockIdx.x / NQ) >= upper_tiles(p)) return;
+    const int tiles = (p + CT - 1) / CT;
+    int t = blockIdx.x / NQ, ta = 0;
+    while (t >= tiles - ta) {
+        t -= tiles - ta;
+        ++ta;
+    }
+    const int tb = ta + t;
+    const int a0 = ta * CT + quarter * QR, b0 = tb * CT;
+    const CorrWs cw = carve_corr(J.corr_ws, p);
+    const int ks = J.k_splits;
+    const double nm1 = (double)(J.n - 1);
+    const size_t pp = (size_t)p * p;
+    const int tid = threadIdx.x;
+    auto gsum = [&](int a, int b) {
+        double g = 0.0;
+        for (int q = 0; q < ks; ++q) g += cw.gram[(size_t)q * pp + (size_t)a * p + b];
+        return g;
+    };
+    __shared__ double s_sdr[QR], s_sdc[CT];
+    __shared__ double s_v[QR][CT + 1];
+    if (tid < CT + QR) {
+        const int idx = tid < CT ? b0 + tid : a0 + (tid - CT);
+        const double sd = idx < p ? sqrt(gsum(idx, idx) / nm1) : 1.0;
+        if (tid < CT)
+            s_sdc[tid] = sd;
+        else
+            s_sdr[tid - CT] = sd;
+    }
+    __syncthreads();
+    const int c = tid & 63, r0 = tid >> 6;
+    for (int r = r0; r < QR; r += 4) {
+        const int a = a0 + r, b = b0 + c;
+        double v = 0.0;
+        if (a < p && b < p) {
+            if (a <= b) {
+                v = gsum(a, b) / nm1;
+                v = v / s_sdr[r];
+                v = v / s_sdc[c];
+            } else {
+                // lower half of a diagonal tile: the mirrored upper element
+                v = gsum(b, a) / nm1;
+                v = v / s_sdc[c];
+                v = v / s_sdr[r];
+            }
+            v = v > 1.0 ? 1.0 : (v < -1.0 ? -1.0 : v);
+            if (J.C64) J.C64[(size_t)a * p + b] = v;
+            J.C32[(size_t)a * p + b] = (float)v;
+        }
+        s_v[r][c] = v;
+    }
+    if (ta == tb) return;
+    __syncthreads();
+    // mirror: C[b0 + c2][a0 + r2] for the 64 x 16 transposed block
+    const int r2 = tid & (QR - 1), c0 = tid / QR;
+    for (int c2 = c0; c2 < CT; c2 += 256 / QR) {
+        const int b = b0 + c2, a = a0 + r2;
+        if (a >= p || b >= p) continue;
+        const double v = s_v[r2][c2];
+        if (J.C64) J.C64[(size_t)b * p + a] = v;
+        J.C32[(size_t)b * p + a] = (float)v;
+    }
 }
 
 __global__ void c64_to_c32_kernel(const Job *__restrict__ jobs) {
@@ -223,12 +304,6 @@ __global__ void c64_to_c32_kernel(const Job *__restrict__ jobs) {
 // ----------------------------------------------------------------------------
 // host side
 // ----------------------------------------------------------------------------
-static int corr_row_splits(int64_t max_n) {
-    int rs = (int)((max_n + 127) / 128);
-    if (rs > CORR_ROW_SPLITS_MAX) rs = CORR_ROW_SPLITS_MAX;
-    return rs < 1 ? 1 : rs;
-}
-
 // split-K factor that gives one dataset >= 1024 workgroups
 int corr_k_splits(int64_t n, int p) {
     const int tu = upper_tiles(p);
@@ -255,20 +330,18 @@ int corr_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStre
     }
     if (dtype != 0 && dtype != 1) return set_error(ERR_ARG, "corr: dtype must be 0 (f32) or 1 (f64)");
     const int rs = corr_row_splits(maxn);
-    const int cb = (maxp + 255) / 256;
+    const int cb = (maxp + 63) / 64;
     if (dtype == 0)
-        GPUCSL_LAUNCH("corr_colsum", colsum_kernel<float>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev, rs);
+        GPUCSL_LAUNCH("corr_colsum", colsum_kernel<float>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
     else
-        GPUCSL_LAUNCH("corr_colsum", colsum_kernel<double>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev, rs);
-    GPUCSL_LAUNCH("corr_mean", mean_kernel, dim3(cb, 1, nj), dim3(256), 0, s, jobs_dev, rs);
+        GPUCSL_LAUNCH("corr_colsum", colsum_kernel<double>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
     if (dtype == 0)
         GPUCSL_LAUNCH("corr_gram_mfma_f64", gram_mfma_kernel<float>, dim3(maxtiles, maxks, nj), dim3(256), 0, s,
                       jobs_dev);
     else
         GPUCSL_LAUNCH("corr_gram_mfma_f64", gram_mfma_kernel<double>, dim3(maxtiles, maxks, nj), dim3(256), 0, s,
                       jobs_dev);
-    GPUCSL_LAUNCH("corr_sd", sd_kernel, dim3(cb, 1, nj), dim3(256), 0, s, jobs_dev);
-    GPUCSL_LAUNCH("corr_normalize", normalize_kernel, dim3(cb, maxp, nj), dim3(256), 0, s, jobs_dev);
+    GPUCSL_LAUNCH("corr_normalize", normalize_tile_kernel, dim3(maxtiles * NQ, 1, nj), dim3(256), 0, s, jobs_dev);
     return check_hip(hipGetLastError(), "corr launch");
 }
 

@@ -197,7 +197,8 @@ __global__ __launch_bounds__(256) void gauss_level0_kernel(const Job *__restrict
 // nbr[row] = ascending neighbours (diagonal excluded), deg, and the split into
 // neighbours below (cntB) and above (cntA) the row index.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jobs) {
+__global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jobs, unsigned int *__restrict__ plan_done) {
+    if (plan_done && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *plan_done = 0;
     const Job &J = jobs[blockIdx.y];
     const int p = J.p;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -225,19 +226,46 @@ __global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jo
     }
 }
 
+struct PlanArgs {
+    unsigned int *done;            // blocks finished; null: scan only
+    int nj;
+    int32_t *baseA, *baseB;
+    unsigned int *counters;
+    unsigned long long *bstats;
+    WorkItem *itemsA, *itemsB;
+};
+__device__ void jobprefix_body(const Job *__restrict__ jobs, int nj, int32_t *__restrict__ baseA,
+                               int32_t *__restrict__ baseB, unsigned int *__restrict__ counters,
+                               unsigned long long *__restrict__ bstats, int *sA, int *sB);
+__device__ void items_body(const Job *__restrict__ jobs, int nj, int level, const Team &team,
+                           const int32_t *__restrict__ baseA, const int32_t *__restrict__ baseB,
+                           WorkItem *__restrict__ itemsA, WorkItem *__restrict__ itemsB);
+
 // Per job (one block each): exclusive scans of cntA/cntB into rowoffA/rowoffB
 // and of their work items into itemoffA/itemoffB; adds T_l and max degree to
 // the launch statistics.  A job whose max level is below `level` gets no work.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
 __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs, int level, Team team,
-                                                    unsigned long long *__restrict__ st) {
+                                                    unsigned long long *__restrict__ st, PlanArgs plan) {
     const Job &J = jobs[blockIdx.x];
     const int p = J.p;
     const bool active = level <= J.max_level;
     JobWs w = carve_job(J.ws, p);
-    __shared__ int sA[1024], sB[1024], sIA[1024], sIB[1024];
-    __shared__ unsigned long long sT[1024];
-    __shared__ int sM[1024];
-    const int t = threadIdx.x;
+    __shared__ int sA[1024], sB[1024];           // job prefix of a batch (last block)
+    __shared__ int s_wsum[4][16], s_woff[4][16], s_tot[4];
+    __shared__ unsigned long long s_T[16];
+    __shared__ int s_M[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const bool single = plan.done && plan.nj == 1;
     const int per = (p + 1023) / 1024;
     const int r0 = t * per;
     int r1 = r0 + per;
@@ -260,29 +288,42 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
             tt += (unsigned long long)d * count;
         }
     }
-    sA[t] = la;
-    sB[t] = lb;
-    sIA[t] = lia;
-    sIB[t] = lib;
-    sT[t] = tt;
-    sM[t] = md;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int va = 0, vb = 0, via = 0, vib = 0;
-        if (t >= off) {
-            va = sA[t - off];
-            vb = sB[t - off];
-            via = sIA[t - off];
-            vib = sIB[t - off];
-        }
-        __syncthreads();
-        sA[t] += va;
-        sB[t] += vb;
-        sIA[t] += via;
-        sIB[t] += vib;
-        __syncthreads();
+    // block-wide exclusive scans of the four counts: wave scans, then a scan
+    // of the 16 wave totals
+    const int ia = wave_incl_scan(la), ib = wave_incl_scan(lb), iia = wave_incl_scan(lia), iib = wave_incl_scan(lib);
+    for (int off = 32; off >= 1; off >>= 1) {
+        tt += __shfl_xor(tt, off);
+        const int o = __shfl_xor(md, off);
+        md = o > md ? o : md;
     }
-    int ea = sA[t] - la, eb = sB[t] - lb, eia = sIA[t] - lia, eib = sIB[t] - lib;
+    if (lane == 63) {
+        s_wsum[0][wv] = ia;
+        s_wsum[1][wv] = ib;
+        s_wsum[2][wv] = iia;
+        s_wsum[3][wv] = iib;
+    }
+    if (lane == 0) {
+        s_T[wv] = tt;
+        s_M[wv] = md;
+    }
+    __syncthreads();
+    if (t < 16) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int x = s_wsum[k][t];
+            const int own = x;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const int y = __shfl_up(x, off);
+                if (t >= off) x += y;
+            }
+            s_woff[k][t] = x - own;
+            if (t == 15) s_tot[k] = x;
+        }
+    }
+    __syncthreads();
+    int ea = s_woff[0][wv] + ia - la, eb = s_woff[1][wv] + ib - lb;
+    int eia = s_woff[2][wv] + iia - lia, eib = s_woff[3][wv] + iib - lib;
     for (int r = r0; r < r1; ++r) {
         const int ca = active ? w.cntA[r] : 0, cb = active ? w.cntB[r] : 0;
         w.rowoffA[r] = ea;
@@ -294,36 +335,66 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         const uint64_t count = active ? binom(w.deg[r] - 1, level) : 0;
         if (count) {
             const int ppi = items_pairs(count, team);
-            eia += (ca + ppi - 1) / ppi;
-            eib += (cb + ppi - 1) / ppi;
+            const int na = (ca + ppi - 1) / ppi, nb = (cb + ppi - 1) / ppi;
+            if (single) {
+                // one job: its item lists start at 0
+                for (int k = 0; k < na; ++k) plan.itemsA[eia + k] = WorkItem{0, r, k * ppi, 0};
+                for (int k = 0; k < nb; ++k) plan.itemsB[eib + k] = WorkItem{0, r, k * ppi, 0};
+            }
+            eia += na;
+            eib += nb;
         }
     }
-    if (t == 1023) {
-        w.rowoffA[p] = sA[1023];
-        w.rowoffB[p] = sB[1023];
-        w.itemoffA[p] = sIA[1023];
-        w.itemoffB[p] = sIB[1023];
-    }
-    for (int off = 512; off >= 1; off >>= 1) {
-        __syncthreads();
-        if (t < off) {
-            sT[t] += sT[t + off];
-            sM[t] = sM[t] > sM[t + off] ? sM[t] : sM[t + off];
+    if (t == 0) {
+        w.rowoffA[p] = s_tot[0];
+        w.rowoffB[p] = s_tot[1];
+        w.itemoffA[p] = s_tot[2];
+        w.itemoffB[p] = s_tot[3];
+        if (st) {
+            unsigned long long T = 0;
+            int M = 0;
+            for (int k = 0; k < 16; ++k) {
+                T += s_T[k];
+                M = s_M[k] > M ? s_M[k] : M;
+            }
+            if (T) atomicAdd(&st[ST_TOTAL], T);
+            atomicMax(&st[ST_MAXDEG], (unsigned long long)M);
         }
     }
-    if (t == 0 && st) {
-        if (sT[0]) atomicAdd(&st[ST_TOTAL], sT[0]);
-        atomicMax(&st[ST_MAXDEG], (unsigned long long)sM[0]);
+    if (!plan.done) return;
+    if (single) {
+        if (t < NUM_COUNTERS) plan.counters[t] = 0;
+        if (t < BSTAT_STRIPES * BSTAT_WORDS) plan.bstats[t] = 0;
+        if (t == 0) {
+            plan.baseA[0] = 0;
+            plan.baseA[1] = s_tot[2];
+            plan.baseB[0] = 0;
+            plan.baseB[1] = s_tot[3];
+        }
+        return;
     }
+    // the block that finishes last plans the level for every job: job bases,
+    // counter resets and the work-item lists
+    __shared__ int s_last;
+    __syncthreads();
+    if (t == 0) {
+        __threadfence();
+        s_last = atomicAdd(plan.done, 1u) == (unsigned int)(plan.nj - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    jobprefix_body(jobs, plan.nj, plan.baseA, plan.baseB, plan.counters, plan.bstats, sA, sB);
+    __syncthreads();
+    items_body(jobs, plan.nj, level, team, plan.baseA, plan.baseB, plan.itemsA, plan.itemsB);
 }
 
 // Work-item bases of every job of the launch (exclusive prefix over jobs of
-// itemoff[p]); resets the counters and the statistics stripes.  One block.
-__global__ __launch_bounds__(1024) void jobprefix_kernel(const Job *__restrict__ jobs, int nj,
-                                                         int32_t *__restrict__ baseA, int32_t *__restrict__ baseB,
-                                                         unsigned int *__restrict__ counters,
-                                                         unsigned long long *__restrict__ bstats) {
-    __shared__ int sA[1024], sB[1024];
+// itemoff[p]); resets the counters and the statistics stripes.  Run by one
+// block of 1024 threads; sA, sB are 1024-int shared arrays.
+__device__ void jobprefix_body(const Job *__restrict__ jobs, int nj, int32_t *__restrict__ baseA,
+                               int32_t *__restrict__ baseB, unsigned int *__restrict__ counters,
+                               unsigned long long *__restrict__ bstats, int *sA, int *sB) {
     const int t = threadIdx.x;
     if (t < NUM_COUNTERS) counters[t] = 0;
     if (t < BSTAT_STRIPES * BSTAT_WORDS) bstats[t] = 0;
@@ -336,6 +407,7 @@ __global__ __launch_bounds__(1024) void jobprefix_kernel(const Job *__restrict__
         la += w.itemoffA[jobs[b].p];
         lb += w.itemoffB[jobs[b].p];
     }
+    __syncthreads();
     sA[t] = la;
     sB[t] = lb;
     __syncthreads();
@@ -364,26 +436,27 @@ __global__ __launch_bounds__(1024) void jobprefix_kernel(const Job *__restrict__
     }
 }
 
-// Work-item lists of both sides: one entry per item, {job, row, first pair},
-// at the job's base plus the row's item offset.  One thread per row.
-__global__ __launch_bounds__(256) void items_kernel(const Job *__restrict__ jobs, int level, Team team,
-                                                    const int32_t *__restrict__ baseA,
-                                                    const int32_t *__restrict__ baseB, WorkItem *__restrict__ itemsA,
-                                                    WorkItem *__restrict__ itemsB) {
-    const int b = blockIdx.y;
-    const Job &J = jobs[b];
-    const int p = J.p;
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= p || level > J.max_level) return;
-    JobWs w = carve_job(J.ws, p);
-    const uint64_t count = binom(w.deg[r] - 1, level);
-    if (!count) return;
-    const int ppi = items_pairs(count, team);
-    const int na = w.itemoffA[r + 1] - w.itemoffA[r], nb = w.itemoffB[r + 1] - w.itemoffB[r];
-    WorkItem *oa = itemsA + baseA[b] + w.itemoffA[r];
-    WorkItem *ob = itemsB + baseB[b] + w.itemoffB[r];
-    for (int k = 0; k < na; ++k) oa[k] = WorkItem{b, r, k * ppi, 0};
-    for (int k = 0; k < nb; ++k) ob[k] = WorkItem{b, r, k * ppi, 0};
+// Work-item lists of both sides, one entry per item {job, row, first pair}
+// at the job's base plus the row's item offset; rows strided over the block.
+__device__ void items_body(const Job *__restrict__ jobs, int nj, int level, const Team &team,
+                           const int32_t *__restrict__ baseA, const int32_t *__restrict__ baseB,
+                           WorkItem *__restrict__ itemsA, WorkItem *__restrict__ itemsB) {
+    for (int b = 0; b < nj; ++b) {
+        const Job &J = jobs[b];
+        if (level > J.max_level) continue;
+        const int p = J.p;
+        JobWs w = carve_job(J.ws, p);
+        for (int r = threadIdx.x; r < p; r += blockDim.x) {
+            const uint64_t count = binom(w.deg[r] - 1, level);
+            if (!count) continue;
+            const int ppi = items_pairs(count, team);
+            const int na = w.itemoffA[r + 1] - w.itemoffA[r], nb = w.itemoffB[r + 1] - w.itemoffB[r];
+            WorkItem *oa = itemsA + baseA[b] + w.itemoffA[r];
+            WorkItem *ob = itemsB + baseB[b] + w.itemoffB[r];
+            for (int k = 0; k < na; ++k) oa[k] = WorkItem{b, r, k * ppi, 0};
+            for (int k = 0; k < nb; ++k) ob[k] = WorkItem{b, r, k * ppi, 0};
+        }
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -845,7 +918,8 @@ template <int L, int GS>
 __global__ __launch_bounds__(256) void gauss_slow_kernel(
     const Job *__restrict__ jobs, int side, float margin_k, unsigned int *__restrict__ stripe_counters,
     const SlowItem *__restrict__ queue, const unsigned int *__restrict__ qcount,
-    unsigned long long *__restrict__ bstats) {
+    unsigned long long *__restrict__ bstats, unsigned int *__restrict__ stats_done,
+    unsigned long long *__restrict__ st) {
     const int lane = __lane_id();
     const int gbase = lane & ~(GS - 1);
     const int gl = lane & (GS - 1);
@@ -1025,14 +1099,21 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 atomicAdd(dst, v);
         }
     }
-}
-
-// Sums the striped accumulators of one level into its statistics slots.
-__global__ __launch_bounds__(64) void level_stats_kernel(const unsigned long long *__restrict__ bstats,
-                                                         unsigned long long *__restrict__ st) {
+    if (!st) return;
+    // the block that finishes last adds the level's striped totals to st
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(stats_done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x >= 64) return;
+    __threadfence();
     const int t = threadIdx.x;
     unsigned long long v[5];
-    for (int k = 0; k < 5; ++k) v[k] = bstats[(size_t)t * BSTAT_WORDS + k];
+    for (int k = 0; k < 5; ++k) v[k] = __hip_atomic_load(bstats + (size_t)t * BSTAT_WORDS + k, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
     for (int off = 32; off >= 1; off >>= 1) {
         for (int k = 0; k < 4; ++k) v[k] += __shfl_xor(v[k], off);
         const unsigned long long o = __shfl_xor(v[4], off);
@@ -1046,6 +1127,7 @@ __global__ __launch_bounds__(64) void level_stats_kernel(const unsigned long lon
         st[ST_MAXRATIO] = v[BS_MAXRATIO] > st[ST_MAXRATIO] ? v[BS_MAXRATIO] : st[ST_MAXRATIO];
     }
 }
+
 
 // ----------------------------------------------------------------------------
 // pmax and output conversion
@@ -1097,7 +1179,7 @@ static int device_cus() {
     }
     return cus;
 }
-static int slow_grid() { return device_cus() * PHASE_BLOCKS_PER_CU / 4; }
+static int slow_grid() { return device_cus() / 2; }
 static int fast_grid(int level) {
     tuning_defaults();
     return device_cus() * g_tuning[level].fast_blocks_per_cu;
@@ -1139,14 +1221,19 @@ int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, uns
     return check_hip(hipGetLastError(), "gauss_level0 launch");
 }
 
-int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
-                        hipStream_t s) {
+static int compact_and_plan(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
+                            const PlanArgs &plan, hipStream_t s) {
     const int mp = max_p(jh, nj);
-    GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd);
+    GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd, plan.done);
     const Team team = team_for(level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
                                gs_min);
-    GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, team, st);
+    GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, team, st, plan);
     return check_hip(hipGetLastError(), "compact launch");
+}
+
+int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
+                        hipStream_t s) {
+    return compact_and_plan(jh, jd, nj, level, gs_min, st, PlanArgs{}, s);
 }
 
 static const char *const kPhaseFastName[] = {
@@ -1158,7 +1245,7 @@ static const char *const kPhaseSlowName[] = {
 
 template <int L>
 static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int gs_min, float margin_k,
-                         hipStream_t s) {
+                         unsigned long long *st, hipStream_t s) {
     const int32_t *base = side == 0 ? w.itembaseA : w.itembaseB;
     const WorkItem *items = side == 0 ? w.itemsA : w.itemsB;
     SlowItem *queue = side == 0 ? w.queueA : w.queueB;
@@ -1168,7 +1255,7 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
     GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj, base,
                   items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
     GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 16>), dim3(slow_grid()), dim3(256), 0, s, jd, side,
-                  margin_k, slow_ctr, queue, qcount, w.bstats);
+                  margin_k, slow_ctr, queue, qcount, w.bstats, w.counters + CNT_STATS_DONE, side == 1 ? st : nullptr);
 }
 
 int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float margin_k, int group_size,
@@ -1184,26 +1271,22 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
     for (int b = 0; b < nj; ++b)
         if (!jh[b].C32 || !jh[b].C64 || !jh[b].G || !jh[b].zmin || !jh[b].ws || (jh[b].max_level > 0 && !jh[b].sepsets))
             return set_error(ERR_ARG, "gauss_level: job %d has a null buffer", b);
-    int rc = gauss_batch_compact(jh, jd, nj, level, group_size, st, s);
-    if (rc) return rc;
     BatchWs w = carve_batch(bws, nj, qcap, icap);
-    GPUCSL_LAUNCH("job_prefix", jobprefix_kernel, dim3(1), dim3(1024), 0, s, jd, nj, w.itembaseA, w.itembaseB,
-                  w.counters, w.bstats);
-    const int mp = max_p(jh, nj);
-    GPUCSL_LAUNCH("work_items", items_kernel, dim3((mp + 255) / 256, nj), dim3(256), 0, s, jd, level,
-                  team_for(level, group_size), w.itembaseA, w.itembaseB, w.itemsA, w.itemsB);
+    const PlanArgs plan{w.counters + CNT_PLAN_DONE, nj, w.itembaseA, w.itembaseB, w.counters, w.bstats, w.itemsA,
+                        w.itemsB};
+    int rc = compact_and_plan(jh, jd, nj, level, group_size, st, plan, s);
+    if (rc) return rc;
     for (int side = 0; side < 2; ++side) {
         switch (level) {
 #define GPUCSL_CASE(LV)                                                     \
     case LV:                                                                \
-        launch_phase<LV>(jd, nj, w, side, group_size, margin_k, s);         \
+        launch_phase<LV>(jd, nj, w, side, group_size, margin_k, st, s);     \
         break;
             GPUCSL_CASE(1) GPUCSL_CASE(2) GPUCSL_CASE(3) GPUCSL_CASE(4)
             GPUCSL_CASE(5) GPUCSL_CASE(6) GPUCSL_CASE(7) GPUCSL_CASE(8)
 #undef GPUCSL_CASE
         }
     }
-    GPUCSL_LAUNCH("level_stats", level_stats_kernel, dim3(1), dim3(64), 0, s, w.bstats, st);
     return check_hip(hipGetLastError(), "gauss_level launch");
 }
 
@@ -1263,11 +1346,29 @@ int single_write_desc(const Job &j, void *ws, hipStream_t s) {
     return check_hip(hipGetLastError(), "job descriptor");
 }
 
-int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, hipStream_t s) {
+// G = 1, zmin = +inf, sepsets = -1 and the statistics zeroed, in one launch
+__global__ void single_init_kernel(uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, float *__restrict__ zmin,
+                                   size_t pp, size_t ns, unsigned long long *__restrict__ st, int nst) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t t = t0; t < pp; t += stride) {
+        Gm[t] = 1;
+        zmin[t] = __builtin_inff();
+    }
+    if (sep)
+        for (size_t t = t0; t < ns; t += stride) sep[t] = -1;
+    if (st && t0 < (size_t)nst) st[t0] = 0;
+}
+
+int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, unsigned long long *st, int nst,
+                      hipStream_t s) {
     if (p < 1 || max_level < 0 || !Gm || !zmin) return set_error(ERR_ARG, "gauss_init: bad arguments");
-    launch_fill<uint8_t>(Gm, (uint8_t)1, (size_t)p * p, s);
-    if (max_level > 0 && sep) launch_fill<int32_t>(sep, -1, (size_t)p * p * max_level, s);
-    launch_fill<float>(zmin, __builtin_inff(), (size_t)p * p, s);
+    const size_t pp = (size_t)p * p, ns = max_level > 0 && sep ? pp * max_level : 0;
+    size_t blocks = ((ns > pp ? ns : pp) + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks * 256 < (size_t)nst) blocks = ((size_t)nst + 255) / 256;
+    GPUCSL_LAUNCH("gauss_init", single_init_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Gm,
+                  max_level > 0 ? sep : nullptr, zmin, pp, ns, st, nst);
     return check_hip(hipGetLastError(), "gauss_init launch");
 }
 

@@ -85,7 +85,9 @@ inline size_t corr_job_ws_bytes(int p, int k_splits) {
 
 // per-launch (batch) workspace
 constexpr int NUM_COUNTERS = 64;
-constexpr int CNT_QUEUE = 32;
+constexpr int CNT_QUEUE = 32;       // 32, 33: queue lengths of sides A and B
+constexpr int CNT_PLAN_DONE = 40;   // scan blocks finished (level planning)
+constexpr int CNT_STATS_DONE = 41;  // side-B recheck blocks finished (level statistics)
 constexpr int STRIPES = 8;
 constexpr int BSTAT_WORDS = 8;
 constexpr int BSTAT_STRIPES = 64;
