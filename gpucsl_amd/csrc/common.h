@@ -118,6 +118,39 @@ __device__ __forceinline__ void unrank_lex(int N, uint64_t r, int (&pos)[L]) {
     unrank_step<L, 0>(N, r, 0, pos);
 }
 
+// The same unranking with the binomials read from a table:
+// tab[(k - 1) * stride + m] = C(m, k) for k = 1..L, m = 0..N; the inverse
+// binomial of each position is a binary search over one table row.
+template <int L, int C>
+__device__ __forceinline__ void unrank_step_tab(int N, uint64_t r, int prev, int (&pos)[L],
+                                                const uint64_t *tab, int stride) {
+    if constexpr (C < L) {
+        constexpr int K = L - C - 1;
+        if constexpr (K == 0) {
+            const uint64_t room = (uint64_t)(N - prev);
+            pos[C] = prev + 1 + (int)(r < room ? r : room - 1);
+        } else {
+            const uint64_t *t = tab + K * stride;   // C(., K + 1)
+            const uint64_t R = t[N - prev] - r;
+            int lo = K, hi = N - prev - 1;         // largest m in [K, hi] with C(m, K+1) < R
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (t[mid] < R)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            pos[C] = N - lo;
+            unrank_step_tab<L, C + 1>(N, t[lo + 1] - R, N - lo, pos, tab, stride);
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void unrank_lex_tab(int N, uint64_t r, int (&pos)[L], const uint64_t *tab, int stride) {
+    unrank_step_tab<L, 0>(N, r, 0, pos, tab, stride);
+}
+
 // ----------------------------------------------------------------------------
 // float min/max on non-negative values through integer atomics
 // ----------------------------------------------------------------------------

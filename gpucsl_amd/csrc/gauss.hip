@@ -517,7 +517,8 @@ __device__ __forceinline__ float z32_from_values(float cij, const float (&ci)[L]
 // (L+2)-step elimination, amplified by the smallest pivot
 template <int L>
 __device__ __forceinline__ float rho_error(float dmin) {
-    return EPS32 * (float)(L + 2) * (1.0f + __frcp_rn(fmaxf(dmin, 1e-30f)));
+    // v_rcp_f32 (about 1 ulp) is ample for a bound scaled by the margin
+    return EPS32 * (float)(L + 2) * (1.0f + __builtin_amdgcn_rcpf(fmaxf(dmin, 1e-30f)));
 }
 
 __device__ __forceinline__ bool rho_bad(float dmin, float ar) { return !(dmin > 1e-5f) || !(ar < 1.0f); }
@@ -562,11 +563,12 @@ __device__ __forceinline__ bool z32_bad(float z, float dmin, float rho) {
 // A team walks the pair's subset ranks in increasing order and stops at the
 // first passing test, so the removing sepset is the first in pcalg order and
 // zmin covers exactly the tests up to it.  What the tests read is staged in LDS
-// by the whole block:
-//   FULL  (l >= 2) C[A, A] with A = adj(i) + [i]: the tests read LDS only;
-//   ROW   neighbour ids, C[i, adj(i)] and diag C[s, s]; C[j, S] and the
-//         off-diagonal C[S, S] come from global memory (L2);
-//   none  (rows beyond the LDS budget) everything from global memory.
+// by the whole block (MODE, a template parameter so that LDS reads compile to
+// ds_read and global reads to global_load):
+//   0 FULL  (l >= 2) C[A, A] with A = adj(i) + [i]: the tests read LDS only;
+//   1 ROW   neighbour ids, C[i, adj(i)] and diag C[s, s]; C[j, S] and the
+//           off-diagonal C[S, S] come from global memory (L2);
+//   2 none  (rows beyond the LDS budget) everything from global memory.
 // Decisions are taken on the partial correlation rho itself: a test passes
 // when |rho| < tanh(threshold), and one within margin * (its FP32 error bound)
 // of that value sends the pair, from the start of that iteration, to the FP64
@@ -574,12 +576,31 @@ __device__ __forceinline__ bool z32_bad(float z, float dmin, float rho) {
 // gpucsl/pc/cuda/gaussian_ci.cu:66-120, set_helpers.cu:25-71,
 // graph_helpers.cu:81-133.
 // ----------------------------------------------------------------------------
-template <int L>
-__device__ __forceinline__ void gather_test(int mode, const float *__restrict__ sf, const int *__restrict__ snb,
-                                            int d, int D, int jpos, const float *__restrict__ C32, int p, int i, int j,
-                                            const int32_t *__restrict__ gnb, const int (&sp)[L], float &cij,
-                                            float (&ci)[L], float (&cj)[L], float (&cm)[L][L]) {
-    if (mode == 0) {
+// Loads through the global address space (global_load instead of flat_load:
+// no LDS counter to wait on).  Only for pointers into device memory.
+typedef const __attribute__((address_space(1))) float gfloat_t;
+typedef const __attribute__((address_space(1))) int gint_t;
+__device__ __forceinline__ float ldg(const float *ptr) { return *(gfloat_t *)(ptr); }
+__device__ __forceinline__ int ldg(const int *ptr) { return *(gint_t *)(ptr); }
+
+struct ItemCtx {
+    const Job *J;
+    int p, i, d, D, Nn, cnt, jbase, q0, q1, job, side;
+    uint64_t count;
+    float rthr, margin_k;
+    const int32_t *gnb, *rowoff;
+    const float *C32;
+    const int *snb;    // LDS
+    const float *sf;   // LDS
+    const uint64_t *tab;  // LDS binomial table C(m, k), k = 1..L, m = 0..Nn; null: computed
+};
+
+template <int L, int MODE>
+__device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, const int (&sp)[L], bool &near) {
+    float cij, ci[L], cj[L], cm[L][L];
+    if constexpr (MODE == 0) {
+        const float *sf = c.sf;
+        const int D = c.D, d = c.d;
         cij = sf[d * D + jpos];
 #pragma unroll
         for (int a = 0; a < L; ++a) {
@@ -588,65 +609,64 @@ __device__ __forceinline__ void gather_test(int mode, const float *__restrict__ 
 #pragma unroll
             for (int b = 0; b <= a; ++b) cm[a][b] = sf[sp[a] * D + sp[b]];
         }
-    } else if (mode == 1) {
-        const float *rowI = sf, *diag = sf + d;
-        const float *rowJ = C32 + (size_t)j * p;
-        cij = rowI[jpos];
+    } else if constexpr (MODE == 1) {
+        const float *rowI = c.sf, *diag = c.sf + c.d;
+        const float *__restrict__ rowJ = c.C32 + (size_t)j * c.p;
         int S[L];
 #pragma unroll
-        for (int a = 0; a < L; ++a) S[a] = snb[sp[a]];
+        for (int a = 0; a < L; ++a) S[a] = c.snb[sp[a]];
+        cij = rowI[jpos];
 #pragma unroll
         for (int a = 0; a < L; ++a) {
             ci[a] = rowI[sp[a]];
-            cj[a] = rowJ[S[a]];
+            cj[a] = ldg(rowJ + S[a]);
 #pragma unroll
-            for (int b = 0; b < a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
+            for (int b = 0; b < a; ++b) cm[a][b] = ldg(c.C32 + (size_t)S[a] * c.p + S[b]);
             cm[a][a] = diag[sp[a]];
         }
     } else {
+        const float *__restrict__ C = c.C32;
+        const int p = c.p;
         int S[L];
 #pragma unroll
-        for (int a = 0; a < L; ++a) S[a] = gnb[sp[a]];
-        cij = C32[(size_t)i * p + j];
+        for (int a = 0; a < L; ++a) S[a] = ldg(c.gnb + sp[a]);
+        cij = ldg(C + (size_t)c.i * p + j);
 #pragma unroll
         for (int a = 0; a < L; ++a) {
-            ci[a] = C32[(size_t)i * p + S[a]];
-            cj[a] = C32[(size_t)j * p + S[a]];
+            ci[a] = ldg(C + (size_t)c.i * p + S[a]);
+            cj[a] = ldg(C + (size_t)j * p + S[a]);
 #pragma unroll
-            for (int b = 0; b <= a; ++b) cm[a][b] = C32[(size_t)S[a] * p + S[b]];
+            for (int b = 0; b <= a; ++b) cm[a][b] = ldg(C + (size_t)S[a] * p + S[b]);
         }
     }
-}
-
-// One FP32 test in rho space: |rho|, and whether it is within the margin of
-// the threshold rthr (or degenerate).
-template <int L>
-__device__ __forceinline__ float rho_test(int mode, const float *__restrict__ sf, const int *__restrict__ snb, int d,
-                                          int D, int jpos, const float *__restrict__ C32, int p, int i, int j,
-                                          const int32_t *__restrict__ gnb, const int (&sp)[L], float rthr,
-                                          float margin_k, bool &near) {
-    float cij, ci[L], cj[L], cm[L][L];
-    gather_test<L>(mode, sf, snb, d, D, jpos, C32, p, i, j, gnb, sp, cij, ci, cj, cm);
     float dmin, rho;
     rho_from_values<L>(cij, ci, cj, cm, dmin, rho);
     const float ar = fabsf(rho);
-    near = rho_bad(dmin, ar) || fabsf(ar - rthr) <= margin_k * rho_error<L>(dmin);
+    near = rho_bad(dmin, ar) || fabsf(ar - c.rthr) <= c.margin_k * rho_error<L>(dmin);
     return ar;
 }
 
-template <int L>
-__device__ __forceinline__ void write_sepset(const Job &J, int p, int i, int j, int mode,
-                                             const int *__restrict__ snb, const int32_t *__restrict__ gnb,
-                                             const int (&sp)[L]) {
+template <int L, int MODE>
+__device__ __forceinline__ int nbr_at(const ItemCtx &c, int k) {
+    if constexpr (MODE == 2)
+        return ldg(c.gnb + k);
+    else
+        return c.snb[k];
+}
+
+template <int L, int MODE>
+__device__ __forceinline__ void write_sepset(const ItemCtx &c, int j, const int (&sp)[L]) {
+    const Job &J = *c.J;
+    const int p = c.p, i = c.i;
     J.G[(size_t)i * p + j] = 0;
     J.G[(size_t)j * p + i] = 0;
     int32_t *sep = J.sepsets;
     const int maxL = J.max_level;
 #pragma unroll
-    for (int c = 0; c < L; ++c) {
-        const int v = mode == 2 ? gnb[sp[c]] : snb[sp[c]];
-        sep[((size_t)i * p + j) * maxL + c] = v;
-        sep[((size_t)j * p + i) * maxL + c] = v;
+    for (int k = 0; k < L; ++k) {
+        const int v = nbr_at<L, MODE>(c, sp[k]);
+        sep[((size_t)i * p + j) * maxL + k] = v;
+        sep[((size_t)j * p + i) * maxL + k] = v;
     }
 }
 
@@ -654,6 +674,160 @@ __device__ __forceinline__ void update_zmin(const Job &J, int p, int i, int j, f
     const int a = i < j ? i : j, b = i < j ? j : i;
     float *zp = J.zmin + (size_t)a * p + b;
     if (z < *zp) *zp = z;
+}
+
+template <int L>
+__device__ __forceinline__ void subset_of(const ItemCtx &c, uint64_t rank, int jpos, int (&sp)[L]) {
+    int pos[L];
+    if (L >= 2 && c.tab)
+        unrank_lex_tab<L>(c.Nn, rank, pos, c.tab, c.Nn + 1);
+    else
+        unrank_lex<L>(c.Nn, rank, pos);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const int idx = pos[k] - 1;
+        sp[k] = idx + (idx >= jpos);
+    }
+}
+
+// the whole block works through the item's pairs in turn
+template <int L, int MODE>
+__device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount,
+                            unsigned long long &n_exec, unsigned long long &n_useful,
+                            unsigned long long (&s_pm)[4], unsigned long long (&s_nm)[4], float (&s_r)[4]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint8_t *__restrict__ Gm = c.J->G;
+    for (int q = c.q0; q < c.q1; ++q) {
+        const int jpos = c.jbase + q;
+        const int j = nbr_at<L, MODE>(c, jpos);
+        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] == 0) continue;
+        float rl = __builtin_inff();  // min |rho| over the tests counted so far
+        for (uint64_t iter = 0;; ++iter) {
+            const uint64_t rank = iter * 256 + tid;
+            const bool act = rank < c.count;
+            bool pass = false, near = false;
+            float ar = __builtin_inff();
+            int sp[L];
+            if (act) {
+                subset_of<L>(c, rank, jpos, sp);
+                ar = rho_test<L, MODE>(c, jpos, j, sp, near);
+                pass = ar < c.rthr;
+            }
+            const unsigned long long pm = __ballot(pass), nm = __ballot(near);
+            if (lane == 0) {
+                s_pm[wave] = pm;
+                s_nm[wave] = nm;
+            }
+            __syncthreads();
+            const bool any_near = (s_nm[0] | s_nm[1] | s_nm[2] | s_nm[3]) != 0;
+            int P = -1;
+#pragma unroll
+            for (int w2 = 0; w2 < 4; ++w2)
+                if (P < 0 && s_pm[w2]) P = w2 * 64 + __ffsll(s_pm[w2]) - 1;
+            bool finish;
+            if (any_near) {
+                finish = true;
+            } else {
+                if (act) ++n_exec;
+                if (P >= 0) {
+                    if (act && tid <= P) {
+                        ++n_useful;
+                        rl = fminf(rl, ar);
+                    }
+                    if (tid == P) write_sepset<L, MODE>(c, j, sp);
+                    finish = true;
+                } else {
+                    if (act) {
+                        ++n_useful;
+                        rl = fminf(rl, ar);
+                    }
+                    finish = (iter + 1) * 256 >= c.count;
+                }
+            }
+            if (finish) {
+                float m = rl;
+                for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+                if (lane == 0) s_r[wave] = m;
+                __syncthreads();
+                if (tid == 0) {
+                    const float zb = z_of_rho(fminf(fminf(s_r[0], s_r[1]), fminf(s_r[2], s_r[3])));
+                    if (any_near) {
+                        const unsigned int slot = atomicAdd(qcount, 1u);
+                        queue[slot] = SlowItem{iter * 256, c.rowoff[c.i] + q, c.job, zb, 0};
+                    } else {
+                        update_zmin(*c.J, c.p, c.i, j, zb);
+                    }
+                }
+            }
+            __syncthreads();  // s_pm / s_nm / s_r are reused
+            if (finish) break;
+        }
+    }
+}
+
+// teams of `team` lanes inside a wave pull pairs from s_next
+template <int L, int MODE>
+__device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__restrict__ queue,
+                           unsigned int *__restrict__ qcount, unsigned long long &n_exec,
+                           unsigned long long &n_useful) {
+    const int lane = threadIdx.x & 63;
+    const int gbase = lane & ~(team - 1);
+    const int gl = lane & (team - 1);
+    const uint64_t gmask = team == 64 ? ~0ull : (((1ull << team) - 1ull) << gbase);
+    const uint8_t *__restrict__ Gm = c.J->G;
+    while (true) {
+        int q = 0;
+        if (gl == 0) q = atomicAdd(s_next, 1);
+        q = __shfl(q, gbase);
+        if (q >= c.q1) break;
+        const int jpos = c.jbase + q;
+        const int j = nbr_at<L, MODE>(c, jpos);
+        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] == 0) continue;
+        float rl = __builtin_inff();
+        for (uint64_t iter = 0;; ++iter) {
+            const uint64_t rank = iter * team + gl;
+            const bool act = rank < c.count;
+            bool pass = false, near = false;
+            float ar = __builtin_inff();
+            int sp[L];
+            if (act) {
+                subset_of<L>(c, rank, jpos, sp);
+                ar = rho_test<L, MODE>(c, jpos, j, sp, near);
+                pass = ar < c.rthr;
+            }
+            if (__ballot(near) & gmask) {
+                for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
+                if (gl == 0) {
+                    const unsigned int slot = atomicAdd(qcount, 1u);
+                    queue[slot] = SlowItem{iter * team, c.rowoff[c.i] + q, c.job, z_of_rho(rl), 0};
+                }
+                break;
+            }
+            if (act) ++n_exec;
+            const uint64_t pm = __ballot(pass) & gmask;
+            bool finish;
+            if (pm) {
+                const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
+                if (act && gl <= P) {
+                    ++n_useful;
+                    rl = fminf(rl, ar);
+                }
+                if (gl == P) write_sepset<L, MODE>(c, j, sp);
+                finish = true;
+            } else {
+                if (act) {
+                    ++n_useful;
+                    rl = fminf(rl, ar);
+                }
+                finish = (iter + 1) * team >= c.count;
+            }
+            if (finish) {
+                for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
+                if (gl == 0) update_zmin(*c.J, c.p, c.i, j, z_of_rho(rl));
+                break;
+            }
+        }
+    }
 }
 
 template <int L>
@@ -672,30 +846,53 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
         const WorkItem it = items[item];
         const Job &J = jobs[it.job];
-        const int p = J.p, i = it.row;
-        JobWs jw = carve_job(J.ws, p);
-        const int32_t *__restrict__ rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
-        const float *__restrict__ C32 = J.C32;
-        const uint8_t *__restrict__ Gm = J.G;
-        const float rthr = (float)tanh(J.thr[L]);
-        const int d = jw.deg[i];
-        const int cnt = rowoff[i + 1] - rowoff[i];
-        const int Nn = d - 1;
-        const uint64_t count = binom_k<L>(Nn);
-        const int team = team_lanes(count, tm);
-        const int q0 = it.q0;
-        const int q1 = min(cnt, q0 + items_pairs(count, tm));
-        const int jbase = side == 0 ? d - cnt : 0;
-        const int D = d + 1;
+        ItemCtx c;
+        c.J = &J;
+        c.p = J.p;
+        c.i = it.row;
+        c.job = it.job;
+        c.side = side;
+        JobWs jw = carve_job(J.ws, c.p);
+        c.rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
+        c.C32 = J.C32;
+        c.rthr = (float)tanh(J.thr[L]);
+        c.margin_k = margin_k;
+        c.d = jw.deg[c.i];
+        c.cnt = c.rowoff[c.i + 1] - c.rowoff[c.i];
+        c.Nn = c.d - 1;
+        c.count = binom_k<L>(c.Nn);
+        const int team = team_lanes(c.count, tm);
+        c.q0 = it.q0;
+        c.q1 = min(c.cnt, c.q0 + items_pairs(c.count, tm));
+        c.jbase = side == 0 ? c.d - c.cnt : 0;
+        c.D = c.d + 1;
+        const int d = c.d, D = c.D, i = c.i, p = c.p;
         const int mode = (L >= 2 && D * D + d <= WORDS) ? 0 : (3 * d <= WORDS ? 1 : 2);
-        const int32_t *gnb = jw.nbr + (size_t)i * p;
+        c.gnb = jw.nbr + (size_t)i * p;
         int *snb = lds_words;
         float *sf = reinterpret_cast<float *>(lds_words + d);
+        c.snb = snb;
+        c.sf = sf;
+        // binomial table after the staged data, when it fits
+        const int tab_off = ((d + (mode == 0 ? D * D : 2 * d)) + 1) & ~1;
+        const bool use_tab = L >= 2 && mode != 2 && c.Nn >= L && tab_off + 2 * L * (c.Nn + 1) <= WORDS;
+        uint64_t *tab = use_tab ? reinterpret_cast<uint64_t *>(lds_words + tab_off) : nullptr;
+        c.tab = tab;
+        const float *__restrict__ C32 = c.C32;
 
         __syncthreads();  // the previous item is done with LDS
-        if (tid == 0) s_next = q0;
+        if (tid == 0) s_next = c.q0;
         if (mode != 2)
-            for (int k = tid; k < d; k += 256) snb[k] = gnb[k];
+            for (int k = tid; k < d; k += 256) snb[k] = ldg(c.gnb + k);
+        if (use_tab)
+            for (int m = tid; m <= c.Nn; m += 256) {
+                uint64_t b = 1;  // C(m, k) = C(m, k-1) (m - k + 1) / k, exact at every step
+#pragma unroll
+                for (int k = 1; k <= L; ++k) {
+                    b = m >= k ? b * (uint64_t)(m - k + 1) / (uint64_t)k : 0;
+                    tab[(k - 1) * (c.Nn + 1) + m] = b;
+                }
+            }
         __syncthreads();
         if (mode == 0) {
             // sub[a * D + b] = C[A[a], A[b]]: wave w stages rows a = w, w + 4, ...,
@@ -707,19 +904,19 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
                     const int a = a0 + 4 * u;
                     const int va = a < d ? snb[a] : i;
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const int b = c * 64 + lane;
+                    for (int cc = 0; cc < 2; ++cc) {
+                        const int b = cc * 64 + lane;
                         const int vb = b < d ? snb[b] : i;
-                        v[u][c] = (a < D && b < D) ? C32[(size_t)va * p + vb] : 0.0f;
+                        v[u][cc] = (a < D && b < D) ? ldg(C32 + (size_t)va * p + vb) : 0.0f;
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int a = a0 + 4 * u;
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const int b = c * 64 + lane;
-                        if (a < D && b < D) sf[a * D + b] = v[u][c];
+                    for (int cc = 0; cc < 2; ++cc) {
+                        const int b = cc * 64 + lane;
+                        if (a < D && b < D) sf[a * D + b] = v[u][cc];
                     }
                 }
             }
@@ -730,8 +927,8 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
                 for (int u = 0; u < 2; ++u) {
                     const int k = k0 + u * 256 + tid;
                     const int vv = k < d ? snb[k] : 0;
-                    v[u] = k < d ? C32[(size_t)i * p + vv] : 0.0f;
-                    w[u] = k < d ? C32[(size_t)vv * p + vv] : 0.0f;
+                    v[u] = k < d ? ldg(C32 + (size_t)i * p + vv) : 0.0f;
+                    w[u] = k < d ? ldg(C32 + (size_t)vv * p + vv) : 0.0f;
                 }
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
@@ -746,144 +943,19 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         __syncthreads();
 
         if (team == 256) {
-            // ---- the whole block works through the item's pairs in turn
-            for (int q = q0; q < q1; ++q) {
-                const int jpos = jbase + q;
-                const int j = mode == 2 ? gnb[jpos] : snb[jpos];
-                if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
-                float rl = __builtin_inff();  // min |rho| over the tests counted so far
-                for (uint64_t iter = 0;; ++iter) {
-                    const uint64_t rank = iter * 256 + tid;
-                    const bool act = rank < count;
-                    bool pass = false, near = false;
-                    float ar = __builtin_inff();
-                    int sp[L];
-                    if (act) {
-                        int pos[L];
-                        unrank_lex<L>(Nn, rank, pos);
-#pragma unroll
-                        for (int c = 0; c < L; ++c) {
-                            const int idx = pos[c] - 1;
-                            sp[c] = idx + (idx >= jpos);
-                        }
-                        ar = rho_test<L>(mode, sf, snb, d, D, jpos, C32, p, i, j, gnb, sp, rthr, margin_k, near);
-                        pass = ar < rthr;
-                    }
-                    const unsigned long long pm = __ballot(pass), nm = __ballot(near);
-                    if (lane == 0) {
-                        s_pm[wave] = pm;
-                        s_nm[wave] = nm;
-                    }
-                    __syncthreads();
-                    const bool any_near = (s_nm[0] | s_nm[1] | s_nm[2] | s_nm[3]) != 0;
-                    int P = -1;
-#pragma unroll
-                    for (int w2 = 0; w2 < 4; ++w2)
-                        if (P < 0 && s_pm[w2]) P = w2 * 64 + __ffsll(s_pm[w2]) - 1;
-                    bool finish;
-                    if (any_near) {
-                        finish = true;
-                    } else {
-                        if (act) ++n_exec;
-                        if (P >= 0) {
-                            if (act && tid <= P) {
-                                ++n_useful;
-                                rl = fminf(rl, ar);
-                            }
-                            if (tid == P) write_sepset<L>(J, p, i, j, mode, snb, gnb, sp);
-                            finish = true;
-                        } else {
-                            if (act) {
-                                ++n_useful;
-                                rl = fminf(rl, ar);
-                            }
-                            finish = (iter + 1) * 256 >= count;
-                        }
-                    }
-                    if (finish) {
-                        float m = rl;
-                        for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-                        if (lane == 0) s_r[wave] = m;
-                        __syncthreads();
-                        if (tid == 0) {
-                            const float zb = z_of_rho(fminf(fminf(s_r[0], s_r[1]), fminf(s_r[2], s_r[3])));
-                            if (any_near) {
-                                const unsigned int slot = atomicAdd(qcount, 1u);
-                                queue[slot] = SlowItem{iter * 256, rowoff[i] + q, it.job, zb, 0};
-                            } else {
-                                update_zmin(J, p, i, j, zb);
-                            }
-                        }
-                    }
-                    __syncthreads();  // s_pm / s_nm / s_r are reused
-                    if (finish) break;
-                }
-            }
-            continue;
-        }
-
-        // ---- teams of `team` lanes inside a wave pull pairs from s_next
-        const int gbase = lane & ~(team - 1);
-        const int gl = lane & (team - 1);
-        const uint64_t gmask = team == 64 ? ~0ull : (((1ull << team) - 1ull) << gbase);
-        while (true) {
-            int q = 0;
-            if (gl == 0) q = atomicAdd(&s_next, 1);
-            q = __shfl(q, gbase);
-            if (q >= q1) break;
-            const int jpos = jbase + q;
-            const int j = mode == 2 ? gnb[jpos] : snb[jpos];
-            if (side == 1 && Gm[(size_t)i * p + j] == 0) continue;
-            float rl = __builtin_inff();
-            for (uint64_t iter = 0;; ++iter) {
-                const uint64_t rank = iter * team + gl;
-                const bool act = rank < count;
-                bool pass = false, near = false;
-                float ar = __builtin_inff();
-                int sp[L];
-                if (act) {
-                    int pos[L];
-                    unrank_lex<L>(Nn, rank, pos);
-#pragma unroll
-                    for (int c = 0; c < L; ++c) {
-                        const int idx = pos[c] - 1;
-                        sp[c] = idx + (idx >= jpos);
-                    }
-                    ar = rho_test<L>(mode, sf, snb, d, D, jpos, C32, p, i, j, gnb, sp, rthr, margin_k, near);
-                    pass = ar < rthr;
-                }
-                if (__ballot(near) & gmask) {
-                    for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
-                    if (gl == 0) {
-                        const unsigned int slot = atomicAdd(qcount, 1u);
-                        queue[slot] = SlowItem{iter * team, rowoff[i] + q, it.job, z_of_rho(rl), 0};
-                    }
-                    break;
-                }
-                if (act) ++n_exec;
-                const uint64_t pm = __ballot(pass) & gmask;
-                bool finish;
-                if (pm) {
-                    const int P = __ffsll((unsigned long long)pm) - 1 - gbase;
-                    if (act && gl <= P) {
-                        ++n_useful;
-                        rl = fminf(rl, ar);
-                    }
-                    if (gl == P) write_sepset<L>(J, p, i, j, mode, snb, gnb, sp);
-                    finish = true;
-                } else {
-                    if (act) {
-                        ++n_useful;
-                        rl = fminf(rl, ar);
-                    }
-                    finish = (iter + 1) * team >= count;
-                }
-                if (finish) {
-                    for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
-                    if (gl == 0) update_zmin(J, p, i, j, z_of_rho(rl));
-                    break;
-                }
-            }
+            if (mode == 0)
+                block_pairs<L, 0>(c, queue, qcount, n_exec, n_useful, s_pm, s_nm, s_r);
+            else if (mode == 1)
+                block_pairs<L, 1>(c, queue, qcount, n_exec, n_useful, s_pm, s_nm, s_r);
+            else
+                block_pairs<L, 2>(c, queue, qcount, n_exec, n_useful, s_pm, s_nm, s_r);
+        } else {
+            if (mode == 0)
+                team_pairs<L, 0>(c, team, &s_next, queue, qcount, n_exec, n_useful);
+            else if (mode == 1)
+                team_pairs<L, 1>(c, team, &s_next, queue, qcount, n_exec, n_useful);
+            else
+                team_pairs<L, 2>(c, team, &s_next, queue, qcount, n_exec, n_useful);
         }
     }
     // block totals -> one statistics stripe
