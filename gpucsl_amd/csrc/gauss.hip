@@ -593,10 +593,12 @@ struct ItemCtx {
     const int *snb;    // LDS
     const float *sf;   // LDS
     const uint64_t *tab;  // LDS binomial table C(m, k), k = 1..L, m = 0..Nn; null: computed
+    float *prow;          // LDS, ROW mode: C[j, adj(i)] of the pair each team works on (d floats per team)
 };
 
 template <int L, int MODE>
-__device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, const int (&sp)[L], bool &near) {
+__device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, const int (&sp)[L], bool &near,
+                                          const float *rowJ_lds) {
     float cij, ci[L], cj[L], cm[L][L];
     if constexpr (MODE == 0) {
         const float *sf = c.sf;
@@ -611,7 +613,6 @@ __device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, con
         }
     } else if constexpr (MODE == 1) {
         const float *rowI = c.sf, *diag = c.sf + c.d;
-        const float *__restrict__ rowJ = c.C32 + (size_t)j * c.p;
         int S[L];
 #pragma unroll
         for (int a = 0; a < L; ++a) S[a] = c.snb[sp[a]];
@@ -619,7 +620,7 @@ __device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, con
 #pragma unroll
         for (int a = 0; a < L; ++a) {
             ci[a] = rowI[sp[a]];
-            cj[a] = ldg(rowJ + S[a]);
+            cj[a] = rowJ_lds[sp[a]];
 #pragma unroll
             for (int b = 0; b < a; ++b) cm[a][b] = ldg(c.C32 + (size_t)S[a] * c.p + S[b]);
             cm[a][a] = diag[sp[a]];
@@ -676,6 +677,24 @@ __device__ __forceinline__ void update_zmin(const Job &J, int p, int i, int j, f
     if (z < *zp) *zp = z;
 }
 
+// C[j, adj(i)] into LDS by `lanes` lanes (lane index `l`)
+__device__ __forceinline__ void stage_pair_row(const ItemCtx &c, int j, float *dst, int l, int lanes) {
+    const float *__restrict__ rowJ = c.C32 + (size_t)j * c.p;
+    for (int k0 = 0; k0 < c.d; k0 += lanes * 4) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * lanes + l;
+            v[u] = k < c.d ? ldg(rowJ + c.snb[k]) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * lanes + l;
+            if (k < c.d) dst[k] = v[u];
+        }
+    }
+}
+
 template <int L>
 __device__ __forceinline__ void subset_of(const ItemCtx &c, uint64_t rank, int jpos, int (&sp)[L]) {
     int pos[L];
@@ -701,6 +720,11 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
         if (c.side == 1 && Gm[(size_t)c.i * c.p + j] == 0) continue;
+        if constexpr (MODE == 1) {
+            __syncthreads();  // the previous pair is done with the row
+            stage_pair_row(c, j, c.prow, tid, 256);
+            __syncthreads();
+        }
         float rl = __builtin_inff();  // min |rho| over the tests counted so far
         for (uint64_t iter = 0;; ++iter) {
             const uint64_t rank = iter * 256 + tid;
@@ -710,7 +734,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
             int sp[L];
             if (act) {
                 subset_of<L>(c, rank, jpos, sp);
-                ar = rho_test<L, MODE>(c, jpos, j, sp, near);
+                ar = rho_test<L, MODE>(c, jpos, j, sp, near, c.prow);
                 pass = ar < c.rthr;
             }
             const unsigned long long pm = __ballot(pass), nm = __ballot(near);
@@ -783,6 +807,17 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
         if (c.side == 1 && Gm[(size_t)c.i * c.p + j] == 0) continue;
+        float *myrow = c.prow + (size_t)(threadIdx.x / team) * c.d;
+        if constexpr (MODE == 1) {
+            // the team's lanes stage C[j, adj(i)]; LDS accesses of one wave
+            // are ordered, the fences keep the compiler from moving them
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            stage_pair_row(c, j, myrow, gl, team);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         float rl = __builtin_inff();
         for (uint64_t iter = 0;; ++iter) {
             const uint64_t rank = iter * team + gl;
@@ -792,7 +827,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
             int sp[L];
             if (act) {
                 subset_of<L>(c, rank, jpos, sp);
-                ar = rho_test<L, MODE>(c, jpos, j, sp, near);
+                ar = rho_test<L, MODE>(c, jpos, j, sp, near, myrow);
                 pass = ar < c.rthr;
             }
             if (__ballot(near) & gmask) {
@@ -867,14 +902,17 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         c.jbase = side == 0 ? c.d - c.cnt : 0;
         c.D = c.d + 1;
         const int d = c.d, D = c.D, i = c.i, p = c.p;
-        const int mode = (L >= 2 && D * D + d <= WORDS) ? 0 : (3 * d <= WORDS ? 1 : 2);
+        const int teams = 256 / team;
+        // ROW mode: neighbour ids, C[i, adj], diag, and one pair row per team
+        const int mode = (L >= 2 && D * D + d <= WORDS) ? 0 : ((3 + teams) * d <= WORDS ? 1 : 2);
         c.gnb = jw.nbr + (size_t)i * p;
         int *snb = lds_words;
         float *sf = reinterpret_cast<float *>(lds_words + d);
         c.snb = snb;
         c.sf = sf;
+        c.prow = sf + 2 * d;
         // binomial table after the staged data, when it fits
-        const int tab_off = ((d + (mode == 0 ? D * D : 2 * d)) + 1) & ~1;
+        const int tab_off = ((d + (mode == 0 ? D * D : (2 + teams) * d)) + 1) & ~1;
         const bool use_tab = L >= 2 && mode != 2 && c.Nn >= L && tab_off + 2 * L * (c.Nn + 1) <= WORDS;
         uint64_t *tab = use_tab ? reinterpret_cast<uint64_t *>(lds_words + tab_off) : nullptr;
         c.tab = tab;

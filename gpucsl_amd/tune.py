@@ -25,7 +25,7 @@ import numpy as np
 from gpucsl_amd import _native as nat
 
 SPACE = {
-    "small_team_iters": (2, 4, 8, 16),
+    "small_team_iters": (1, 2, 4, 8),
     "wave_team_iters": (2, 4, 8),
     "fast_blocks_per_cu": (2, 4, 8),
     "fast_lds_bytes": (16384, 32768, 65536),
