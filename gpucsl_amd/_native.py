@@ -51,6 +51,7 @@ SIGNATURES = {
     "gpucsl_skeleton_to_u16": (_I, [_P, _I, _P, _P]),
     "gpucsl_unrank": (_I, [_I, _I, _P, _I, _P, _P]),
     "gpucsl_disc_workspace_bytes": (_SZ, [_I, _I, _I]),
+    "gpucsl_disc_min_workspace_bytes": (_SZ, [_I, _I, _I]),
     "gpucsl_disc_init": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_disc_level0": (_I, [_P, _I64, _I64, _P, _I, _I, _I, _D, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_disc_level": (_I, [_I, _P, _I64, _I64, _P, _I, _I, _I, _D, _P, _P, _P, _P, _P, _SZ, _P]),

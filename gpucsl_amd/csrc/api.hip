@@ -103,6 +103,7 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
                       unsigned long long *st, void *bws, size_t bws_bytes, hipStream_t s);
 int gauss_batch_finish(const Job *jh, const Job *jd, int nj, hipStream_t s);
 size_t disc_workspace_bytes(int p, int max_level, int max_dim);
+size_t disc_min_workspace_bytes(int p, int max_level, int max_dim);
 int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *sep, float *pmax, void *ws,
                      hipStream_t s);
 int disc_level0_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
@@ -360,9 +361,13 @@ size_t gpucsl_disc_workspace_bytes(int p, int max_level, int max_dim) {
     return disc_workspace_bytes(p, max_level, max_dim);
 }
 
+size_t gpucsl_disc_min_workspace_bytes(int p, int max_level, int max_dim) {
+    return disc_min_workspace_bytes(p, max_level, max_dim);
+}
+
 int gpucsl_disc_init(int p, int max_level, int max_dim, uint8_t *G, int32_t *sepsets, float *pmax,
                      uint64_t *stats, void *ws, size_t ws_bytes, void *stream) {
-    if (ws_bytes < disc_workspace_bytes(p, max_level, max_dim))
+    if (ws_bytes < disc_min_workspace_bytes(p, max_level, max_dim))
         return set_error(ERR_WORKSPACE, "disc_init: workspace too small");
     int rc = disc_init_launch(p, max_level, max_dim, G, sepsets, pmax, ws, S(stream));
     if (rc) return rc;

@@ -418,11 +418,19 @@ static int disc_grid() {
 }
 
 // Largest table (+ marginals) one test can need, in words.
+// Words of the largest contingency table plus marginals; saturates at
+// TABLE_WORDS_LIMIT (tables that large can never be held).
+constexpr size_t TABLE_WORDS_LIMIT = (size_t)1 << 48;
 static size_t max_table_words(int max_dim, int level) {
-    size_t cells = (size_t)max_dim * max_dim, ds = 1;
-    for (int t = 0; t < level; ++t) ds *= (size_t)max_dim;
-    cells *= ds;
-    return cells + ds * (2 * (size_t)max_dim + 1);
+    size_t ds = 1;
+    for (int t = 0; t < level; ++t) {
+        ds *= (size_t)max_dim;
+        if (ds >= TABLE_WORDS_LIMIT) return TABLE_WORDS_LIMIT;
+    }
+    const size_t cells = (size_t)max_dim * max_dim * ds;
+    if (cells >= TABLE_WORDS_LIMIT) return TABLE_WORDS_LIMIT;
+    const size_t w = cells + ds * (2 * (size_t)max_dim + 1);
+    return w >= TABLE_WORDS_LIMIT ? TABLE_WORDS_LIMIT : w;
 }
 
 static size_t gtab_words_for(int max_dim, int max_level) {
@@ -430,15 +438,38 @@ static size_t gtab_words_for(int max_dim, int max_level) {
     return need <= (size_t)DT_LDS_WORDS ? 0 : align_up(need, 64);
 }
 
-size_t disc_workspace_bytes(int p, int max_level, int max_dim) {
+// Workspace = a fixed part (level lists, test offsets, removal keys) plus one
+// global table slice per concurrently running block when the largest table
+// exceeds the LDS budget.  A smaller workspace runs fewer blocks at once.
+static size_t disc_fixed_bytes(int p) {
     size_t off = align_up(gauss_workspace_bytes(p), 256);
     off = align_up(off + sizeof(unsigned long long) * (size_t)(p + 1), 256);
     off = align_up(off + sizeof(unsigned long long) * (size_t)p * p, 256);
-    off = align_up(off + sizeof(uint32_t) * gtab_words_for(max_dim, max_level) * (size_t)disc_grid(), 256);
     return off;
 }
 
-static DiscWs disc_carve(void *ws, int p, int max_level, int max_dim) {
+size_t disc_workspace_bytes(int p, int max_level, int max_dim) {
+    if (max_table_words(max_dim, max_level) >= TABLE_WORDS_LIMIT) return SIZE_MAX;
+    return align_up(disc_fixed_bytes(p) + sizeof(uint32_t) * gtab_words_for(max_dim, max_level) * (size_t)disc_grid(),
+                    256);
+}
+
+size_t disc_min_workspace_bytes(int p, int max_level, int max_dim) {
+    if (max_table_words(max_dim, max_level) >= TABLE_WORDS_LIMIT) return SIZE_MAX;
+    return align_up(disc_fixed_bytes(p) + sizeof(uint32_t) * gtab_words_for(max_dim, max_level), 256);
+}
+
+// blocks that the workspace has table slices for (all of the grid when the
+// tables fit in LDS)
+static int disc_slices(size_t ws_bytes, int p, int max_level, int max_dim) {
+    const size_t words = gtab_words_for(max_dim, max_level);
+    if (words == 0) return disc_grid();
+    const size_t fixed = disc_fixed_bytes(p);
+    const size_t n = ws_bytes > fixed ? (ws_bytes - fixed) / (sizeof(uint32_t) * words) : 0;
+    return n < (size_t)disc_grid() ? (int)n : disc_grid();
+}
+
+static DiscWs disc_carve(void *ws, int p, int max_level, int max_dim, size_t ws_bytes) {
     DiscWs w;
     char *b = (char *)ws;
     size_t off = align_up(gauss_workspace_bytes(p), 256);
@@ -448,7 +479,7 @@ static DiscWs disc_carve(void *ws, int p, int max_level, int max_dim) {
     off = align_up(off + sizeof(unsigned long long) * (size_t)p * p, 256);
     w.gtab = (uint32_t *)(b + off);
     w.gtab_words = gtab_words_for(max_dim, max_level);
-    w.grid = disc_grid();
+    w.grid = disc_slices(ws_bytes, p, max_level, max_dim);
     return w;
 }
 
@@ -458,7 +489,7 @@ int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *se
     launch_fill<uint8_t>(Gm, (uint8_t)1, (size_t)p * p, s);
     if (max_level > 0 && sep) launch_fill<int32_t>(sep, -1, (size_t)p * p * max_level, s);
     launch_fill<float>(pmax, 0.0f, (size_t)p * p, s);
-    DiscWs w = disc_carve(ws, p, max_level, max_dim);
+    DiscWs w = disc_carve(ws, p, max_level, max_dim, disc_min_workspace_bytes(p, max_level, max_dim));
     launch_fill<unsigned long long>(w.keys, KEY_NONE, (size_t)p * p, s);
     return check_hip(hipGetLastError(), "disc_init launch");
 }
@@ -471,11 +502,11 @@ int disc_level0_launch(const uint8_t *data, int64_t n, int64_t stride, const uin
     if (stride < (n + 15) / 16 * 16 || stride % 16 != 0)
         return set_error(ERR_ARG, "disc_level0: stride must be a multiple of 16 and >= n rounded up to 16");
     if (((uintptr_t)data & 15) != 0) return set_error(ERR_ARG, "disc_level0: data must be 16-byte aligned");
-    if (ws_bytes < disc_workspace_bytes(p, max_level, max_dim))
+    if (ws_bytes < disc_min_workspace_bytes(p, max_level, max_dim))
         return set_error(ERR_WORKSPACE, "disc_level0: workspace too small");
     if (max_table_words(max_dim, 0) > (size_t)DT_LDS_WORDS && gtab_words_for(max_dim, max_level) < max_table_words(max_dim, 0))
         return set_error(ERR_WORKSPACE, "disc_level0: table workspace too small");
-    DiscWs w = disc_carve(ws, p, max_level, max_dim);
+    DiscWs w = disc_carve(ws, p, max_level, max_dim, ws_bytes);
     GPUCSL_LAUNCH("disc_level0", disc_level0_kernel, dim3(w.grid), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n,
                   stride, dims, p, alpha, Gm, pmax, w.gtab, w.gtab_words, st);
     return check_hip(hipGetLastError(), "disc_level0 launch");
@@ -503,13 +534,13 @@ int disc_level_launch(int level, const uint8_t *data, int64_t n, int64_t stride,
     if (!data || !dims || !Gm || !sep || !pmax || !st || !ws) return set_error(ERR_ARG, "disc_level: null pointer");
     if (stride < (n + 15) / 16 * 16 || stride % 16 != 0)
         return set_error(ERR_ARG, "disc_level: stride must be a multiple of 16 and >= n rounded up to 16");
-    if (ws_bytes < disc_workspace_bytes(p, max_level, max_dim))
+    if (ws_bytes < disc_min_workspace_bytes(p, max_level, max_dim))
         return set_error(ERR_WORKSPACE, "disc_level: workspace too small");
     int rc = compact_launch(Gm, p, level, ws, nullptr, s);
     if (rc) return rc;
     int32_t *nbr, *deg;
     level_list_views(ws, p, &nbr, &deg);
-    DiscWs w = disc_carve(ws, p, max_level, max_dim);
+    DiscWs w = disc_carve(ws, p, max_level, max_dim, ws_bytes);
     GPUCSL_LAUNCH("disc_testoff", disc_testoff_kernel, dim3(1), dim3(1024), 0, s, deg, p, level, w.testoff, st);
     switch (level) {
 #define GPUCSL_D(LV)                                                                                   \

@@ -42,14 +42,27 @@ class DiscreteResult:
     stats: list
 
 
-def disc_layout(n: int, p: int, max_level: int, max_dim: int):
+def disc_workspace(p: int, max_level: int, max_dim: int, limit: int | None = None) -> int:
+    """Workspace bytes: the full size (most tests at once), or `limit` when it
+    is smaller but still holds the minimum.  RuntimeError when the tables of
+    max_level can never fit (gpucsl/pc/kernel_management.py:434-442)."""
+    lib = nat.load()
+    full = int(lib.gpucsl_disc_workspace_bytes(p, max_level, max(max_dim, 1)))
+    least = int(lib.gpucsl_disc_min_workspace_bytes(p, max_level, max(max_dim, 1)))
+    if least == (1 << 64) - 1 or (limit is not None and limit < least):
+        raise RuntimeError(f"Out of memory for maximum level of {max_level}: the workspace needs at least "
+                           f"{least} bytes" + (f", the restriction is {limit}" if limit is not None else ""))
+    return full if limit is None or limit >= full else int(limit)
+
+
+def disc_layout(n: int, p: int, max_level: int, max_dim: int, ws_limit: int | None = None):
     """Byte offsets of one discrete job's buffers in a single arena, and its size."""
     stride = (n + COLUMN_ALIGN - 1) // COLUMN_ALIGN * COLUMN_ALIGN
     L = max(max_level, 1)
     pp = p * p
     sizes = [("data", p * stride), ("dims", p), ("G", pp), ("sep", 4 * pp * L), ("pmax", 4 * pp),
              ("skel16", 2 * pp), ("stats", 8 * (max_level + 1) * nat.STAT_SLOTS),
-             ("ws", int(nat.load().gpucsl_disc_workspace_bytes(p, max_level, max(max_dim, 1))))]
+             ("ws", disc_workspace(p, max_level, max_dim, ws_limit))]
     off, out = 0, {}
     for name, sz in sizes:
         out[name] = (off, sz)
@@ -58,9 +71,11 @@ def disc_layout(n: int, p: int, max_level: int, max_dim: int):
 
 
 class DiscreteEngine:
-    def __init__(self, p: int, n: int, max_level: int, alpha: float, max_dim: int, arena=None):
+    def __init__(self, p: int, n: int, max_level: int, alpha: float, max_dim: int, arena=None,
+                 ws_limit: int | None = None):
         """arena: optional uint8 device tensor of at least disc_layout(...)[1]
-        bytes that holds every buffer of the job (the serving pool)."""
+        bytes that holds every buffer of the job (the serving pool).
+        ws_limit: cap on the workspace bytes (fewer tests run at once)."""
         torch = nat.require_gpu()
         if p < 2:
             raise GpucslError("need at least two variables")
@@ -71,7 +86,7 @@ class DiscreteEngine:
         self.device = dev
         self.stride = (self.n + COLUMN_ALIGN - 1) // COLUMN_ALIGN * COLUMN_ALIGN
         L = max(self.max_level, 1)
-        layout, nbytes = disc_layout(self.n, self.p, self.max_level, self.max_dim)
+        layout, nbytes = disc_layout(self.n, self.p, self.max_level, self.max_dim, ws_limit)
         if arena is None:
             arena = torch.empty((nbytes,), dtype=torch.uint8, device=dev)
         elif arena.numel() < nbytes:

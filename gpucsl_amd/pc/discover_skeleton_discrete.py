@@ -26,9 +26,10 @@ def discover_skeleton_gpu_discrete(
 
     The data are ordinal-encoded on the host, uploaded once in a padded
     column-major layout and every level runs on the GPU.  ``memory_restriction``
-    bounds the workspace in bytes: a RuntimeError is raised if it is smaller
-    than what the tests of max_level need (the reference raises in the same
-    situation, gpucsl/pc/kernel_management.py:434-442).
+    bounds the workspace in bytes: below the full size fewer CI tests run at
+    once, and a RuntimeError is raised if it cannot hold one test of max_level
+    (the reference raises in the same situation,
+    gpucsl/pc/kernel_management.py:434-442).
     """
     import torch
 
@@ -38,10 +39,13 @@ def discover_skeleton_gpu_discrete(
         raise GpucslError(f"skeleton must be {num_variables}x{num_variables}")
     codes, dims = ordinal_encode(data)
     max_dim = int(dims.max()) if dims.size else 1
-    engine = DiscreteEngine(num_variables, num_observations, max_level, alpha, max_dim)
-    if memory_restriction is not None and engine.workspace_bytes() > memory_restriction:
-        raise RuntimeError(f"Out of memory for maximum level of {max_level}: the workspace needs "
-                           f"{engine.workspace_bytes()} bytes, the restriction is {memory_restriction}")
+    ci = getattr(kernels, "ci_test", None)
+    if memory_restriction is None and ci is not None and getattr(ci, "max_memory_size", None):
+        memory_restriction = ci.max_memory_size
+    # a restriction below the minimum workspace raises RuntimeError; between the
+    # minimum and the full size fewer tests run at once
+    engine = DiscreteEngine(num_variables, num_observations, max_level, alpha, max_dim,
+                            ws_limit=memory_restriction)
     engine.set_codes(codes, dims)
     torch.cuda.synchronize()
     start = timer()

@@ -192,7 +192,15 @@ int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void
  * data[v*stride + k], stride a multiple of 16 and >= n rounded up to 16, data
  * 16-byte aligned; dims[v] = number of categories of v; max_dim = max dims.
  * ------------------------------------------------------------------------- */
+/* Workspace of the discrete entry points; SIZE_MAX when the contingency tables
+ * of max_level can never be held (the out-of-memory case of
+ * find_execution_specification, gpucsl/pc/kernel_management.py:382-456). */
 size_t gpucsl_disc_workspace_bytes(int p, int max_level, int max_dim);
+/* The smallest workspace the discrete entry points accept: with less than
+ * gpucsl_disc_workspace_bytes they run fewer tests at once (one global table
+ * slice per running block), as find_execution_specification reduces
+ * parallelism under a memory restriction. */
+size_t gpucsl_disc_min_workspace_bytes(int p, int max_level, int max_dim);
 
 /* G = 1, sepsets = -1, pmax = 0, stats = 0, removal keys cleared.  Replaces the
  * buffer set-up of gpucsl/pc/discover_skeleton_discrete.py:41-55. */

@@ -142,3 +142,26 @@ def test_dropin_api_and_encoding():
     assert (res.seperation_sets == ref["sepsets"]).all()
     with pytest.raises(RuntimeError):
         discover_skeleton_gpu_discrete(init_pc_graph(raw), raw, 0.05, 2, 15, 3000, memory_restriction=10)
+
+
+def test_memory_restriction_runs_fewer_tests_at_once_same_result():
+    """Tables of max_dim=8 at level 3 exceed the LDS budget; a workspace with
+    room for three global table slices runs three tests at a time and must
+    give the same result as the full workspace."""
+    from gpucsl_amd import _native as nat
+    from gpucsl_amd.pc.discover_skeleton_discrete import discover_skeleton_gpu_discrete
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    d = syn.discrete_bn(12, 3000, 2.0, seed=31, card_low=6, card_high=8)
+    raw = d.data.astype(np.int64)
+    lib = nat.load()
+    full = int(lib.gpucsl_disc_workspace_bytes(12, 3, 8))
+    least = int(lib.gpucsl_disc_min_workspace_bytes(12, 3, 8))
+    slice_bytes = (full - least) // max(1, 1023)   # grid of 1024 blocks -> 1023 extra slices
+    assert least < full
+    a, _ = discover_skeleton_gpu_discrete(init_pc_graph(raw), raw, 0.05, 3, 12, 3000)
+    b, _ = discover_skeleton_gpu_discrete(init_pc_graph(raw), raw, 0.05, 3, 12, 3000,
+                                          memory_restriction=least + 2 * slice_bytes)
+    say(f"workspace full {full} min {least}")
+    assert (a.skeleton == b.skeleton).all() and (a.seperation_sets == b.seperation_sets).all()
+    assert np.array_equal(a.pmax.view(np.uint32), b.pmax.view(np.uint32))
