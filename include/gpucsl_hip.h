@@ -187,6 +187,18 @@ int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning);
 int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Edge orientation (host function; all pointers are HOST memory).
+ * skeleton: p x p bytes, symmetric 0/1; sepsets: p x p x max_level int32, -1
+ * padded (NULL when max_level is 0); dag: p x p bytes out, dag[i*p+j] = arc
+ * i -> j (both arcs = undirected edge).  v-structures, then Meek R1-R3 until
+ * no rule fires, in the visiting order of gpucsl/pc/edge_orientation/
+ * (v_structures.py:8-35, orientation_rules.py:7-89).  passes (may be NULL)
+ * receives the number of rule passes.  Returns 0, or 1 on bad arguments.
+ * ------------------------------------------------------------------------- */
+int gpucsl_orient_edges(const uint8_t *skeleton, int p, const int32_t *sepsets, int max_level, uint8_t *dag,
+                        int *passes);
+
+/* ---------------------------------------------------------------------------
  * discrete skeleton (Pearson chi^2)
  * data: uint8 category codes, column-major with a padded column stride:
  * data[v*stride + k], stride a multiple of 16 and >= n rounded up to 16, data

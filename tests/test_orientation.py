@@ -96,3 +96,48 @@ def test_orient_edges_is_a_pdag_of_the_skeleton():
     und = {frozenset(e) for e in dag.edges}
     want = {frozenset((a, b)) for a in range(p) for b in range(p) if skel[a, b]}
     assert und == want                                    # no edge added or lost
+
+
+def _python_orientation(skel, sep):
+    g = nx.DiGraph(skel)
+    dag = g.to_directed()
+    orient_v_structure(dag, sep, g)
+    apply_rules(dag)
+    return set(dag.edges)
+
+
+def test_native_orientation_equals_the_reference_order_restatement():
+    """The library's dense routine against the networkx restatement of the
+    reference's visiting order, on random skeletons with random sepsets."""
+    from gpucsl_amd.pc.edge_orientation.edge_orientation import orient_dense
+
+    rng = np.random.default_rng(7)
+    for trial in range(60):
+        p = int(rng.integers(3, 40))
+        dens = rng.uniform(0.05, 0.5)
+        skel = np.triu((rng.random((p, p)) < dens).astype(np.uint16), 1)
+        skel = skel + skel.T
+        L = int(rng.integers(0, 4))
+        sep = np.full((p, p, L), -1, np.int32)
+        for a in range(p):
+            for b in range(a + 1, p):
+                if not skel[a, b] and L and rng.random() < 0.7:
+                    k = int(rng.integers(1, L + 1))
+                    vals = np.sort(rng.choice(p, size=k, replace=False))
+                    sep[a, b, :k] = vals
+                    sep[b, a, :k] = vals
+        want = _python_orientation(skel, sep)
+        dag = orient_dense(skel, sep)
+        got = {(int(i), int(j)) for i, j in zip(*np.nonzero(dag))}
+        assert got == want, f"trial {trial} p={p}"
+
+
+def test_orient_edges_uses_native_and_matches_on_pc_output():
+    rng = np.random.default_rng(3)
+    p = 30
+    skel = np.triu((rng.random((p, p)) < 0.15).astype(np.uint16), 1)
+    skel = skel + skel.T
+    sep = np.full((p, p, 2), -1, np.int32)
+    dag, _ = orient_edges(nx.DiGraph(skel), sep)
+    assert set(dag.edges) == _python_orientation(skel, sep)
+    assert sorted(dag.nodes) == list(range(p))
