@@ -33,12 +33,23 @@ class GaussJob(ct.Structure):
     """ctypes mirror of struct gpucsl_gauss_job (include/gpucsl_hip.h)."""
     _fields_ = [
         ("p", ct.c_int32), ("max_level", ct.c_int32), ("n", ct.c_int64),
-        ("x_dtype", ct.c_int32), ("k_splits", ct.c_int32),
+        ("x_dtype", ct.c_int32), ("k_splits", ct.c_int32), ("flags", ct.c_int32), ("reserved", ct.c_int32),
         ("X", ct.c_void_p), ("C32", ct.c_void_p), ("C64", ct.c_void_p), ("G", ct.c_void_p),
         ("sepsets", ct.c_void_p), ("zmin", ct.c_void_p), ("pmax", ct.c_void_p),
         ("skeleton", ct.c_void_p), ("ws", ct.c_void_p), ("corr_ws", ct.c_void_p),
         ("thr", ct.c_double * MAX_LEVELS),
     ]
+
+
+PCALG_COMPAT = 1   # GPUCSL_JOB_PCALG_COMPAT
+
+
+def pcalg_thresholds(n: int, alpha: float, levels: int = MAX_LEVELS) -> np.ndarray:
+    """pcalg's |Phi^-1(alpha/2)| / sqrt(n - l - 3), 0.0 where n - l - 3 <= 0."""
+    from scipy.stats import norm
+
+    q = abs(norm.ppf(alpha / 2))
+    return np.array([q / np.sqrt(n - l - 3) if n - l - 3 > 0 else 0.0 for l in range(levels)], np.float64)
 
 
 def fisher_z_thresholds(n: int, alpha: float, levels: int = MAX_LEVELS) -> np.ndarray:
@@ -90,7 +101,8 @@ class GaussJobState:
     """HBM state of one Gaussian job inside one arena (a uint8 torch tensor)."""
 
     def __init__(self, n: int, p: int, max_level: int, alpha: float | None = None, thresholds=None,
-                 x_dtype: str = "float32", k_splits: int | None = None, device=None, arena=None):
+                 x_dtype: str = "float32", k_splits: int | None = None, device=None, arena=None,
+                 pcalg_compat: bool = False):
         torch = nat.require_gpu()
         if p < 2 or n < 5:
             raise GpucslError(f"job needs p >= 2 and n >= 5 (got p={p}, n={n})")
@@ -107,10 +119,13 @@ class GaussJobState:
         if k_splits is None:
             k_splits = int(nat.load().gpucsl_corr_k_splits(self.n, self.p))
         self.k_splits = max(1, int(k_splits))
+        # pcalg conventions: thresholds sqrt(n - l - 3), pMax over every test,
+        # one-way sepsets (GPUCSL_JOB_PCALG_COMPAT)
+        self.flags = PCALG_COMPAT if pcalg_compat else 0
         if thresholds is None:
             if alpha is None:
                 raise GpucslError("give alpha or thresholds")
-            thresholds = fisher_z_thresholds(self.n, float(alpha))
+            thresholds = (pcalg_thresholds if pcalg_compat else fisher_z_thresholds)(self.n, float(alpha))
         self.thr = thresholds_for_levels(thresholds, self.max_level)
         isz = 4 if x_dtype == "float32" else 8
         self.layout, self.nbytes = job_layout(self.n, self.p, self.max_level, isz, self.k_splits)
@@ -160,6 +175,7 @@ class GaussJobState:
         j.p, j.max_level, j.n = self.p, self.max_level, self.n
         j.x_dtype = 0 if self.x_dtype == "float32" else 1
         j.k_splits = self.k_splits
+        j.flags = self.flags
         j.X = self.ptr("X") if self.has_x else None
         for name in ("C32", "C64", "G", "sepsets", "zmin", "pmax", "skeleton", "ws", "corr_ws"):
             setattr(j, name, self.ptr(name))

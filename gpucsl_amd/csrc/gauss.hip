@@ -101,6 +101,16 @@ int get_level_tuning(int level, gpucsl_level_tuning *t) {
 enum BStat { BS_EXEC = 0, BS_USEFUL = 1, BS_RECHECK = 2, BS_FLIP = 3, BS_MAXRATIO = 4 };
 enum StatSlot { ST_EXEC = 0, ST_TOTAL = 1, ST_RECHECK = 2, ST_FLIP = 3, ST_MAXRATIO = 4, ST_MAXDEG = 5, ST_USEFUL = 6, ST_SLOTS = 8 };
 
+// zmin scale of a level's tests: 1, or sqrt(n - l - 3) in pcalg mode (so
+// that zmin orders tests by their p-value across levels)
+__device__ __forceinline__ float level_zscale(const Job &J, int level) {
+    if (!(J.flags & GPUCSL_JOB_PCALG_COMPAT)) return 1.0f;
+    const double m = (double)(J.n - level - 3);
+    return m > 0.0 ? (float)sqrt(m) : 0.0f;
+}
+
+__device__ __forceinline__ bool sepset_one_way(const Job &J) { return (J.flags & GPUCSL_JOB_PCALG_COMPAT) != 0; }
+
 __device__ __forceinline__ int job_of(const int32_t *__restrict__ base, int nj, long long t) {
     int lo = 0, hi = nj;
     while (hi - lo > 1) {
@@ -138,6 +148,7 @@ __global__ __launch_bounds__(256) void gauss_level0_kernel(const Job *__restrict
     const int b = blockIdx.x * 256 + threadIdx.x;
     const double thr = J.thr[0];
     const float thr32 = (float)thr;
+    const float zscale = level_zscale(J, 0);
     uint8_t *__restrict__ Gm = J.G;
     unsigned long long n_re = 0, n_flip = 0;
     float mr = 0.0f;
@@ -167,7 +178,7 @@ __global__ __launch_bounds__(256) void gauss_level0_kernel(const Job *__restrict
             const uint8_t keep = pass ? 0 : 1;
             Gm[(size_t)a * p + b] = keep;
             Gm[(size_t)b * p + a] = keep;
-            if (!(zf != zf)) J.zmin[(size_t)a * p + b] = zf;
+            if (!(zf != zf)) J.zmin[(size_t)a * p + b] = zf * zscale;
         }
     }
     // executed = total = useful = p(p-1)/2 per job; only the rare recheck
@@ -587,7 +598,7 @@ struct ItemCtx {
     const Job *J;
     int p, i, d, D, Nn, cnt, jbase, q0, q1, job, side;
     uint64_t count;
-    float rthr, margin_k;
+    float rthr, margin_k, zscale;
     const int32_t *gnb, *rowoff;
     const float *C32;
     const int *snb;    // LDS
@@ -663,11 +674,12 @@ __device__ __forceinline__ void write_sepset(const ItemCtx &c, int j, const int 
     J.G[(size_t)j * p + i] = 0;
     int32_t *sep = J.sepsets;
     const int maxL = J.max_level;
+    const bool both = !sepset_one_way(J);
 #pragma unroll
     for (int k = 0; k < L; ++k) {
         const int v = nbr_at<L, MODE>(c, sp[k]);
         sep[((size_t)i * p + j) * maxL + k] = v;
-        sep[((size_t)j * p + i) * maxL + k] = v;
+        if (both) sep[((size_t)j * p + i) * maxL + k] = v;
     }
 }
 
@@ -779,7 +791,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
                         const unsigned int slot = atomicAdd(qcount, 1u);
                         queue[slot] = SlowItem{iter * 256, c.rowoff[c.i] + q, c.job, zb, 0};
                     } else {
-                        update_zmin(*c.J, c.p, c.i, j, zb);
+                        update_zmin(*c.J, c.p, c.i, j, zb * c.zscale);
                     }
                 }
             }
@@ -858,7 +870,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
             }
             if (finish) {
                 for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
-                if (gl == 0) update_zmin(*c.J, c.p, c.i, j, z_of_rho(rl));
+                if (gl == 0) update_zmin(*c.J, c.p, c.i, j, z_of_rho(rl) * c.zscale);
                 break;
             }
         }
@@ -892,6 +904,7 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         c.C32 = J.C32;
         c.rthr = (float)tanh(J.thr[L]);
         c.margin_k = margin_k;
+        c.zscale = level_zscale(J, L);
         c.d = jw.deg[c.i];
         c.cnt = c.rowoff[c.i + 1] - c.rowoff[c.i];
         c.Nn = c.d - 1;
@@ -1155,7 +1168,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
 #pragma unroll
                 for (int c = 0; c < L; ++c) {
                     J->sepsets[((size_t)i * p + j) * maxL + c] = S[c];
-                    J->sepsets[((size_t)j * p + i) * maxL + c] = S[c];
+                    if (!sepset_one_way(*J)) J->sepsets[((size_t)j * p + i) * maxL + c] = S[c];
                 }
             }
             finish = true;
@@ -1173,7 +1186,8 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
             if (gl == 0) {
                 const int a = i < j ? i : j, b = i < j ? j : i;
                 float *zp = J->zmin + (size_t)a * p + b;
-                if (zl < *zp) *zp = zl;
+                const float zs = zl * level_zscale(*J, L);
+                if (zs < *zp) *zp = zs;
             }
             have = false;
         }
@@ -1265,7 +1279,9 @@ __global__ void job_finish_kernel(const Job *__restrict__ jobs) {
     const int a = blockIdx.y;
     if (b >= p || a >= p) return;
     const size_t t = (size_t)a * p + b;
-    if (J.pmax) J.pmax[t] = b <= a ? -1.0f : pmax_of(J.zmin[t], sqrt((double)(J.n - 3)));
+    // pcalg mode: zmin already carries sqrt(n - |S| - 3)
+    const double s = (J.flags & GPUCSL_JOB_PCALG_COMPAT) ? 1.0 : sqrt((double)(J.n - 3));
+    if (J.pmax) J.pmax[t] = b <= a ? -1.0f : pmax_of(J.zmin[t], s);
     if (J.skeleton) J.skeleton[t] = J.G[t];
 }
 

@@ -39,11 +39,16 @@ def discover_skeleton_gpu_gaussian(
     should_log: bool = False,
     devices: List[int] = [0],
     sync_device: int = None,
+    pcalg_compat: bool = False,
 ) -> SkeletonResult:
     """Signature and result of gpucsl/pc/discover_skeleton_gaussian.py:116-172.
 
     ``skeleton`` contributes its shape only (level 0 sets every pair, as in
     gpucsl/pc/cuda/gaussian_ci.cu:39-50).  ``devices`` must name one GPU.
+    ``pcalg_compat`` (no counterpart in gpucsl) switches to pcalg's
+    conventions: thresholds |Phi^-1(alpha/2)| / sqrt(n - l - 3), pmax the largest
+    p-value of any test (each with its own sqrt(n - |S| - 3)), and the sepset
+    stored only for the ordered pair whose test removed the edge.
     """
     import torch
 
@@ -52,6 +57,9 @@ def discover_skeleton_gpu_gaussian(
     validate_given_devices(devices, sync_device)
     if skeleton is not None and tuple(np.shape(skeleton)) != (num_variables, num_variables):
         raise GpucslError(f"skeleton must be {num_variables}x{num_variables}")
+    if pcalg_compat:
+        return _pcalg_skeleton(data, correlation_matrix, alpha, max_level, num_variables, num_observations,
+                               devices[0], kernels)
     thresholds = get_gaussian_thresholds(data, alpha)
     group_size, margin = _tuning(kernels)
     with torch.cuda.device(devices[0]):
@@ -81,4 +89,28 @@ def discover_skeleton_gpu_gaussian(
             for level, st in enumerate(res.stats):
                 log(f"level {level}: {st}")
 
+    return SkeletonResult(res.skeleton, res.separation_sets, res.pmax, computation_time)
+
+
+def _pcalg_skeleton(data, correlation_matrix, alpha, max_level, p, n, device, kernels) -> SkeletonResult:
+    """pcalg conventions through the batched entry points (a batch of one job
+    carrying GPUCSL_JOB_PCALG_COMPAT)."""
+    import torch
+
+    from gpucsl_amd.batch import GaussJobState, GaussianBatch
+
+    group_size, margin = _tuning(kernels)
+    with torch.cuda.device(device):
+        job = GaussJobState(n, p, max_level, alpha=alpha, pcalg_compat=True)
+        if correlation_matrix is None:
+            job.set_data(np.asarray(data, np.float64).astype(np.float32))
+        else:
+            job.set_correlation(correlation_matrix)
+        batch = GaussianBatch([job], margin=margin, group_size=group_size)
+        torch.cuda.synchronize()
+        start = timer()
+        batch.launch()
+        res = job.results()
+        computation_time = timer() - start
+        log_time("Computation time", computation_time)
     return SkeletonResult(res.skeleton, res.separation_sets, res.pmax, computation_time)

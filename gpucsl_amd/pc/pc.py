@@ -56,10 +56,13 @@ class GaussianPC(PC):
         return discover_skeleton_gpu_gaussian(
             self.graph, self.data, self.correlation_matrix, self.alpha, self.max_level,
             self.num_variables, self.num_observations, self.kernels, self.is_debug,
-            self.should_log, self.devices, self.sync_device)
+            self.should_log, self.devices, self.sync_device, self.pcalg_compat)
 
     def set_distribution_specific_options(self, devices: List[int] = [0], sync_device: int = None,
-                                          correlation_matrix: np.ndarray = None):
+                                          correlation_matrix: np.ndarray = None, pcalg_compat: bool = False):
+        """pcalg_compat (no counterpart in gpucsl): pcalg's thresholds, pMax and
+        one-way separation sets instead of gpucsl's."""
+        self.pcalg_compat = pcalg_compat
         warn_against_too_high_memory_consumption_gaussian_pc(self.max_level, self.num_variables)
         assert len(devices) > 0, "Invalid device list, must not be empty"
         validate_given_devices(devices, sync_device)

@@ -77,12 +77,21 @@ int gpucsl_corr_c64_to_c32(const double *C64, int p, float *C32, void *stream);
  * step and may be NULL when C64 is supplied.  The single-dataset entry points
  * below build one of these internally; the batched entry points take an array
  * of them, the same array in host memory and in device memory. */
+/* pcalg's conventions instead of gpucsl's: the caller passes thresholds
+ * |Phi^-1(alpha/2)| / sqrt(n - l - 3); zmin holds min over tests of
+ * sqrt(n - |S| - 3) |z| (so pmax = 2 (1 - Phi(zmin)) is pcalg's pMax, the
+ * largest p-value of any test), and a sepset is stored only for the ordered
+ * pair whose test removed the edge. */
+#define GPUCSL_JOB_PCALG_COMPAT 1
+
 typedef struct gpucsl_gauss_job {
     int32_t p;
     int32_t max_level;
     int64_t n;
     int32_t x_dtype;
     int32_t k_splits;   /* split-K factor of the correlation GEMM for this job */
+    int32_t flags;      /* GPUCSL_JOB_* bits */
+    int32_t reserved;
     const void *X;
     float *C32;
     double *C64;

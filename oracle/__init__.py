@@ -56,6 +56,8 @@ def lib():
         L.or_compact.argtypes = [_P, ct.c_int, _P]
         L.or_gauss_skeleton.restype = ct.c_int
         L.or_gauss_skeleton.argtypes = [_P, ct.c_int, _P, ct.c_int, _P, _P, _P, _P, _P]
+        L.or_gauss_skeleton_ex.restype = ct.c_int
+        L.or_gauss_skeleton_ex.argtypes = [_P, ct.c_int, _P, ct.c_int, _i64, ct.c_int, _P, _P, _P, _P, _P]
         L.or_pmax_from_zmin.argtypes = [_P, ct.c_int, _i64, _P]
         L.or_chi2_table.restype = ct.c_double
         L.or_chi2_table.argtypes = [_P, _P, _P, _P, ct.c_int, ct.c_int, ct.c_int, ct.c_int]
@@ -129,12 +131,21 @@ def thresholds(n: int, alpha: float, levels: int = 50) -> np.ndarray:
     return thr
 
 
-def gauss_skeleton(C: np.ndarray, n: int, alpha: float, max_level: int, thr=None) -> dict:
-    """FP64 Gaussian PC-stable skeleton on a correlation matrix."""
+def pcalg_thresholds(n: int, alpha: float, levels: int = 50) -> np.ndarray:
+    """pcalg's |Phi^-1(alpha/2)| / sqrt(n - l - 3) (0 where n - l - 3 <= 0)."""
+    from scipy.stats import norm
+
+    q = abs(norm.ppf(alpha / 2))
+    return np.array([q / np.sqrt(n - l - 3) if n - l - 3 > 0 else 0.0 for l in range(levels)])
+
+
+def gauss_skeleton(C: np.ndarray, n: int, alpha: float, max_level: int, thr=None, pcalg_compat: bool = False) -> dict:
+    """FP64 Gaussian PC-stable skeleton on a correlation matrix.  pcalg_compat:
+    pcalg's thresholds, pMax (largest p-value of any test) and one-way sepsets."""
     C = np.ascontiguousarray(C, np.float64)
     p = C.shape[0]
     if thr is None:
-        thr = thresholds(n, alpha)
+        thr = pcalg_thresholds(n, alpha) if pcalg_compat else thresholds(n, alpha)
     thr = np.ascontiguousarray(thr, np.float64)
     L = max(max_level, 1)
     G = np.zeros((p, p), np.uint8)
@@ -142,10 +153,11 @@ def gauss_skeleton(C: np.ndarray, n: int, alpha: float, max_level: int, thr=None
     zmin = np.empty((p, p), np.float32)
     ex = np.zeros(max_level + 1, np.int64)
     tot = np.zeros(max_level + 1, np.int64)
-    last = lib().or_gauss_skeleton(_ptr(C), p, _ptr(thr), max_level, _ptr(G), _ptr(sep),
-                                   _ptr(zmin), _ptr(ex), _ptr(tot))
+    last = lib().or_gauss_skeleton_ex(_ptr(C), p, _ptr(thr), max_level, ct.c_int64(n), 1 if pcalg_compat else 0,
+                                      _ptr(G), _ptr(sep), _ptr(zmin), _ptr(ex), _ptr(tot))
     pmax = np.empty((p, p), np.float32)
-    lib().or_pmax_from_zmin(_ptr(zmin), p, n, _ptr(pmax))
+    # pcalg mode: zmin already carries sqrt(n - |S| - 3); n = 4 makes the scale 1
+    lib().or_pmax_from_zmin(_ptr(zmin), p, 4 if pcalg_compat else n, _ptr(pmax))
     return dict(skeleton=G.astype(np.uint16), sepsets=sep[:, :, :max_level], zmin=zmin,
                 pmax=pmax, executed=ex, total=tot, last_level=last)
 

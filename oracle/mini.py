@@ -135,3 +135,58 @@ def disc_skeleton(data: np.ndarray, dims: np.ndarray, alpha: float, max_level: i
             G[a, b] = G[b, a] = False
             sep[(a, b)] = tuple(S)
     return G.astype(np.uint16), sep, int(ties)
+
+
+def gauss_skeleton_pcalg(C: np.ndarray, n: int, alpha: float, max_level: int, margin: float = 1e-9):
+    """Textbook restatement of PC-stable with pcalg's conventions: a test of
+    (i, j | S) has z = sqrt(n - |S| - 3) atanh|rho| and p = 2 (1 - Phi(z)); the
+    edge goes when p >= alpha; pMax[a, b] is the largest p of the tests run on
+    the pair; the sepset is recorded for the ordered pair (i, j) whose test
+    removed the edge only.  Ordered pairs are tried (a, b) then (b, a), a < b."""
+    from scipy.stats import norm
+
+    p = C.shape[0]
+    G = np.ones((p, p), bool)
+    np.fill_diagonal(G, False)
+    sep = {}
+    pmax = np.full((p, p), -1.0)
+    ties = 0
+    q = norm.isf(alpha / 2)
+
+    def test(i, j, S):
+        nonlocal ties
+        zz = np.sqrt(n - len(S) - 3) * (_z(C, i, j, S) if S else abs(np.arctanh(C[i, j])))
+        ties += abs(zz - q) < margin
+        a, b = min(i, j), max(i, j)
+        pmax[a, b] = max(pmax[a, b], 2 * norm.sf(zz))
+        return zz < q
+
+    for a in range(p):
+        for b in range(a + 1, p):
+            if test(a, b, ()):
+                G[a, b] = G[b, a] = False
+                sep[(a, b)] = ()
+    for l in range(1, max_level + 1):
+        nbrs = [list(np.flatnonzero(G[r])) for r in range(p)]
+        if max(len(x) for x in nbrs) - 1 < l:
+            break
+        removed = []
+        for a in range(p):
+            for b in nbrs[a]:
+                if b <= a:
+                    continue
+                found = None
+                for vi, vj in ((a, b), (b, a)):
+                    cand = [v for v in nbrs[vi] if v != vj]
+                    for S in itertools.combinations(cand, l):
+                        if test(vi, vj, S):
+                            found = (vi, vj, S)
+                            break
+                    if found is not None:
+                        break
+                if found is not None:
+                    removed.append(found)
+        for vi, vj, S in removed:
+            G[vi, vj] = G[vj, vi] = False
+            sep[(vi, vj)] = tuple(S)
+    return G.astype(np.uint16), sep, pmax, int(ties)

@@ -362,6 +362,12 @@ static void fmin_into(float *dst, double z) {
 /* Gaussian PC-stable skeleton                                                */
 /* ------------------------------------------------------------------------- */
 
+/* sqrt(n - l - 3), the scale of pcalg's z statistic for |S| = l */
+static double level_scale(int64_t n, int l) {
+    const double m = (double)(n - l - 3);
+    return m > 0.0 ? sqrt(m) : 0.0;
+}
+
 /* Level 0: gpucsl/pc/cuda/gaussian_ci.cu:18-51 (upper-triangle pairs, zmin at
  * [min*p+max], removal on Z < thr, diagonal cleared).
  * Levels >= 1: gpucsl/pc/discover_skeleton_gaussian.py:30-113 and
@@ -370,10 +376,14 @@ static void fmin_into(float *dst, double z) {
  * G: p*p uint8 (out), sep: p*p*max_level int32 (out, -1 filled here),
  * zmin: p*p float (out), executed/total: per-level counters (out, length
  * max_level+1).  Returns the last level that ran. */
-OR_EXPORT int or_gauss_skeleton(const double *C, int p, const double *thr, int max_level,
-                                uint8_t *G, int32_t *sep, float *zmin, int64_t *executed,
-                                int64_t *total) {
+/* flags & 1 (pcalg conventions): zmin holds min over tests of
+ * sqrt(n - |S| - 3) |z| (pcalg's pMax = 2 (1 - Phi(zmin))), and the sepset is
+ * stored only for the ordered pair whose test removed the edge.  The caller
+ * passes pcalg's thresholds |Phi^-1(alpha/2)| / sqrt(n - l - 3). */
+OR_EXPORT int or_gauss_skeleton_ex(const double *C, int p, const double *thr, int max_level, int64_t n, int flags,
+                                   uint8_t *G, int32_t *sep, float *zmin, int64_t *executed, int64_t *total) {
     const int L = max_level > 0 ? max_level : 1;
+    const int pcalg = flags & 1;
     for (size_t t = 0; t < (size_t)p * p; ++t) zmin[t] = INFINITY;
     for (size_t t = 0; t < (size_t)p * p * (size_t)L; ++t) sep[t] = -1;
     for (int l = 0; l <= max_level; ++l) executed[l] = total[l] = 0;
@@ -382,7 +392,7 @@ OR_EXPORT int or_gauss_skeleton(const double *C, int p, const double *thr, int m
         G[(size_t)a * p + a] = 0;
         for (int b = a + 1; b < p; ++b) {
             double z = or_gauss_z0(C[(size_t)a * p + b]);
-            fmin_into(&zmin[(size_t)a * p + b], z);
+            fmin_into(&zmin[(size_t)a * p + b], pcalg ? z * level_scale(n, 0) : z);
             uint8_t keep = (z < thr[0]) ? 0 : 1;
             G[(size_t)a * p + b] = G[(size_t)b * p + a] = keep;
         }
@@ -422,12 +432,12 @@ OR_EXPORT int or_gauss_skeleton(const double *C, int p, const double *thr, int m
                         or_resolve(nbr + (size_t)vi * p, pos, l, vj, S);
                         double z = or_gauss_z(C, p, vi, vj, S, l);
                         ++ex;
-                        fmin_into(&zl, z);
+                        fmin_into(&zl, pcalg ? z * level_scale(n, l) : z);
                         if (z < thr[l]) {
                             found = 1;
                             for (int c = 0; c < l; ++c) {
-                                sep[((size_t)a * p + b) * L + c] = S[c];
-                                sep[((size_t)b * p + a) * L + c] = S[c];
+                                sep[((size_t)vi * p + vj) * L + c] = S[c];
+                                if (!pcalg) sep[((size_t)vj * p + vi) * L + c] = S[c];
                             }
                             break;
                         }
@@ -444,6 +454,12 @@ OR_EXPORT int or_gauss_skeleton(const double *C, int p, const double *thr, int m
     free(nbr);
     free(deg);
     return last;
+}
+
+OR_EXPORT int or_gauss_skeleton(const double *C, int p, const double *thr, int max_level,
+                                uint8_t *G, int32_t *sep, float *zmin, int64_t *executed,
+                                int64_t *total) {
+    return or_gauss_skeleton_ex(C, p, thr, max_level, 4, 0, G, sep, zmin, executed, total);
 }
 
 /* pmax = 2 (1 - Phi(sqrt(n-3) |zmin|)) = erfc(sqrt(n-3)|zmin| / sqrt 2), then

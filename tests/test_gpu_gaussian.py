@@ -245,3 +245,26 @@ def test_single_dataset_graph_replay_matches_eager():
         assert np.array_equal(got.pmax.view(np.uint32), want.pmax.view(np.uint32))
         assert [s["useful"] for s in got.stats] == [s["useful"] for s in want.stats]
         del junk
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_pcalg_mode_matches_oracle(seed):
+    """pcalg conventions (thresholds sqrt(n-l-3), pMax over every test with its
+    own scale, one-way sepsets) against the oracle's pcalg mode."""
+    from gpucsl_amd.pc.discover_skeleton_gaussian import discover_skeleton_gpu_gaussian
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    X = syn.gaussian_sem(50, 2000, 3.0, seed=seed).data.astype(np.float32)
+    res, _ = discover_skeleton_gpu_gaussian(init_pc_graph(X), X, None, 0.01, 3, 50, 2000, pcalg_compat=True)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), 2000, 0.01, 3, pcalg_compat=True)
+    assert (res.skeleton == ref["skeleton"]).all()
+    assert (res.seperation_sets == ref["sepsets"]).all()
+    one_way = [(a, b) for a in range(50) for b in range(50)
+               if ref["sepsets"][a, b, 0] != -1 and ref["sepsets"][b, a, 0] == -1]
+    iu = np.triu_indices(50, 1)
+    dp = np.abs(res.pmax[iu].astype(np.float64) - ref["pmax"][iu]).max()
+    say(f"[pcalg seed {seed}] edges {int(res.skeleton.sum()) // 2}, one-way sepsets {len(one_way)}, "
+        f"max|dpmax| {dp:.3g}")
+    assert dp <= 2e-5
+    plain = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), 2000, 0.01, 3)
+    assert (np.abs(plain["pmax"][iu] - ref["pmax"][iu]).max() > 0) or len(one_way) > 0

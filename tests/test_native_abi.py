@@ -52,15 +52,15 @@ def test_product_does_not_import_oracle():
 
 
 def test_job_struct_layout_matches_header():
-    """The ctypes mirror of gpucsl_gauss_job has the C layout: 4+4+8+4+4 bytes
-    of scalars, ten pointers, nine doubles (x86-64 / gfx950 alignment)."""
+    """The ctypes mirror of gpucsl_gauss_job has the C layout: 4+4+8+4+4+4+4
+    bytes of scalars, ten pointers, nine doubles (x86-64 / gfx950 alignment)."""
     import ctypes as ct
 
     from gpucsl_amd.batch import GaussJob, MAX_LEVELS
 
     assert MAX_LEVELS == 9
-    assert GaussJob.n.offset == 8 and GaussJob.X.offset == 24 and GaussJob.corr_ws.offset == 96
-    assert GaussJob.thr.offset == 104 and ct.sizeof(GaussJob) == 176
+    assert GaussJob.n.offset == 8 and GaussJob.flags.offset == 24 and GaussJob.X.offset == 32
+    assert GaussJob.corr_ws.offset == 104 and GaussJob.thr.offset == 112 and ct.sizeof(GaussJob) == 184
     src = (ROOT / "include" / "gpucsl_hip.h").read_text()
     body = src[src.index("typedef struct gpucsl_gauss_job"):src.index("} gpucsl_gauss_job;")]
     names = [ln.split(";")[0].replace("*", " ").split()[-1].split("[")[0]

@@ -155,3 +155,37 @@ def test_disc_level0_stat_is_pearson():
     want = stats.chi2_contingency(table, correction=False)[0]
     assert chi2 == pytest.approx(want, rel=1e-12)
     assert df == 6
+
+
+def test_pcalg_mode_matches_textbook_restatement():
+    """The oracle's pcalg mode (thresholds sqrt(n-l-3), pMax over every test,
+    one-way sepsets) against the brute-force restatement in oracle/mini.py."""
+    import oracle
+    from oracle import mini
+    from gpucsl_amd import synthetic as syn
+
+    for seed in range(4):
+        d = syn.gaussian_sem(10, 300, 3.0, seed=70 + seed).data
+        C = oracle.corrcoef(d)
+        got = oracle.gauss_skeleton(C, 300, 0.05, 3, pcalg_compat=True)
+        G, sep, pmax, ties = mini.gauss_skeleton_pcalg(C, 300, 0.05, 3)
+        assert ties == 0
+        assert (got["skeleton"] == G).all()
+        p = 10
+        for a in range(p):
+            for b in range(p):
+                if (a, b) in sep and sep[(a, b)] != ():
+                    assert list(got["sepsets"][a, b][: len(sep[(a, b)])]) == list(sep[(a, b)])
+                elif not ((b, a) in sep and a > b and sep[(b, a)] == ()) and not ((a, b) in sep):
+                    assert (got["sepsets"][a, b] == -1).all(), (a, b)
+        iu = np.triu_indices(p, 1)
+        assert np.abs(got["pmax"][iu] - pmax[iu]).max() < 2e-6
+
+
+def test_pcalg_thresholds_closed_form():
+    import oracle
+    from scipy.stats import norm
+
+    thr = oracle.pcalg_thresholds(100, 0.01, 10)
+    for l in range(10):
+        assert thr[l] == pytest.approx(norm.isf(0.005) / np.sqrt(100 - l - 3), rel=1e-14)
