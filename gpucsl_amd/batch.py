@@ -306,5 +306,4 @@ def run_plain(jobs: list, margin: float = DEFAULT_MARGIN, group_size: int = DEFA
         nat.call("gpucsl_gauss_skeleton", j.ptr("C32"), j.ptr("C64"), p, j.max_level, thr.ctypes.data,
                  float(margin), int(group_size), j.ptr("G"), j.ptr("sepsets"), j.ptr("zmin"),
                  j._plain_stats.data_ptr(), j._plain_ws.data_ptr(), ws_bytes, s)
-        nat.call("gpucsl_pmax_from_zmin", j.ptr("zmin"), p, j.n, j.ptr("pmax"), s)
-        nat.call("gpucsl_skeleton_to_u16", j.ptr("G"), p, j.ptr("skeleton"), s)
+        nat.call("gpucsl_gauss_outputs", j.ptr("zmin"), j.ptr("G"), p, j.n, j.ptr("pmax"), j.ptr("skeleton"), s)

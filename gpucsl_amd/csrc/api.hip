@@ -86,12 +86,13 @@ int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, 
 int gauss_level_launch(int level, const float *C32, const double *C64, int p, int max_level, double thr,
                        float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
                        unsigned long long *stats, void *ws, size_t ws_bytes, hipStream_t s);
-int gauss_skeleton_levels(const float *C32, const double *C64, int p, int max_level, const double *thr,
+int gauss_skeleton_launch(const float *C32, const double *C64, int p, int max_level, const double *thr,
                           float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
-                          unsigned long long *stats, void *ws, size_t ws_bytes, hipStream_t s);
+                          unsigned long long *stats, int nst, void *ws, size_t ws_bytes, hipStream_t s);
 int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats, hipStream_t s);
 void level_list_views(void *ws, int p, int32_t **nbr, int32_t **deg);
 int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s);
+int outputs_launch(const float *zmin, const uint8_t *Gm, int p, int64_t n, float *pmax, uint16_t *out, hipStream_t s);
 size_t gauss_batch_ws_bytes(const Job *jobs_host, int nj);
 int set_level_tuning(int level, const gpucsl_level_tuning *t);
 int get_level_tuning(int level, gpucsl_level_tuning *t);
@@ -246,10 +247,9 @@ int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_le
                           void *stream) {
     if (!thr || !stats) return set_error(ERR_ARG, "gauss_skeleton: null thr or stats");
     if (!C32 || !C64) return set_error(ERR_ARG, "gauss_skeleton: null correlation matrix");
-    int rc = gpucsl_gauss_init(p, max_level, G, sepsets, zmin, stats, stream);
-    if (rc) return rc;
-    rc = gauss_skeleton_levels(C32, C64, p, max_level, thr, margin, group_size, G, sepsets, zmin,
-                               reinterpret_cast<unsigned long long *>(stats), ws, ws_bytes, S(stream));
+    int rc = gauss_skeleton_launch(C32, C64, p, max_level, thr, margin, group_size, G, sepsets, zmin,
+                                   reinterpret_cast<unsigned long long *>(stats), (max_level + 1) * GPUCSL_STAT_SLOTS,
+                                   ws, ws_bytes, S(stream));
     if (rc) return rc;
     return check_unbuilt_levels(
         max_level, stats,
@@ -481,6 +481,11 @@ int gpucsl_profile_end(char *out, size_t cap) {
         if (txt.size() >= cap) return set_error(ERR_ARG, "profile_end: buffer too small (%zu needed)", txt.size() + 1);
     }
     return OK;
+}
+
+int gpucsl_gauss_outputs(const float *zmin, const uint8_t *G, int p, int64_t n, float *pmax, uint16_t *skeleton,
+                         void *stream) {
+    return outputs_launch(zmin, G, p, n, pmax, skeleton, S(stream));
 }
 
 int gpucsl_skeleton_to_u16(const uint8_t *G, int p, uint16_t *out, void *stream) {

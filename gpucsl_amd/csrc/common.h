@@ -49,7 +49,7 @@ __host__ __device__ __forceinline__ uint64_t binom(int64_t m, int k) {
 // C(m, K) for a compile-time K: K multiply/divide steps by constants (the
 // divisions compile to multiply-high sequences).
 template <int K>
-__device__ __forceinline__ uint64_t binom_k(int m) {
+__host__ __device__ __forceinline__ uint64_t binom_k(int m) {
     if (m < K) return 0;
     uint64_t r = 1;
 #pragma unroll
@@ -407,6 +407,22 @@ inline void launch_fill(T *dst, T value, size_t count, hipStream_t s) {
 }
 
 __host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Compute units of the current device, read once per process (256 when no
+// device answers, e.g. on a host without a GPU, where only sizes are asked).
+inline int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, nn = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&nn, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nn <= 0) {
+            (void)hipGetLastError();
+            nn = 256;
+        }
+        cus = nn;
+    }
+    return cus;
+}
 
 // ----------------------------------------------------------------------------
 // optional per-launch event timing (enabled by gpucsl_profile_begin; never

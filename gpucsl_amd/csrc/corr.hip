@@ -35,8 +35,7 @@ __host__ __device__ inline int corr_row_splits(int64_t n) {
 // row groups per block (group g takes rows g, g+4, ... of the split), the four
 // group sums added in order.
 template <typename T>
-__global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ jobs) {
-    const Job &J = jobs[blockIdx.z];
+__device__ __forceinline__ void colsum_body(const Job &J) {
     const int p = J.p;
     const int64_t n = J.n;
     const int rs = corr_row_splits(n);
@@ -66,6 +65,19 @@ __global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ job
     __syncthreads();
     if (g == 0 && a < p)
         carve_corr(J.corr_ws, p).partial[(size_t)s * p + a] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ jobs) {
+    colsum_body<T>(jobs[blockIdx.z]);
+}
+
+// One job passed by value; block (0, 0) also stores its descriptor at dst for
+// the launches that follow, so a single dataset needs no separate write.
+template <typename T>
+__global__ __launch_bounds__(256) void colsum_one_kernel(Job one, Job *dst) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *dst = one;
+    colsum_body<T>(one);
 }
 
 // gram[s] (p x p, only tiles with ta <= tb written) = Zs^T Zs over k-range s,
@@ -243,34 +255,65 @@ __global__ __launch_bounds__(256) void normalize_tile_kernel(const Job *__restri
     const double nm1 = (double)(J.n - 1);
     const size_t pp = (size_t)p * p;
     const int tid = threadIdx.x;
-    auto gsum = [&](int a, int b) {
-        double g = 0.0;
-        for (int q = 0; q < ks; ++q) g += cw.gram[(size_t)q * pp + (size_t)a * p + b];
-        return g;
+    // Sums of the ks partials of R elements in split order.  The loads go out
+    // four splits at a time for all R elements (clamped to split ks-1, whose
+    // surplus values are not added) before any add waits on them.
+    auto gsum_rows = [&](auto &g, const size_t (&off)[sizeof(g) / sizeof(double)]) {
+        constexpr int R = sizeof(g) / sizeof(double);
+#pragma unroll
+        for (int e = 0; e < R; ++e) g[e] = 0.0;
+        for (int q0 = 0; q0 < ks; q0 += 4) {
+            double x[R][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const size_t base = (size_t)(q0 + u < ks ? q0 + u : ks - 1) * pp;
+#pragma unroll
+                for (int e = 0; e < R; ++e) x[e][u] = cw.gram[base + off[e]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (q0 + u < ks)
+#pragma unroll
+                    for (int e = 0; e < R; ++e) g[e] += x[e][u];
+        }
     };
     __shared__ double s_sdr[QR], s_sdc[CT];
     __shared__ double s_v[QR][CT + 1];
     if (tid < CT + QR) {
         const int idx = tid < CT ? b0 + tid : a0 + (tid - CT);
-        const double sd = idx < p ? sqrt(gsum(idx, idx) / nm1) : 1.0;
+        double g[1];
+        const size_t off[1] = {idx < p ? (size_t)idx * p + idx : 0};
+        gsum_rows(g, off);
+        const double sd = idx < p ? sqrt(g[0] / nm1) : 1.0;
         if (tid < CT)
             s_sdc[tid] = sd;
         else
             s_sdr[tid - CT] = sd;
     }
     __syncthreads();
+    // thread: column c, rows r0, r0 + 4, r0 + 8, r0 + 12 of the quarter
+    constexpr int RPT = QR / 4;
     const int c = tid & 63, r0 = tid >> 6;
-    for (int r = r0; r < QR; r += 4) {
-        const int a = a0 + r, b = b0 + c;
+    const int b = b0 + c;
+    size_t off[RPT];
+#pragma unroll
+    for (int e = 0; e < RPT; ++e) {
+        const int a = a0 + r0 + 4 * e;
+        // lower half of a diagonal tile: the mirrored upper element
+        off[e] = (a < p && b < p) ? (a <= b ? (size_t)a * p + b : (size_t)b * p + a) : 0;
+    }
+    double g[RPT];
+    gsum_rows(g, off);
+#pragma unroll
+    for (int e = 0; e < RPT; ++e) {
+        const int r = r0 + 4 * e, a = a0 + r;
         double v = 0.0;
         if (a < p && b < p) {
+            v = g[e] / nm1;
             if (a <= b) {
-                v = gsum(a, b) / nm1;
                 v = v / s_sdr[r];
                 v = v / s_sdc[c];
             } else {
-                // lower half of a diagonal tile: the mirrored upper element
-                v = gsum(b, a) / nm1;
                 v = v / s_sdc[c];
                 v = v / s_sdr[r];
             }
@@ -304,17 +347,34 @@ __global__ void c64_to_c32_kernel(const Job *__restrict__ jobs) {
 // ----------------------------------------------------------------------------
 // host side
 // ----------------------------------------------------------------------------
-// split-K factor that gives one dataset >= 1024 workgroups
+// Split-K factor of one dataset's Gram matrix: ks in 1..16 (at least 256 rows
+// per split) that minimises the busiest compute unit's work when the
+// tiles x ks blocks spread evenly over the device, ceil(tiles ks / CUs) blocks
+// of ceil(n / ks / KT) slabs each, plus the split-K partial tiles (written
+// once, read once; about four slabs' time per block).  Depends on (n, p) and
+// the device only, so a dataset gets the same ks, and the same bits, alone and
+// in any batch.
 int corr_k_splits(int64_t n, int p) {
-    const int tu = upper_tiles(p);
-    int ks = (1024 + tu - 1) / tu;
-    const int64_t kmax = (n + 255) / 256;
-    if (ks > kmax) ks = (int)kmax;
-    if (ks > 16) ks = 16;
-    return ks < 1 ? 1 : ks;
+    const int64_t tu = upper_tiles(p);
+    const int64_t cus = device_cus();
+    int64_t kmax = (n + 255) / 256;
+    if (kmax > 16) kmax = 16;
+    int best = 1;
+    double best_cost = 0.0;
+    for (int64_t ks = 1; ks <= (kmax < 1 ? 1 : kmax); ++ks) {
+        const int64_t rows = (n + ks - 1) / ks;
+        const int64_t slabs = (rows + KT - 1) / KT;
+        const double cost = (double)((tu * ks + cus - 1) / cus) * (double)slabs + 4.0 * (double)(tu * ks) / (double)cus;
+        if (ks == 1 || cost < best_cost) {
+            best = (int)ks;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
-int corr_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStream_t s) {
+// With `one` (nj == 1), jobs_dev is where the first launch stores *one.
+static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, const Job *one, hipStream_t s) {
     if (nj < 1) return OK;
     int maxp = 0, maxks = 1, maxtiles = 1, dtype = jobs_host[0].x_dtype;
     int64_t maxn = 0;
@@ -331,7 +391,12 @@ int corr_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStre
     if (dtype != 0 && dtype != 1) return set_error(ERR_ARG, "corr: dtype must be 0 (f32) or 1 (f64)");
     const int rs = corr_row_splits(maxn);
     const int cb = (maxp + 63) / 64;
-    if (dtype == 0)
+    Job *dst = const_cast<Job *>(jobs_dev);
+    if (one && dtype == 0)
+        GPUCSL_LAUNCH("corr_colsum", colsum_one_kernel<float>, dim3(cb, rs, 1), dim3(256), 0, s, *one, dst);
+    else if (one)
+        GPUCSL_LAUNCH("corr_colsum", colsum_one_kernel<double>, dim3(cb, rs, 1), dim3(256), 0, s, *one, dst);
+    else if (dtype == 0)
         GPUCSL_LAUNCH("corr_colsum", colsum_kernel<float>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
     else
         GPUCSL_LAUNCH("corr_colsum", colsum_kernel<double>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
@@ -343,6 +408,10 @@ int corr_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStre
                       jobs_dev);
     GPUCSL_LAUNCH("corr_normalize", normalize_tile_kernel, dim3(maxtiles * NQ, 1, nj), dim3(256), 0, s, jobs_dev);
     return check_hip(hipGetLastError(), "corr launch");
+}
+
+int corr_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStream_t s) {
+    return corr_jobs_launch(jobs_host, jobs_dev, nj, nullptr, s);
 }
 
 int c64_to_c32_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, hipStream_t s) {
@@ -377,9 +446,7 @@ int corr_launch(const void *X, int dtype, int64_t n, int p, double *C64, float *
     j.C32 = C32;
     j.C64 = C64;
     j.corr_ws = (char *)ws + JOB_DESC_BYTES;
-    Job *dj = (Job *)ws;
-    hipLaunchKernelGGL(write_job_kernel, dim3(1), dim3(1), 0, s, dj, j);
-    return corr_batch_launch(&j, dj, 1, s);
+    return corr_jobs_launch(&j, (Job *)ws, 1, &j, s);
 }
 
 __global__ void c64_to_c32_flat_kernel(const double *__restrict__ C64, size_t count, float *__restrict__ C32) {

@@ -405,17 +405,7 @@ int lower_minus_one_launch(float *pmax, int p, hipStream_t s) {
     return check_hip(hipGetLastError(), "pmax finalize launch");
 }
 
-static int disc_grid() {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0, nn = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&nn, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nn <= 0)
-            nn = 256;
-        cus = nn;
-    }
-    return cus * DT_BLOCKS_PER_CU;
-}
+static int disc_grid() { return device_cus() * DT_BLOCKS_PER_CU; }
 
 // Largest table (+ marginals) one test can need, in words.
 // Words of the largest contingency table plus marginals; saturates at

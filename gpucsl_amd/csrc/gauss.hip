@@ -40,6 +40,7 @@ constexpr int FB_MAX_LDS_BYTES = 65536;
 struct Team {
     int gs_min, small_max, wave_max;  // count limits of the gs_min and wave teams
     int item_tests;                   // subset tests a work item aims at
+    int wave_items;                   // 1: every wave takes whole items on its own (teams of <= 64 lanes)
 };
 
 __host__ __device__ __forceinline__ int team_lanes(uint64_t count, const Team &t) {
@@ -51,10 +52,26 @@ __host__ __device__ __forceinline__ int team_lanes(uint64_t count, const Team &t
 // Pairs per work item: enough for every team of the block, and about
 // item_tests subset tests in all.
 __host__ __device__ __forceinline__ int items_pairs(uint64_t count, const Team &t) {
-    const int teams = 256 / team_lanes(count, t);
-    const uint64_t want = count ? ((uint64_t)t.item_tests + count - 1) / count : 1;
-    const int w = want > (1u << 20) ? (1 << 20) : (int)want;
-    return w > teams ? w : teams;
+    const int teams = (t.wave_items ? 64 : 256) / team_lanes(count, t);
+    // ceil(item_tests / count); item_tests < 2^25, so a 32-bit division
+    const uint32_t it = (uint32_t)t.item_tests;
+    const uint32_t w = count == 0 ? 1u : (count >= it ? 1u : (it + (uint32_t)count - 1u) / (uint32_t)count);
+    return (int)w > teams ? (int)w : teams;
+}
+
+// C(m, level) with the level's divisions by constants
+__host__ __device__ __forceinline__ uint64_t binom_level(int m, int level) {
+    switch (level) {
+        case 1: return m >= 1 ? (uint64_t)m : 0;
+        case 2: return binom_k<2>(m);
+        case 3: return binom_k<3>(m);
+        case 4: return binom_k<4>(m);
+        case 5: return binom_k<5>(m);
+        case 6: return binom_k<6>(m);
+        case 7: return binom_k<7>(m);
+        case 8: return binom_k<8>(m);
+        default: return binom(m, level);
+    }
 }
 
 // per-level launch shapes of the FAST kernel (settable at run time, e.g. from
@@ -64,14 +81,15 @@ static bool g_tuning_init = false;
 
 static void tuning_defaults() {
     if (g_tuning_init) return;
-    for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = gpucsl_level_tuning{8, 8, 4, 32768, 4096};
+    for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = gpucsl_level_tuning{8, 8, 4, 32768, 4096, 0};
     g_tuning_init = true;
 }
 
 static Team team_for(int level, int gs_min) {
     tuning_defaults();
     const gpucsl_level_tuning &t = g_tuning[level];
-    return Team{gs_min, gs_min * t.small_team_iters, 64 * t.wave_team_iters, t.item_tests};
+    if (t.wave_items) return Team{gs_min, gs_min * t.small_team_iters, 0x7fffffff, t.item_tests, 1};
+    return Team{gs_min, gs_min * t.small_team_iters, 64 * t.wave_team_iters, t.item_tests, 0};
 }
 
 int set_level_tuning(int level, const gpucsl_level_tuning *t) {
@@ -79,7 +97,7 @@ int set_level_tuning(int level, const gpucsl_level_tuning *t) {
     if (!t) return set_error(ERR_ARG, "tuning: null");
     if (t->small_team_iters < 1 || t->wave_team_iters < 1 || t->fast_blocks_per_cu < 1 || t->fast_blocks_per_cu > 32 ||
         t->fast_lds_bytes < 4096 || t->fast_lds_bytes > FB_MAX_LDS_BYTES || t->fast_lds_bytes % 1024 ||
-        t->item_tests < 64 || t->item_tests > (1 << 24))
+        t->item_tests < 64 || t->item_tests > (1 << 24) || t->wave_items < 0 || t->wave_items > 1)
         return set_error(ERR_ARG, "tuning: a field is out of range");
     if (level == 0) {
         for (int l = 1; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = *t;
@@ -203,40 +221,6 @@ __global__ __launch_bounds__(256) void gauss_level0_kernel(const Job *__restrict
     }
 }
 
-// ----------------------------------------------------------------------------
-// compaction: one wave per row (gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56)
-// nbr[row] = ascending neighbours (diagonal excluded), deg, and the split into
-// neighbours below (cntB) and above (cntA) the row index.
-// ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jobs, unsigned int *__restrict__ plan_done) {
-    if (plan_done && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *plan_done = 0;
-    const Job &J = jobs[blockIdx.y];
-    const int p = J.p;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= p) return;
-    JobWs w = carve_job(J.ws, p);
-    const int lane = __lane_id();
-    const uint8_t *g = J.G + (size_t)row * p;
-    int32_t *out = w.nbr + (size_t)row * p;
-    int base = 0, below = 0;
-    for (int c0 = 0; c0 < p; c0 += 64) {
-        const int c = c0 + lane;
-        const bool e = c < p && c != row && g[c] != 0;
-        const uint64_t m = __ballot(e);
-        if (e) out[base + __popcll(m & lanemask_lt())] = c;
-        base += __popcll(m);
-        if (c0 + 64 <= row)
-            below += __popcll(m);
-        else if (c0 < row)
-            below += __popcll(m & ((row - c0) >= 64 ? ~0ull : ((1ull << (row - c0)) - 1)));
-    }
-    if (lane == 0) {
-        w.deg[row] = base;
-        w.cntB[row] = below;
-        w.cntA[row] = base - below;
-    }
-}
-
 struct PlanArgs {
     unsigned int *done;            // blocks finished; null: scan only
     int nj;
@@ -252,9 +236,6 @@ __device__ void items_body(const Job *__restrict__ jobs, int nj, int level, cons
                            const int32_t *__restrict__ baseA, const int32_t *__restrict__ baseB,
                            WorkItem *__restrict__ itemsA, WorkItem *__restrict__ itemsB);
 
-// Per job (one block each): exclusive scans of cntA/cntB into rowoffA/rowoffB
-// and of their work items into itemoffA/itemoffB; adds T_l and max degree to
-// the launch statistics.  A job whose max level is below `level` gets no work.
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = __lane_id();
 #pragma unroll
@@ -265,19 +246,28 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
     return x;
 }
 
-__global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs, int level, Team team,
-                                                    unsigned long long *__restrict__ st, PlanArgs plan) {
-    const Job &J = jobs[blockIdx.x];
+// One job's level plan, by one block of NT threads (NT = 256 or 1024):
+// exclusive scans of cntA/cntB into rowoffA/rowoffB and of the rows' work
+// items into itemoffA/itemoffB; adds T_l and the max degree to the launch
+// statistics.  A job whose max level is below `level` gets no work.  With
+// `single` (a launch of one job) it also writes the work-item lists, the item
+// bases and resets the launch counters, so the level needs no other planning.
+//
+// With `pre`, the compaction launch has already left each row's item counts
+// in itemoffA/itemoffB and its pairs per item in rowoffA; the scans read them
+// before overwriting them with the offsets.
+template <int NT>
+__device__ void scan_job(const Job &J, int level, const Team &team, unsigned long long *__restrict__ st,
+                         const PlanArgs &plan, bool single, bool pre) {
+    constexpr int NW = NT / 64;
     const int p = J.p;
     const bool active = level <= J.max_level;
     JobWs w = carve_job(J.ws, p);
-    __shared__ int sA[1024], sB[1024];           // job prefix of a batch (last block)
-    __shared__ int s_wsum[4][16], s_woff[4][16], s_tot[4];
-    __shared__ unsigned long long s_T[16];
-    __shared__ int s_M[16];
+    __shared__ int s_wsum[4][NW], s_woff[4][NW], s_tot[4];
+    __shared__ unsigned long long s_T[NW];
+    __shared__ int s_M[NW];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const bool single = plan.done && plan.nj == 1;
-    const int per = (p + 1023) / 1024;
+    const int per = (p + NT - 1) / NT;
     const int r0 = t * per;
     int r1 = r0 + per;
     if (r1 > p) r1 = p;
@@ -288,8 +278,11 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         la += ca;
         lb += cb;
         const int d = w.deg[r];
-        const uint64_t count = active ? binom(d - 1, level) : 0;
-        if (count) {
+        const uint64_t count = active ? binom_level(d - 1, level) : 0;
+        if (pre) {
+            lia += w.itemoffA[r];
+            lib += w.itemoffB[r];
+        } else if (count) {
             const int ppi = items_pairs(count, team);
             lia += (ca + ppi - 1) / ppi;
             lib += (cb + ppi - 1) / ppi;
@@ -300,7 +293,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         }
     }
     // block-wide exclusive scans of the four counts: wave scans, then a scan
-    // of the 16 wave totals
+    // of the NW wave totals
     const int ia = wave_incl_scan(la), ib = wave_incl_scan(lb), iia = wave_incl_scan(lia), iib = wave_incl_scan(lib);
     for (int off = 32; off >= 1; off >>= 1) {
         tt += __shfl_xor(tt, off);
@@ -318,18 +311,18 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         s_M[wv] = md;
     }
     __syncthreads();
-    if (t < 16) {
+    if (t < NW) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             int x = s_wsum[k][t];
             const int own = x;
 #pragma unroll
-            for (int off = 1; off < 16; off <<= 1) {
+            for (int off = 1; off < NW; off <<= 1) {
                 const int y = __shfl_up(x, off);
                 if (t >= off) x += y;
             }
             s_woff[k][t] = x - own;
-            if (t == 15) s_tot[k] = x;
+            if (t == NW - 1) s_tot[k] = x;
         }
     }
     __syncthreads();
@@ -337,16 +330,26 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
     int eia = s_woff[2][wv] + iia - lia, eib = s_woff[3][wv] + iib - lib;
     for (int r = r0; r < r1; ++r) {
         const int ca = active ? w.cntA[r] : 0, cb = active ? w.cntB[r] : 0;
+        int na = 0, nb = 0, ppi = 1;
+        if (pre) {
+            na = w.itemoffA[r];
+            nb = w.itemoffB[r];
+            ppi = w.rowoffA[r];
+        } else {
+            const uint64_t count = active ? binom_level(w.deg[r] - 1, level) : 0;
+            if (count) {
+                ppi = items_pairs(count, team);
+                na = (ca + ppi - 1) / ppi;
+                nb = (cb + ppi - 1) / ppi;
+            }
+        }
         w.rowoffA[r] = ea;
         w.rowoffB[r] = eb;
         w.itemoffA[r] = eia;
         w.itemoffB[r] = eib;
         ea += ca;
         eb += cb;
-        const uint64_t count = active ? binom(w.deg[r] - 1, level) : 0;
-        if (count) {
-            const int ppi = items_pairs(count, team);
-            const int na = (ca + ppi - 1) / ppi, nb = (cb + ppi - 1) / ppi;
+        if (na + nb) {
             if (single) {
                 // one job: its item lists start at 0
                 for (int k = 0; k < na; ++k) plan.itemsA[eia + k] = WorkItem{0, r, k * ppi, 0};
@@ -364,7 +367,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
         if (st) {
             unsigned long long T = 0;
             int M = 0;
-            for (int k = 0; k < 16; ++k) {
+            for (int k = 0; k < NW; ++k) {
                 T += s_T[k];
                 M = s_M[k] > M ? s_M[k] : M;
             }
@@ -372,23 +375,95 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
             atomicMax(&st[ST_MAXDEG], (unsigned long long)M);
         }
     }
-    if (!plan.done) return;
-    if (single) {
-        if (t < NUM_COUNTERS) plan.counters[t] = 0;
-        if (t < BSTAT_STRIPES * BSTAT_WORDS) plan.bstats[t] = 0;
-        if (t == 0) {
-            plan.baseA[0] = 0;
-            plan.baseA[1] = s_tot[2];
-            plan.baseB[0] = 0;
-            plan.baseB[1] = s_tot[3];
-        }
-        return;
+    if (!single) return;
+    for (int k = t; k < NUM_COUNTERS; k += NT) plan.counters[k] = 0;
+    for (int k = t; k < BSTAT_STRIPES * BSTAT_WORDS; k += NT) plan.bstats[k] = 0;
+    if (t == 0) {
+        plan.baseA[0] = 0;
+        plan.baseA[1] = s_tot[2];
+        plan.baseB[0] = 0;
+        plan.baseB[1] = s_tot[3];
     }
-    // the block that finishes last plans the level for every job: job bases,
-    // counter resets and the work-item lists
+}
+
+// ----------------------------------------------------------------------------
+// compaction: one wave per row (gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56)
+// nbr[row] = ascending neighbours (diagonal excluded), deg, and the split into
+// neighbours below (cntB) and above (cntA) the row index.  In a planning
+// launch (plan.done set) each row's wave also leaves its share of the plan for
+// the scan launch: item counts in itemoffA/itemoffB and pairs per item in
+// rowoffA, until the scans replace them with offsets.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jobs, int level, Team team,
+                                                      unsigned long long *__restrict__ st, PlanArgs plan) {
+    const bool pre = plan.done != nullptr;
+    if (pre && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *plan.done = 0;
+    const Job &J = jobs[blockIdx.y];
+    const int p = J.p;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    JobWs w = carve_job(J.ws, p);
+    const bool active = level <= J.max_level;
+    if (row < p) {
+        const int lane = __lane_id();
+        const uint8_t *g = J.G + (size_t)row * p;
+        int32_t *out = w.nbr + (size_t)row * p;
+        int base = 0, below = 0;
+        // 1024 columns per group: the group's 16 loads go out before its first
+        // store (the byte loads may alias the stores, so they are not reordered)
+        constexpr int U = 16;
+        for (int g0 = 0; g0 < p; g0 += 64 * U) {
+            uint8_t v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int c = g0 + u * 64 + lane;
+                v[u] = c < p ? g[c] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int c0 = g0 + u * 64, c = c0 + lane;
+                const bool e = v[u] != 0 && c != row;
+                const uint64_t m = __ballot(e);
+                if (e) out[base + __popcll(m & lanemask_lt())] = c;
+                base += __popcll(m);
+                if (c0 + 64 <= row)
+                    below += __popcll(m);
+                else if (c0 < row)
+                    below += __popcll(m & ((1ull << (row - c0)) - 1));
+            }
+        }
+        if (lane == 0) {
+            w.deg[row] = base;
+            w.cntB[row] = below;
+            w.cntA[row] = base - below;
+            if (pre) {
+                const uint64_t count = active ? binom_level(base - 1, level) : 0;
+                int ppi = 1, na = 0, nb = 0;
+                if (count) {
+                    ppi = items_pairs(count, team);
+                    na = (base - below + ppi - 1) / ppi;
+                    nb = (below + ppi - 1) / ppi;
+                }
+                w.itemoffA[row] = na;
+                w.itemoffB[row] = nb;
+                w.rowoffA[row] = ppi;
+            }
+        }
+    }
+}
+
+// Per job (one block each) scan_job; in a batch with a plan, the block that
+// finishes last plans the level for every job: job bases, counter resets and
+// the work-item lists.
+__global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs, int level, Team team,
+                                                    unsigned long long *__restrict__ st, PlanArgs plan) {
+    const Job &J = jobs[blockIdx.x];
+    __shared__ int sA[1024], sB[1024];  // job prefix of a batch (last block)
+    const bool single = plan.done && plan.nj == 1;
+    scan_job<1024>(J, level, team, st, plan, single, plan.done != nullptr);
+    if (!plan.done || single) return;
     __shared__ int s_last;
     __syncthreads();
-    if (t == 0) {
+    if (threadIdx.x == 0) {
         __threadfence();
         s_last = atomicAdd(plan.done, 1u) == (unsigned int)(plan.nj - 1);
     }
@@ -458,7 +533,7 @@ __device__ void items_body(const Job *__restrict__ jobs, int nj, int level, cons
         const int p = J.p;
         JobWs w = carve_job(J.ws, p);
         for (int r = threadIdx.x; r < p; r += blockDim.x) {
-            const uint64_t count = binom(w.deg[r] - 1, level);
+            const uint64_t count = binom_level(w.deg[r] - 1, level);
             if (!count) continue;
             const int ppi = items_pairs(count, team);
             const int na = w.itemoffA[r + 1] - w.itemoffA[r], nb = w.itemoffB[r + 1] - w.itemoffB[r];
@@ -1031,6 +1106,75 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
     }
 }
 
+// Level l >= 1, FAST phase, one wave per work item: every wave of a block
+// takes whole items on its own and its teams (gs_min lanes, or 64) pull the
+// item's pairs, reading C from global memory (L2); no LDS staging and no block
+// barriers, so a wave never waits for another.
+template <int L>
+__global__ __launch_bounds__(256) void gauss_level_wave_kernel(
+    const Job *__restrict__ jobs, int nj, const int32_t *__restrict__ itembase, const WorkItem *__restrict__ items,
+    int side, Team tm, float margin_k, SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount,
+    unsigned long long *__restrict__ bstats) {
+    __shared__ int s_next[4];
+    __shared__ unsigned long long s_e[4], s_u[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nitems = itembase[nj];
+    const int nwaves = gridDim.x * 4;
+    unsigned long long n_exec = 0, n_useful = 0;
+    for (int item = blockIdx.x * 4 + wave; item < nitems; item += nwaves) {
+        const WorkItem it = items[item];
+        const Job &J = jobs[it.job];
+        ItemCtx c;
+        c.J = &J;
+        c.p = J.p;
+        c.i = it.row;
+        c.job = it.job;
+        c.side = side;
+        JobWs jw = carve_job(J.ws, c.p);
+        c.rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
+        c.C32 = J.C32;
+        c.rthr = (float)tanh(J.thr[L]);
+        c.margin_k = margin_k;
+        c.zscale = level_zscale(J, L);
+        c.d = jw.deg[c.i];
+        c.cnt = c.rowoff[c.i + 1] - c.rowoff[c.i];
+        c.Nn = c.d - 1;
+        c.count = binom_k<L>(c.Nn);
+        c.q0 = it.q0;
+        c.q1 = min(c.cnt, c.q0 + items_pairs(c.count, tm));
+        c.jbase = side == 0 ? c.d - c.cnt : 0;
+        c.D = c.d + 1;
+        c.gnb = jw.nbr + (size_t)c.i * c.p;
+        c.snb = nullptr;
+        c.sf = nullptr;
+        c.tab = nullptr;
+        c.prow = nullptr;
+        if (lane == 0) s_next[wave] = c.q0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        team_pairs<L, 2>(c, team_lanes(c.count, tm), &s_next[wave], queue, qcount, n_exec, n_useful);
+        __builtin_amdgcn_wave_barrier();  // every team is done with s_next before lane 0 resets it
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        n_exec += __shfl_xor(n_exec, off);
+        n_useful += __shfl_xor(n_useful, off);
+    }
+    if (lane == 0) {
+        s_e[wave] = n_exec;
+        s_u[wave] = n_useful;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned long long e = s_e[0] + s_e[1] + s_e[2] + s_e[3];
+        const unsigned long long u = s_u[0] + s_u[1] + s_u[2] + s_u[3];
+        if (e) {
+            unsigned long long *dst = bstats + (size_t)(blockIdx.x % BSTAT_STRIPES) * BSTAT_WORDS;
+            atomicAdd(dst + BS_EXEC, e);
+            atomicAdd(dst + BS_USEFUL, u);
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------
 // level l >= 1, SLOW phase: finishes the pairs the FAST kernel queued, from the
 // queued subset rank on, rechecking near-threshold FP32 tests in FP64 with the
@@ -1285,6 +1429,16 @@ __global__ void job_finish_kernel(const Job *__restrict__ jobs) {
     if (J.skeleton) J.skeleton[t] = J.G[t];
 }
 
+// pmax_kernel and the uint16 skeleton of one dataset in one pass
+__global__ void outputs_kernel(const float *__restrict__ zmin, const uint8_t *__restrict__ g, int p, double sqrt_nm3,
+                               float *__restrict__ pmax, uint16_t *__restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (b >= p) return;
+    const size_t t = (size_t)a * p + b;
+    pmax[t] = b <= a ? -1.0f : pmax_of(zmin[t], sqrt_nm3);
+    out[t] = g[t];
+}
 __global__ void to_u16_kernel(const uint8_t *__restrict__ g, size_t count, uint16_t *__restrict__ out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1294,17 +1448,6 @@ __global__ void to_u16_kernel(const uint8_t *__restrict__ g, size_t count, uint1
 // ----------------------------------------------------------------------------
 // host side: launches over a job table
 // ----------------------------------------------------------------------------
-static int device_cus() {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0, nn = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&nn, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nn <= 0)
-            nn = 256;
-        cus = nn;
-    }
-    return cus;
-}
 static int slow_grid() { return device_cus() / 2; }
 static int fast_grid(int level) {
     tuning_defaults();
@@ -1350,9 +1493,9 @@ int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, uns
 static int compact_and_plan(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
                             const PlanArgs &plan, hipStream_t s) {
     const int mp = max_p(jh, nj);
-    GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd, plan.done);
     const Team team = team_for(level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
                                gs_min);
+    GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd, level, team, st, plan);
     GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, team, st, plan);
     return check_hip(hipGetLastError(), "compact launch");
 }
@@ -1378,8 +1521,12 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
     unsigned int *qcount = w.counters + CNT_QUEUE + side;
     unsigned int *slow_ctr = w.counters + (2 * side + 1) * STRIPES;
     const int lds = g_tuning[L].fast_lds_bytes;
-    GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj, base,
-                  items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
+    if (g_tuning[L].wave_items)
+        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_wave_kernel<L>, dim3(fast_grid(L)), dim3(256), 0, s, jd, nj,
+                      base, items, side, team_for(L, gs_min), margin_k, queue, qcount, w.bstats);
+    else
+        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj,
+                      base, items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
     GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 16>), dim3(slow_grid()), dim3(256), 0, s, jd, side,
                   margin_k, slow_ctr, queue, qcount, w.bstats, w.counters + CNT_STATS_DONE, side == 1 ? st : nullptr);
 }
@@ -1472,11 +1619,14 @@ int single_write_desc(const Job &j, void *ws, hipStream_t s) {
     return check_hip(hipGetLastError(), "job descriptor");
 }
 
-// G = 1, zmin = +inf, sepsets = -1 and the statistics zeroed, in one launch
+// G = 1, zmin = +inf, sepsets = -1 and the statistics zeroed, in one launch;
+// with `dst`, also the job descriptor of a single-dataset workspace.
 __global__ void single_init_kernel(uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, float *__restrict__ zmin,
-                                   size_t pp, size_t ns, unsigned long long *__restrict__ st, int nst) {
+                                   size_t pp, size_t ns, unsigned long long *__restrict__ st, int nst, Job *dst,
+                                   Job src) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dst && t0 == 0) *dst = src;
     for (size_t t = t0; t < pp; t += stride) {
         Gm[t] = 1;
         zmin[t] = __builtin_inff();
@@ -1486,16 +1636,21 @@ __global__ void single_init_kernel(uint8_t *__restrict__ Gm, int32_t *__restrict
     if (st && t0 < (size_t)nst) st[t0] = 0;
 }
 
-int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, unsigned long long *st, int nst,
-                      hipStream_t s) {
+static int init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, unsigned long long *st, int nst,
+                       Job *dst, const Job &src, hipStream_t s) {
     if (p < 1 || max_level < 0 || !Gm || !zmin) return set_error(ERR_ARG, "gauss_init: bad arguments");
     const size_t pp = (size_t)p * p, ns = max_level > 0 && sep ? pp * max_level : 0;
     size_t blocks = ((ns > pp ? ns : pp) + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     if (blocks * 256 < (size_t)nst) blocks = ((size_t)nst + 255) / 256;
     GPUCSL_LAUNCH("gauss_init", single_init_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Gm,
-                  max_level > 0 ? sep : nullptr, zmin, pp, ns, st, nst);
+                  max_level > 0 ? sep : nullptr, zmin, pp, ns, st, nst, dst, src);
     return check_hip(hipGetLastError(), "gauss_init launch");
+}
+
+int gauss_init_launch(int p, int max_level, uint8_t *Gm, int32_t *sep, float *zmin, unsigned long long *st, int nst,
+                      hipStream_t s) {
+    return init_launch(p, max_level, Gm, sep, zmin, st, nst, nullptr, Job{}, s);
 }
 
 int gauss_level0_launch(const float *C32, const double *C64, int p, double thr, float margin_k, uint8_t *Gm,
@@ -1525,14 +1680,16 @@ int gauss_level_launch(int level, const float *C32, const double *C64, int p, in
     return gauss_batch_level(level, &j, sw.desc, 1, margin_k, group_size, stats, sw.bws, sw.bws_bytes, s);
 }
 
-int gauss_skeleton_levels(const float *C32, const double *C64, int p, int max_level, const double *thr,
+// init (with the job descriptor), level 0 and levels 1..max_level of one
+// dataset; stats holds nst words
+int gauss_skeleton_launch(const float *C32, const double *C64, int p, int max_level, const double *thr,
                           float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
-                          unsigned long long *stats, void *ws, size_t ws_bytes, hipStream_t s) {
+                          unsigned long long *stats, int nst, void *ws, size_t ws_bytes, hipStream_t s) {
     if (!ws || ws_bytes < gauss_workspace_bytes(p)) return set_error(ERR_WORKSPACE, "gauss_skeleton: workspace too small");
     Job j = single_job(p, max_level, C32, C64, Gm, sep, zmin, thr, max_level + 1, ws);
-    int rc = single_write_desc(j, ws, s);
-    if (rc) return rc;
     SingleWs sw = carve_single(ws, p);
+    int rc = init_launch(p, max_level, Gm, sep, zmin, stats, nst, sw.desc, j, s);
+    if (rc) return rc;
     rc = gauss_batch_level0(&j, sw.desc, 1, margin_k, stats, s);
     if (rc) return rc;
     const int built = max_level < GPUCSL_MAX_LEVEL_TEMPLATE ? max_level : GPUCSL_MAX_LEVEL_TEMPLATE;
@@ -1563,6 +1720,13 @@ int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s)
     GPUCSL_LAUNCH("pmax", pmax_kernel, dim3((p + 255) / 256, p), dim3(256), 0, s, zmin, p, sqrt((double)(n - 3)),
                   pmax);
     return check_hip(hipGetLastError(), "pmax launch");
+}
+
+int outputs_launch(const float *zmin, const uint8_t *Gm, int p, int64_t n, float *pmax, uint16_t *out, hipStream_t s) {
+    if (!zmin || !Gm || !pmax || !out || p < 1 || n < 4) return set_error(ERR_ARG, "gauss_outputs: bad arguments");
+    GPUCSL_LAUNCH("outputs", outputs_kernel, dim3((p + 255) / 256, p), dim3(256), 0, s, zmin, Gm, p,
+                  sqrt((double)(n - 3)), pmax, out);
+    return check_hip(hipGetLastError(), "gauss_outputs launch");
 }
 
 int to_u16_launch(const uint8_t *Gm, int p, uint16_t *out, hipStream_t s) {

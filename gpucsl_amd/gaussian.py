@@ -161,8 +161,8 @@ class GaussianEngine:
 
     def finish(self) -> None:
         s = nat.stream_handle()
-        nat.call("gpucsl_pmax_from_zmin", self.zmin.data_ptr(), self.p, self.n, self.pmax.data_ptr(), s)
-        nat.call("gpucsl_skeleton_to_u16", self.G.data_ptr(), self.p, self.skel16.data_ptr(), s)
+        nat.call("gpucsl_gauss_outputs", self.zmin.data_ptr(), self.G.data_ptr(), self.p, self.n,
+                 self.pmax.data_ptr(), self.skel16.data_ptr(), s)
 
     # ----------------------------------------------------------------- output
     def level_stats(self):

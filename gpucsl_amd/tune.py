@@ -29,9 +29,11 @@ SPACE = {
     "wave_team_iters": (2, 4, 8),
     "fast_blocks_per_cu": (2, 4, 8),
     "fast_lds_bytes": (16384, 32768, 65536),
-    "item_tests": (1024, 4096, 16384),
+    "item_tests": (256, 1024, 4096, 16384),
+    "wave_items": (0, 1),
 }
-FIELDS = ("small_team_iters", "wave_team_iters", "fast_blocks_per_cu", "fast_lds_bytes", "item_tests")
+FIELDS = ("small_team_iters", "wave_team_iters", "fast_blocks_per_cu", "fast_lds_bytes", "item_tests",
+          "wave_items")
 
 
 class LevelTuning(ct.Structure):

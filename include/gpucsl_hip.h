@@ -178,14 +178,17 @@ int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss
  * group_size * small_team_iters, 64 while it is <= 64 * wave_team_iters, else
  * 256; fast_blocks_per_cu and fast_lds_bytes (4096..65536, a multiple of 1024)
  * size the grid and its LDS staging; a work item holds about item_tests
- * (64..2^24) subset tests.  Results do not depend on these values.  level 0
- * sets every level.  Defaults: {8, 8, 4, 32768, 4096}. */
+ * (64..2^24) subset tests; wave_items = 1 lets every wave take whole items on
+ * its own (64 lanes per pair, reads from L2, no LDS staging).  Results do not
+ * depend on these values.  level 0 sets every level.  Defaults:
+ * {8, 8, 4, 32768, 4096, 0}. */
 typedef struct gpucsl_level_tuning {
     int32_t small_team_iters;
     int32_t wave_team_iters;
     int32_t fast_blocks_per_cu;
     int32_t fast_lds_bytes;
     int32_t item_tests;
+    int32_t wave_items;
 } gpucsl_level_tuning;
 int gpucsl_set_level_tuning(int level, const gpucsl_level_tuning *tuning);
 int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning);
@@ -272,6 +275,11 @@ int gpucsl_compact(const uint8_t *G, int p, int32_t *Gc, void *ws, size_t ws_byt
 /* uint8 adjacency -> the uint16 skeleton the Python API returns
  * (gpucsl/pc/helpers.py:53-55). */
 int gpucsl_skeleton_to_u16(const uint8_t *G, int p, uint16_t *out, void *stream);
+
+/* gpucsl_pmax_from_zmin and gpucsl_skeleton_to_u16 of one dataset in one
+ * launch, with the same results (gpucsl/pc/helpers.py:53-82). */
+int gpucsl_gauss_outputs(const float *zmin, const uint8_t *G, int p, int64_t n, float *pmax, uint16_t *skeleton,
+                         void *stream);
 
 /* Lexicographic unranking used by the CI kernels, for tests: out[c*l + k] is
  * the k-th 1-based element of combination ranks[c] of l out of N

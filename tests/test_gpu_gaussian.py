@@ -268,3 +268,50 @@ def test_pcalg_mode_matches_oracle(seed):
     assert dp <= 2e-5
     plain = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), 2000, 0.01, 3)
     assert (np.abs(plain["pmax"][iu] - ref["pmax"][iu]).max() > 0) or len(one_way) > 0
+
+
+def test_level_by_level_and_batch_of_one_equal_whole_skeleton():
+    """The three ways of launching one dataset's levels give the same bits:
+    the whole-skeleton entry point (level planning inside the compaction
+    launch), level by level (gpucsl_gauss_init/level0/level), and a batch of
+    one job (separate planning launch)."""
+    from gpucsl_amd.batch import GaussianBatch, GaussJobState
+    from gpucsl_amd.gaussian import GaussianEngine
+    from gpucsl_amd.pc.helpers import get_gaussian_thresholds
+
+    d = syn.gaussian_sem(150, 4000, 3.0, seed=5).data.astype(np.float32)
+    X = torch.from_numpy(d).cuda()
+    thr = get_gaussian_thresholds(d, 0.01)
+    eng = GaussianEngine(150, 4000, 4, thr)
+    eng.set_data(X)
+    eng.run()
+    torch.cuda.synchronize()
+    want = eng.results()
+    for rep in range(2):
+        eng.run_levels()
+        torch.cuda.synchronize()
+        got = eng.results()
+        say(f"[levels rep {rep}] edges={int(got.skeleton.sum()) // 2}")
+        assert (got.skeleton == want.skeleton).all() and (got.separation_sets == want.separation_sets).all()
+        assert np.array_equal(got.pmax.view(np.uint32), want.pmax.view(np.uint32))
+    job = GaussJobState(4000, 150, 4, thresholds=thr)
+    job.set_data(X)
+    b = GaussianBatch([job])
+    b.run()
+    torch.cuda.synchronize()
+    r = b.results()[0]
+    assert (r.skeleton == want.skeleton).all() and (r.separation_sets == want.separation_sets).all()
+    assert np.array_equal(r.pmax.view(np.uint32), want.pmax.view(np.uint32))
+
+
+def test_fused_outputs_equal_separate_entry_points():
+    """gpucsl_gauss_outputs equals gpucsl_pmax_from_zmin + gpucsl_skeleton_to_u16."""
+    eng, res = _engine_run(syn.gaussian_sem(70, 1500, 3.0, seed=9).data, 1500, 0.05, 3)
+    p, s = 70, nat.stream_handle()
+    pm = torch.empty((p, p), dtype=torch.float32, device="cuda")
+    sk = torch.empty((p, p), dtype=torch.int16, device="cuda")
+    nat.call("gpucsl_pmax_from_zmin", eng.zmin.data_ptr(), p, eng.n, pm.data_ptr(), s)
+    nat.call("gpucsl_skeleton_to_u16", eng.G.data_ptr(), p, sk.data_ptr(), s)
+    torch.cuda.synchronize()
+    assert np.array_equal(pm.cpu().numpy().view(np.uint32), res.pmax.view(np.uint32))
+    assert np.array_equal(sk.cpu().numpy().view(np.uint16), res.skeleton)

@@ -79,13 +79,13 @@ def test_level_tuning_roundtrip_and_validation():
     before = {lv: get_level(lv) for lv in range(1, 9)}
     try:
         shape = dict(small_team_iters=2, wave_team_iters=4, fast_blocks_per_cu=8, fast_lds_bytes=16384,
-                     item_tests=2048)
+                     item_tests=2048, wave_items=1)
         set_level(3, shape)
         assert get_level(3) == shape
         set_level(0, dict(shape, fast_blocks_per_cu=2))
         assert all(get_level(lv)["fast_blocks_per_cu"] == 2 for lv in range(1, 9))
         for bad in (dict(shape, fast_lds_bytes=70000), dict(shape, small_team_iters=0),
-                    dict(shape, fast_lds_bytes=5000), dict(shape, item_tests=10)):
+                    dict(shape, fast_lds_bytes=5000), dict(shape, item_tests=10), dict(shape, wave_items=2)):
             with pytest.raises(GpucslError):
                 set_level(1, bad)
         with pytest.raises(GpucslError):
