@@ -29,6 +29,10 @@ WORKLOADS = {
     "gaussian_p1000_n10000": (1000, 10000, 4.0, 100, 0.01, 4, np.float32),
     "gaussian_p50_n2000": (50, 2000, 3.0, 0, 0.01, 3, np.float64),
 }
+DISCRETE_WORKLOADS = {
+    # name: (p, n, degree, seed, alpha, max_level)
+    "discrete_p200_n50000": (200, 50000, 2.0, 200, 0.01, 3),
+}
 BATCH_WORKLOADS = ("batched_bootstrap256_p200_n5000_plus64",)
 SERVING_WORKLOADS = ("serving_poisson_mix",)
 
@@ -100,6 +104,115 @@ def roofline(timings, profile_steps, stats, shapes, workload):
     roof["traffic"] = load_traffic(f"{dominant}@{workload}") or (
         load_traffic(dominant) if workload == "gaussian_p1000_n10000" else None)
     return roof
+
+
+def discrete_bytes(kernel: str, stats, n: int, launches_per_step: float):
+    """Algorithmic bytes per launch of a discrete CI-test kernel: the uint8
+    columns one test reads, 2 n B per level-0 pair and (l + 2) n B per level-l
+    test (gpucsl/pc/cuda/discrete_ci.cu:37-97, helpers/discrete_helpers.cu:74-80)."""
+    lv = 0 if kernel == "disc_level0" else int(kernel[len("disc_level"):])
+    return stats[lv]["executed"] * (lv + 2) * n / launches_per_step
+
+
+def bench_discrete(args):
+    """Discrete (Pearson chi^2) skeleton of one dataset: p=200, n=50000 uint8
+    codes with 2..5 categories, ER-DAG Bayesian network of degree 2 (seed 200),
+    alpha=0.01, levels 0..3.  One step = every level + pmax, captured as one
+    HIP graph, codes resident in HBM."""
+    import torch
+
+    from gpucsl_amd import _native as nat
+    from gpucsl_amd import synthetic as syn
+    from gpucsl_amd.discrete import DiscreteEngine
+
+    p, n, degree, seed, alpha, L = DISCRETE_WORKLOADS[args.workload]
+    log(f"[bench] workload {args.workload}: generating data")
+    codes = syn.discrete_bn(p, n, degree, seed=seed).data
+    dims = (codes.max(axis=0).astype(np.int64) + 1).astype(np.uint8)
+    torch.cuda.set_device(0)
+    eng = DiscreteEngine(p, n, L, alpha, int(dims.max()))
+    eng.set_codes(codes, dims)
+    eng.run()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    cs = torch.cuda.Stream()
+    cs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cs):
+        g.capture_begin()
+        eng.run()
+        g.capture_end()
+    torch.cuda.current_stream().wait_stream(cs)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.replay()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / max(args.steps, 1)
+    stats = eng.level_stats()
+    executed = sum(st["executed"] for st in stats)
+
+    log(f"[bench] profiled pass {args.profile_steps} steps")
+    with nat.KernelTimer() as kt:
+        for _ in range(args.profile_steps):
+            eng.run()
+        torch.cuda.synchronize()
+    timings = kt.timings
+    tests_k = {k: v for k, v in timings.items() if k.startswith("disc_level")}
+    dominant = max(tests_k, key=lambda k: tests_k[k][1])
+    cnt, tot, _, _ = tests_k[dominant]
+    avg_s = tot / cnt / 1e3
+    achieved = discrete_bytes(dominant, stats, n, cnt / args.profile_steps) / avg_s / 1e9
+    roof = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
+                kernel=dominant, avg_launch_ms=avg_s * 1e3,
+                traffic=load_traffic(f"{dominant}@{args.workload}"))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+
+        log("[bench] cpu baseline (oracle)")
+        oracle.lib()
+        colmajor = np.ascontiguousarray(codes.T)
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < 5.0:
+            ref = oracle.disc_skeleton(colmajor, dims, alpha, L)
+            reps += 1
+        cs_ = (time.perf_counter() - t0) / reps
+        tests = int(ref["executed"].sum())
+        cpu = dict(value=tests / cs_, unit="CI tests/s", cores=oracle.num_threads(), kind="port",
+                   sample=f"full workload, {reps} runs of {cs_:.3f}s, {tests} tests each (C oracle, OpenMP)")
+
+    out = {
+        "metric": METRIC,
+        "value": executed / sec,
+        "unit": "CI tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: ER-DAG discrete BN degree {degree}, 2..5 categories, "
+                               f"seed {seed}, alpha={alpha}, max_level={L}",
+                   "p": p, "n": n, "max_level": L, "alpha": alpha,
+                   "ci_tests": "Pearson chi^2 on exact integer contingency tables (full enumeration)",
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "tests_per_step": {"executed": executed},
+        "gpu_vs_cpu": (executed / sec) / cpu["value"] if cpu else None,
+        "per_level": stats,
+        "kernel_ms_per_step": {k: v[1] / args.profile_steps for k, v in timings.items()},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def batched_datasets(n_boot: int, n_sets: int):
@@ -411,7 +524,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gaussian_p1000_n10000",
-                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVING_WORKLOADS))
+                    choices=sorted(WORKLOADS) + sorted(DISCRETE_WORKLOADS) + list(BATCH_WORKLOADS)
+                    + list(SERVING_WORKLOADS))
     ap.add_argument("--requests", type=int, default=96, help="serving workload: requests in the stream")
     ap.add_argument("--load", type=float, default=0.8, help="serving workload: arrival rate / capacity")
     ap.add_argument("--bootstraps", type=int, default=256, help="batched workload: bootstrap resamples")
@@ -426,6 +540,8 @@ def main():
               "by design but this build runs on a single MI355X.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload in DISCRETE_WORKLOADS:
+        return bench_discrete(args)
     if args.workload in BATCH_WORKLOADS:
         return bench_batched(args)
     if args.workload in SERVING_WORKLOADS:

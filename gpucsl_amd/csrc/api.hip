@@ -106,7 +106,7 @@ int gauss_batch_finish(const Job *jh, const Job *jd, int nj, hipStream_t s);
 size_t disc_workspace_bytes(int p, int max_level, int max_dim);
 size_t disc_min_workspace_bytes(int p, int max_level, int max_dim);
 int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *sep, float *pmax, void *ws,
-                     hipStream_t s);
+                     unsigned long long *st, int nst, hipStream_t s);
 int disc_level0_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
                        int max_level, int max_dim, double alpha, uint8_t *Gm, float *pmax,
                        unsigned long long *st, void *ws, size_t ws_bytes, hipStream_t s);
@@ -369,10 +369,9 @@ int gpucsl_disc_init(int p, int max_level, int max_dim, uint8_t *G, int32_t *sep
                      uint64_t *stats, void *ws, size_t ws_bytes, void *stream) {
     if (ws_bytes < disc_min_workspace_bytes(p, max_level, max_dim))
         return set_error(ERR_WORKSPACE, "disc_init: workspace too small");
-    int rc = disc_init_launch(p, max_level, max_dim, G, sepsets, pmax, ws, S(stream));
-    if (rc) return rc;
-    if (stats) launch_fill<uint64_t>(stats, 0ull, (size_t)(max_level + 1) * GPUCSL_STAT_SLOTS, S(stream));
-    return check_hip(hipGetLastError(), "disc_init stats");
+    return disc_init_launch(p, max_level, max_dim, G, sepsets, pmax, ws,
+                            reinterpret_cast<unsigned long long *>(stats),
+                            stats ? (max_level + 1) * GPUCSL_STAT_SLOTS : 0, S(stream));
 }
 
 int gpucsl_disc_level0(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p, int max_level,

@@ -47,6 +47,8 @@ using IArr = int[L > 0 ? L : 1];
 struct DiscWs {
     unsigned long long *testoff;  // p + 1 prefix of tests per row
     unsigned long long *keys;     // p * p removal keys, upper triangle used
+    int32_t *gram;                // level-0 Gram matrix of the one-hot codes (gram_w^2 words)
+    int gram_w;                   // its padded width; 0: level 0 counts pair by pair
     uint32_t *gtab;               // global tables for tests too large for LDS
     size_t gtab_words;            // words per workgroup slice
     int grid;                     // workgroups of the test kernels
@@ -211,6 +213,198 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level0_kernel(
         }
         __syncthreads();
     }
+}
+
+// ----------------------------------------------------------------------------
+// level 0 on int8 MFMA: the contingency tables of every pair are blocks of one
+// Gram matrix.  With the one-hot encoding H (n x W, W = p D padded to 64,
+// column v D + a = [x_v = a]), N = H^T H holds N[v D + a][u D + b] =
+// #{k : x_v[k] = a, x_u[k] = b}: the level-0 table of (v, u) for v != u.
+// ----------------------------------------------------------------------------
+constexpr int GRAM_TILE = 64;
+constexpr int GRAM_MAX_DIM = 16;
+constexpr int GRAM_MAX_W = 4096;
+
+// padded one-hot width, or 0 when level 0 counts its tables pair by pair
+// (categories above GRAM_MAX_DIM or a Gram matrix above GRAM_MAX_W^2 words)
+static int gram_width(int p, int max_dim) {
+    if (max_dim < 1 || max_dim > GRAM_MAX_DIM) return 0;
+    const size_t w = align_up((size_t)p * max_dim, GRAM_TILE);
+    return w <= (size_t)GRAM_MAX_W ? (int)w : 0;
+}
+
+typedef int gram_v4i __attribute__((ext_vector_type(4)));
+
+// 8 observations of one variable as one-hot bytes for category `a`: byte q is
+// 1 iff x[q] == a (exact per-byte zero test, no carries between bytes)
+__device__ __forceinline__ uint64_t onehot8(uint64_t x, unsigned a) {
+    const uint64_t t = x ^ (0x0101010101010101ull * a);
+    const uint64_t lo7 = 0x7f7f7f7f7f7f7f7full;
+    const uint64_t z = ~(((t & lo7) + lo7) | t | lo7);
+    return z >> 7;
+}
+
+// N (W x W int32, zeroed before) += the upper 64 x 64 tiles (tile row <= tile
+// column) of H^T H over this block's share of the observations.  256 threads
+// = 4 waves in a 2 x 2 arrangement of 32 x 32 sub-tiles, each 2 x 2
+// v_mfma_i32_16x16x32_i8 blocks.  Operand layout (checked on gfx950): lane l
+// gives row/column l % 16 of its block and observations 8 (l / 16) .. +7 of the
+// 32-observation step; accumulator item v holds row 4 (l / 16) + v, column
+// l % 16.  Split-K shares add with int32 atomics, exact in any order.
+__global__ __launch_bounds__(256) void disc_onehot_gram_kernel(const uint8_t *__restrict__ data, int64_t n,
+                                                               int64_t stride, int p, int D, int W,
+                                                               int32_t *__restrict__ N) {
+    const int tiles = W / GRAM_TILE;
+    int t = blockIdx.x, tr = 0;
+    while (t >= tiles - tr) {
+        t -= tiles - tr;
+        ++tr;
+    }
+    const int tc = tr + t;
+    const int64_t steps = (n + 31) / 32;
+    const int64_t per = (steps + gridDim.y - 1) / gridDim.y;
+    const int64_t s0 = (int64_t)blockIdx.y * per;
+    const int64_t s1 = s0 + per < steps ? s0 + per : steps;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int ci = lane & 15, kq = lane >> 4;
+    const uint8_t *pa[2], *pb[2];
+    unsigned ca[2], cb[2];
+    bool va[2], vb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = tr * GRAM_TILE + wr + i * 16 + ci;
+        const int c = tc * GRAM_TILE + wc + i * 16 + ci;
+        va[i] = r / D < p;
+        vb[i] = c / D < p;
+        pa[i] = data + (size_t)(va[i] ? r / D : 0) * stride;
+        pb[i] = data + (size_t)(vb[i] ? c / D : 0) * stride;
+        ca[i] = (unsigned)(r % D);
+        cb[i] = (unsigned)(c % D);
+    }
+    gram_v4i acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = gram_v4i{0, 0, 0, 0};
+    for (int64_t st = s0; st < s1; ++st) {
+        const int64_t k = st * 32 + 8 * kq;
+        // observations k .. k+7; bytes at or past n are masked (reads stay
+        // below n rounded up to 8 <= stride)
+        const int64_t rem = n - k;
+        const uint64_t mask = rem >= 8 ? ~0ull : (rem <= 0 ? 0ull : ((1ull << (8 * rem)) - 1ull));
+        uint64_t xa[2] = {0, 0}, xb[2] = {0, 0};
+        if (rem > 0) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                if (va[i]) xa[i] = *reinterpret_cast<const uint64_t *>(pa[i] + k);
+                if (vb[i]) xb[i] = *reinterpret_cast<const uint64_t *>(pb[i] + k);
+            }
+        }
+        long oa[2], ob[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            oa[i] = va[i] ? (long)(onehot8(xa[i], ca[i]) & mask) : 0;
+            ob[i] = vb[i] ? (long)(onehot8(xb[i], cb[i]) & mask) : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x32_i8(oa[i], ob[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = tr * GRAM_TILE + wr + i * 16 + 4 * kq + v;
+                const int c = tc * GRAM_TILE + wc + j * 16 + ci;
+                if (acc[i][j][v] != 0) atomicAdd(&N[(size_t)r * W + c], acc[i][j][v]);
+            }
+}
+
+// One thread per pair (vj < vi): the table N[a][b] (a a value of vi, b of vj)
+// read from the Gram matrix, integer marginals, chi^2 summed over the cells in
+// (a, b) order with the per-cell term of chi2_term, Perlman's pchisq, then pmax
+// and G as disc_level0_kernel writes them.  MD >= max_dim bounds the
+// marginal arrays.
+template <int MD>
+__global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__restrict__ dims, int p, int D, int W,
+                                                               const int32_t *__restrict__ N, double alpha,
+                                                               uint8_t *__restrict__ Gm, float *__restrict__ pmax,
+                                                               unsigned long long *__restrict__ st) {
+#pragma clang fp contract(off)
+    const long long pairs = (long long)p * (p - 1) / 2;
+    if (blockIdx.x == 0) {
+        for (int v = threadIdx.x; v < p; v += blockDim.x) Gm[(size_t)v * p + v] = 0;
+        if (threadIdx.x == 0) {
+            st[DS_EXEC] = pairs;
+            st[DS_TOTAL] = pairs;
+            st[DS_USEFUL] = pairs;
+        }
+    }
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pairs) return;
+    int vi = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
+    while ((long long)vi * (vi - 1) / 2 > t) --vi;
+    while ((long long)(vi + 1) * vi / 2 <= t) ++vi;
+    const int vj = (int)(t - (long long)vi * (vi - 1) / 2);
+    const int di = dims[vi], dj = dims[vj];
+    // N[a][b] = Gram[vj D + b][vi D + a] (upper triangle: vj D + b < vi D)
+    const int32_t *T = N + (size_t)vj * D * W + (size_t)vi * D;
+    uint32_t Ni[MD], Nj[MD], Ns = 0;
+#pragma unroll
+    for (int a = 0; a < MD; ++a) Ni[a] = Nj[a] = 0;
+#pragma unroll
+    for (int a = 0; a < MD; ++a)
+#pragma unroll
+        for (int b = 0; b < MD; ++b)
+            if (a < di && b < dj) {
+                const uint32_t v = (uint32_t)T[(size_t)b * W + a];
+                Ni[a] += v;
+                Nj[b] += v;
+                Ns += v;
+            }
+    double chi2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < MD; ++a)
+#pragma unroll
+        for (int b = 0; b < MD; ++b)
+            if (a < di && b < dj) {
+                const uint32_t o = (uint32_t)T[(size_t)b * W + a];
+                const double e = Ni[a] * (double)Nj[b] / Ns;
+                double term = 0.0;
+                if (!(Ns == 0 || e == 0.0)) {
+                    const double d = o - e;
+                    term = d * d / e;
+                }
+                chi2 += term;
+            }
+    const double pv = pchisq(chi2, (di - 1) * (dj - 1));
+    pmax[(size_t)vj * p + vi] = (float)pv;
+    const uint8_t keep = (pv >= alpha) ? 0 : 1;
+    Gm[(size_t)vi * p + vj] = keep;
+    Gm[(size_t)vj * p + vi] = keep;
+}
+
+// Buffer set-up of a discrete skeleton in one launch: G = 1, sepsets = -1,
+// pmax = 0, removal keys = none, the level-0 Gram matrix and the statistics 0.
+__global__ void disc_init_kernel(uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, float *__restrict__ pmax,
+                                 unsigned long long *__restrict__ keys, int32_t *__restrict__ gram,
+                                 unsigned long long *__restrict__ st, size_t pp, size_t ns, size_t ng, int nst) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t t = t0; t < pp; t += stride) {
+        Gm[t] = 1;
+        pmax[t] = 0.0f;
+        keys[t] = KEY_NONE;
+    }
+    if (sep)
+        for (size_t t = t0; t < ns; t += stride) sep[t] = -1;
+    if (gram)
+        for (size_t t = t0; t < ng; t += stride) gram[t] = 0;
+    if (st && t0 < (size_t)nst) st[t0] = 0;
 }
 
 // ----------------------------------------------------------------------------
@@ -431,22 +625,25 @@ static size_t gtab_words_for(int max_dim, int max_level) {
 // Workspace = a fixed part (level lists, test offsets, removal keys) plus one
 // global table slice per concurrently running block when the largest table
 // exceeds the LDS budget.  A smaller workspace runs fewer blocks at once.
-static size_t disc_fixed_bytes(int p) {
+static size_t disc_fixed_bytes(int p, int max_dim) {
     size_t off = align_up(gauss_workspace_bytes(p), 256);
     off = align_up(off + sizeof(unsigned long long) * (size_t)(p + 1), 256);
     off = align_up(off + sizeof(unsigned long long) * (size_t)p * p, 256);
+    const size_t W = (size_t)gram_width(p, max_dim);
+    off = align_up(off + sizeof(int32_t) * W * W, 256);
     return off;
 }
 
 size_t disc_workspace_bytes(int p, int max_level, int max_dim) {
     if (max_table_words(max_dim, max_level) >= TABLE_WORDS_LIMIT) return SIZE_MAX;
-    return align_up(disc_fixed_bytes(p) + sizeof(uint32_t) * gtab_words_for(max_dim, max_level) * (size_t)disc_grid(),
+    return align_up(disc_fixed_bytes(p, max_dim) +
+                        sizeof(uint32_t) * gtab_words_for(max_dim, max_level) * (size_t)disc_grid(),
                     256);
 }
 
 size_t disc_min_workspace_bytes(int p, int max_level, int max_dim) {
     if (max_table_words(max_dim, max_level) >= TABLE_WORDS_LIMIT) return SIZE_MAX;
-    return align_up(disc_fixed_bytes(p) + sizeof(uint32_t) * gtab_words_for(max_dim, max_level), 256);
+    return align_up(disc_fixed_bytes(p, max_dim) + sizeof(uint32_t) * gtab_words_for(max_dim, max_level), 256);
 }
 
 // blocks that the workspace has table slices for (all of the grid when the
@@ -454,7 +651,7 @@ size_t disc_min_workspace_bytes(int p, int max_level, int max_dim) {
 static int disc_slices(size_t ws_bytes, int p, int max_level, int max_dim) {
     const size_t words = gtab_words_for(max_dim, max_level);
     if (words == 0) return disc_grid();
-    const size_t fixed = disc_fixed_bytes(p);
+    const size_t fixed = disc_fixed_bytes(p, max_dim);
     const size_t n = ws_bytes > fixed ? (ws_bytes - fixed) / (sizeof(uint32_t) * words) : 0;
     return n < (size_t)disc_grid() ? (int)n : disc_grid();
 }
@@ -467,6 +664,9 @@ static DiscWs disc_carve(void *ws, int p, int max_level, int max_dim, size_t ws_
     off = align_up(off + sizeof(unsigned long long) * (size_t)(p + 1), 256);
     w.keys = (unsigned long long *)(b + off);
     off = align_up(off + sizeof(unsigned long long) * (size_t)p * p, 256);
+    w.gram = (int32_t *)(b + off);
+    w.gram_w = gram_width(p, max_dim);
+    off = align_up(off + sizeof(int32_t) * (size_t)w.gram_w * w.gram_w, 256);
     w.gtab = (uint32_t *)(b + off);
     w.gtab_words = gtab_words_for(max_dim, max_level);
     w.grid = disc_slices(ws_bytes, p, max_level, max_dim);
@@ -474,13 +674,16 @@ static DiscWs disc_carve(void *ws, int p, int max_level, int max_dim, size_t ws_
 }
 
 int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *sep, float *pmax, void *ws,
-                     hipStream_t s) {
+                     unsigned long long *st, int nst, hipStream_t s) {
     if (p < 1 || max_level < 0 || !Gm || !pmax || !ws) return set_error(ERR_ARG, "disc_init: bad arguments");
-    launch_fill<uint8_t>(Gm, (uint8_t)1, (size_t)p * p, s);
-    if (max_level > 0 && sep) launch_fill<int32_t>(sep, -1, (size_t)p * p * max_level, s);
-    launch_fill<float>(pmax, 0.0f, (size_t)p * p, s);
     DiscWs w = disc_carve(ws, p, max_level, max_dim, disc_min_workspace_bytes(p, max_level, max_dim));
-    launch_fill<unsigned long long>(w.keys, KEY_NONE, (size_t)p * p, s);
+    const size_t pp = (size_t)p * p, ns = max_level > 0 && sep ? pp * max_level : 0;
+    const size_t ng = (size_t)w.gram_w * w.gram_w;
+    size_t blocks = ((ns > pp ? (ns > ng ? ns : ng) : (pp > ng ? pp : ng)) + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks * 256 < (size_t)nst) blocks = ((size_t)nst + 255) / 256;
+    GPUCSL_LAUNCH("disc_init", disc_init_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Gm,
+                  max_level > 0 ? sep : nullptr, pmax, w.keys, ng ? w.gram : nullptr, st, pp, ns, ng, nst);
     return check_hip(hipGetLastError(), "disc_init launch");
 }
 
@@ -497,6 +700,29 @@ int disc_level0_launch(const uint8_t *data, int64_t n, int64_t stride, const uin
     if (max_table_words(max_dim, 0) > (size_t)DT_LDS_WORDS && gtab_words_for(max_dim, max_level) < max_table_words(max_dim, 0))
         return set_error(ERR_WORKSPACE, "disc_level0: table workspace too small");
     DiscWs w = disc_carve(ws, p, max_level, max_dim, ws_bytes);
+    if (w.gram_w) {
+        // the tables of every pair from one int8 Gram matrix (zeroed by disc_init)
+        const int W = w.gram_w, tiles = W / GRAM_TILE, ut = tiles * (tiles + 1) / 2;
+        const int64_t steps = (n + 31) / 32;
+        int64_t ks = (4 * (int64_t)device_cus() + ut - 1) / ut;
+        const int64_t kmax = (steps + 63) / 64;  // at least 64 steps (2048 observations) per share
+        if (ks > kmax) ks = kmax;
+        if (ks < 1) ks = 1;
+        GPUCSL_LAUNCH("disc_level0_gram", disc_onehot_gram_kernel, dim3(ut, (unsigned)ks), dim3(256), 0, s, data, n,
+                      stride, p, max_dim, W, w.gram);
+        const long long pairs = (long long)p * (p - 1) / 2;
+        const unsigned blocks = (unsigned)((pairs + 255) / 256 > 0 ? (pairs + 255) / 256 : 1);
+        if (max_dim <= 4)
+            GPUCSL_LAUNCH("disc_level0", disc_level0_chi2_kernel<4>, dim3(blocks), dim3(256), 0, s, dims, p, max_dim,
+                          W, w.gram, alpha, Gm, pmax, st);
+        else if (max_dim <= 8)
+            GPUCSL_LAUNCH("disc_level0", disc_level0_chi2_kernel<8>, dim3(blocks), dim3(256), 0, s, dims, p, max_dim,
+                          W, w.gram, alpha, Gm, pmax, st);
+        else
+            GPUCSL_LAUNCH("disc_level0", disc_level0_chi2_kernel<16>, dim3(blocks), dim3(256), 0, s, dims, p,
+                          max_dim, W, w.gram, alpha, Gm, pmax, st);
+        return check_hip(hipGetLastError(), "disc_level0 launch");
+    }
     GPUCSL_LAUNCH("disc_level0", disc_level0_kernel, dim3(w.grid), dim3(DT_THREADS), DT_LDS_BYTES, s, data, n,
                   stride, dims, p, alpha, Gm, pmax, w.gtab, w.gtab_words, st);
     return check_hip(hipGetLastError(), "disc_level0 launch");

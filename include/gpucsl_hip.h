@@ -226,13 +226,18 @@ size_t gpucsl_disc_workspace_bytes(int p, int max_level, int max_dim);
  * parallelism under a memory restriction. */
 size_t gpucsl_disc_min_workspace_bytes(int p, int max_level, int max_dim);
 
-/* G = 1, sepsets = -1, pmax = 0, stats = 0, removal keys cleared.  Replaces the
- * buffer set-up of gpucsl/pc/discover_skeleton_discrete.py:41-55. */
+/* G = 1, sepsets = -1, pmax = 0, stats = 0, removal keys cleared and the
+ * level-0 table matrix zeroed, in one launch.  Replaces the buffer set-up of
+ * gpucsl/pc/discover_skeleton_discrete.py:41-55. */
 int gpucsl_disc_init(int p, int max_level, int max_dim, uint8_t *G, int32_t *sepsets, float *pmax,
                      uint64_t *stats, void *ws, size_t ws_bytes, void *stream);
 
 /* Level 0: chi^2 test of every pair, removal on p >= alpha, pmax.  Replaces
- * discrete_ci_level_0 (gpucsl/pc/cuda/discrete_ci.cu:37-97). */
+ * discrete_ci_level_0 (gpucsl/pc/cuda/discrete_ci.cu:37-97).  Runs after
+ * gpucsl_disc_init on the same workspace.  With max_dim <= 16 and
+ * p * max_dim <= 4096 every pair's table comes from one int8-MFMA Gram matrix
+ * of the one-hot codes; otherwise tables are counted pair by pair.  Results
+ * are the same either way. */
 int gpucsl_disc_level0(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
                        int max_level, int max_dim, double alpha, uint8_t *G, float *pmax,
                        uint64_t *stats_level0, void *ws, size_t ws_bytes, void *stream);

@@ -120,6 +120,20 @@ def test_c3_parity():
     _compare(res, ref, "C3")
 
 
+@pytest.mark.parametrize("p,card,n,label", [
+    (14, (14, 16), 2500, "one-hot Gram matrix, 16 categories"),
+    (12, (17, 20), 2500, "pair-by-pair tables, more than 16 categories"),
+    (300, (14, 14), 1500, "pair-by-pair tables, p * max_dim > 4096"),
+])
+def test_level0_table_paths(p, card, n, label):
+    """Level 0 takes its tables from the int8-MFMA Gram matrix of the one-hot
+    codes or counts them pair by pair, by category count and width; both
+    match the oracle."""
+    d = syn.discrete_bn(p, n, 2.0, seed=70 + p, card_low=card[0], card_high=card[1])
+    res, ref = _run(d.data, 0.05, 1)
+    _compare(res, ref, label)
+
+
 def test_constant_column_and_unaligned_n():
     d = syn.discrete_bn(10, 1237, 2.0, seed=5)
     codes = np.concatenate([d.data, np.zeros((1237, 1), np.uint8)], axis=1)
