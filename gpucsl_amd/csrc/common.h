@@ -388,6 +388,59 @@ __device__ __forceinline__ double pchisq(double x, int df) {
     return c * y + s;
 }
 
+// pchisq by the 64 lanes of one wave (uniform x, df; every lane returns the
+// value).  The same operations in the same order as pchisq, so the same bits:
+// the lanes evaluate 64 iterations' independent log(z), a / z and exp terms
+// at once, and the running values (e = log(z) + e, s += term, e = e * (a / z),
+// c = c + e) are updated in iteration order by every lane from broadcasts.
+// Long loops (df in the thousands at levels >= 2) then cost about as many
+// dependent adds as the serial loop has iterations instead of as many
+// dependent log/exp evaluations.
+__device__ double pchisq_wave(double x, int df) {
+#pragma clang fp contract(off)
+    if (x <= 0.0 || df < 1) return 1.0;
+    const int lane = __lane_id();
+    const double a = 0.5 * x;
+    const bool even = (2 * (df / 2)) == df;
+    double y = 0.0;
+    if (df > 1) y = ex20(-a);
+    double s = even ? y : (2.0 * poz(-sqrt(x)));
+    if (df <= 2) return s;
+    const double half_dfm1 = 0.5 * (df - 1.0);
+    const double z0 = even ? 1.0 : 0.5;
+    // iterations z = z0, z0 + 1, ... <= half_dfm1 (all exact in FP64)
+    const int m = half_dfm1 >= z0 ? (int)(half_dfm1 - z0) + 1 : 0;
+    if (a > 20.0) {
+        double e = even ? 0.0 : 0.5723649429247000870717135;
+        const double c = log(a);
+        for (int k0 = 0; k0 < m; k0 += 64) {
+            const int cnt = m - k0 < 64 ? m - k0 : 64;
+            const double z = z0 + (double)(k0 + lane);
+            const double lz = lane < cnt ? log(z) : 0.0;
+            double ek = 0.0;
+            for (int q = 0; q < cnt; ++q) {
+                e = __shfl(lz, q) + e;
+                if (lane == q) ek = e;
+            }
+            const double t = lane < cnt ? ex20(c * z - a - ek) : 0.0;
+            for (int q = 0; q < cnt; ++q) s += __shfl(t, q);
+        }
+        return s;
+    }
+    double e = even ? 1.0 : (0.5641895835477562869480795 / sqrt(a));
+    double c = 0.0;
+    for (int k0 = 0; k0 < m; k0 += 64) {
+        const int cnt = m - k0 < 64 ? m - k0 : 64;
+        const double z = z0 + (double)(k0 + lane);
+        const double r = lane < cnt ? a / z : 0.0;
+        for (int q = 0; q < cnt; ++q) {
+            e = e * __shfl(r, q);
+            c = c + e;
+        }
+    }
+    return c * y + s;
+}
+
 // ----------------------------------------------------------------------------
 // small fills usable inside graph capture (no hipMemsetAsync)
 // ----------------------------------------------------------------------------

@@ -506,12 +506,14 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level_kernel(
         count_table<L>(data, n, stride, i, j, S, di, dj, radix, tab, cells, replicas);
         uint32_t *Nis = tab + (size_t)replicas * cells, *Njs = Nis + dim_s * di, *Ns = Njs + dim_s * dj;
         marginals(tab, dim_s, di, dj, Nis, Njs, Ns);
-        const double chi2 = chi2_sum(tab, Nis, Njs, Ns, dim_s, di, dj, false);
-        if (threadIdx.x == 0) {
-            const float pf = (float)pchisq(chi2, (di - 1) * (dj - 1) * dim_s);
+        double chi2 = chi2_sum(tab, Nis, Njs, Ns, dim_s, di, dj, false);
+        if (threadIdx.x < 64) {
+            // the sum sits in lane 0; wave 0 evaluates the p-value together
+            chi2 = __shfl(chi2, 0);
+            const float pf = (float)pchisq_wave(chi2, (di - 1) * (dj - 1) * dim_s);
             const int a = i < j ? i : j, b = i < j ? j : i;
-            atomic_max_nonneg(&pmax[(size_t)a * p + b], pf);
-            if ((double)pf >= alpha) {
+            if (threadIdx.x == 0) atomic_max_nonneg(&pmax[(size_t)a * p + b], pf);
+            if (threadIdx.x == 0 && (double)pf >= alpha) {
                 const unsigned long long key = ((unsigned long long)(i < j ? 0 : 1) << 62) | rank;
                 atomicMin(&keys[(size_t)a * p + b], key);
             }
@@ -571,15 +573,28 @@ __global__ __launch_bounds__(DT_THREADS) void disc_single_test_kernel(
     }
     const int di = dims[vi], dj = dims[vj];
     const int cells = dim_s * di * dj;
+    if (cells + dim_s * (di + dj + 1) > DT_LDS_WORDS) {
+        // table and marginals must fit the workgroup's LDS: report, do not count
+        if (threadIdx.x == 0) {
+            out[0] = __builtin_nan("");
+            out[1] = __builtin_nan("");
+            out[2] = -1.0;
+        }
+        return;
+    }
     count_table<L>(data, n, stride, vi, vj, S, di, dj, radix, lds, cells, 1);
     uint32_t *Nis = lds + cells, *Njs = Nis + dim_s * di, *Ns = Njs + dim_s * dj;
     marginals(lds, dim_s, di, dj, Nis, Njs, Ns);
-    const double chi2 = chi2_sum(lds, Nis, Njs, Ns, dim_s, di, dj, sequential != 0);
+    double chi2 = chi2_sum(lds, Nis, Njs, Ns, dim_s, di, dj, sequential != 0);
     for (int c = threadIdx.x; c < cells; c += DT_THREADS) table_out[c] = lds[c];
-    if (threadIdx.x == 0) {
-        out[0] = chi2;
-        out[1] = pchisq(chi2, (di - 1) * (dj - 1) * dim_s);
-        out[2] = (double)((di - 1) * (dj - 1) * dim_s);
+    if (threadIdx.x < 64) {
+        chi2 = __shfl(chi2, 0);
+        const double pv = pchisq_wave(chi2, (di - 1) * (dj - 1) * dim_s);
+        if (threadIdx.x == 0) {
+            out[0] = chi2;
+            out[1] = pv;
+            out[2] = (double)((di - 1) * (dj - 1) * dim_s);
+        }
     }
 }
 

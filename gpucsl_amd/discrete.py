@@ -183,4 +183,6 @@ class DiscreteEngine:
         nat.call("gpucsl_disc_ci_test", self.data.data_ptr(), self.n, self.stride, self.dims.data_ptr(), vi, vj,
                  dS.data_ptr(), l, 1 if l == 0 else 0, table.data_ptr(), out.data_ptr(), nat.stream_handle())
         o = out.cpu().numpy()
+        if o[2] < 0:
+            raise GpucslError(f"ci_test: the table of {cells} cells and its marginals exceed one workgroup's LDS")
         return table.cpu().numpy().view(np.uint32)[:cells].copy(), float(o[0]), float(o[1]), int(o[2])

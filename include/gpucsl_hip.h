@@ -263,7 +263,9 @@ int gpucsl_disc_finish(const uint8_t *G, int p, float *pmax, uint16_t *skeleton,
 
 /* One CI test, for known-answer checks: table_out gets the dims_s*di*dj counts
  * N[g][a][b], out[0..2] = chi^2, p value, degrees of freedom.  sepset is a
- * device array of l ids; sequential selects the level-0 summation order.
+ * device array of l ids; sequential selects the level-0 summation order.  The
+ * table and its marginals (dim_s*(di*dj + di + dj + 1) words) must fit 8192
+ * words; a larger test is not counted and gets out = {NaN, NaN, -1}.
  * (gpucsl/pc/cuda/helpers/discrete_helpers.cu:25-186, math_helpers.cu:128-175) */
 int gpucsl_disc_ci_test(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int vi, int vj,
                         const int32_t *sepset, int l, int sequential, uint32_t *table_out, double *out,

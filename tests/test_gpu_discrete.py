@@ -9,6 +9,7 @@ import pytest
 import oracle
 from gpucsl_amd import _native as nat
 from gpucsl_amd import synthetic as syn
+from gpucsl_amd.cli.cli_util import GpucslError
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +80,32 @@ def test_single_tests_match_oracle(l):
         assert df == odf
         assert chi2 == ochi2 or abs(chi2 - ochi2) <= 1e-12 * abs(ochi2)
         assert p == pytest.approx(oracle.pchisq(ochi2, odf), rel=1e-13, abs=1e-300)
+
+
+def test_large_df_pvalues_match_oracle():
+    """Level-3 tests with 5 categories: df = 4 * 4 * 125 = 2000, so the
+    p-value series runs a thousand iterations (evaluated by a whole wave)."""
+    d = syn.discrete_bn(10, 4000, 2.0, seed=91, card_low=5, card_high=5)
+    eng, dims = _engine(d.data)
+    colmajor = np.ascontiguousarray(d.data.T)
+    rng = np.random.default_rng(91)
+    dfs = []
+    for _ in range(12):
+        vs = rng.choice(10, size=5, replace=False)
+        vi, vj, S = int(vs[0]), int(vs[1]), [int(v) for v in sorted(vs[2:])]
+        table, chi2, p, df = eng.ci_test(vi, vj, S)
+        ochi2, odf, work = oracle.disc_chi2(colmajor, dims, vi, vj, S)
+        assert table.tolist() == work[: table.size].tolist() and df == odf
+        assert chi2 == ochi2 or abs(chi2 - ochi2) <= 1e-12 * abs(ochi2)
+        assert p == pytest.approx(oracle.pchisq(ochi2, odf), rel=1e-12, abs=1e-300)
+        dfs.append(df)
+    say(f"[large df] df range {min(dfs)}..{max(dfs)}")
+    assert max(dfs) >= 2000
+    # 6 categories: 7776 cells + marginals exceed the LDS of the single-test entry point
+    d6 = syn.discrete_bn(6, 2000, 2.0, seed=92, card_low=6, card_high=6)
+    eng6, _ = _engine(d6.data)
+    with pytest.raises(GpucslError):
+        eng6.ci_test(0, 1, [2, 3, 4])
 
 
 def _run(codes, alpha, L):
