@@ -797,6 +797,20 @@ __device__ __forceinline__ void subset_of(const ItemCtx &c, uint64_t rank, int j
 }
 
 // the whole block works through the item's pairs in turn
+// Hands a pair with a near test to the FP64 queue.  A side-A pair is marked
+// pending (G = 2 both ways) so that side B leaves it to the SLOW kernel, which
+// finishes side A and then, if no test removed the pair, runs its side B and
+// sets G back to 1.
+__device__ __forceinline__ void enqueue_slow(const ItemCtx &c, int j, const SlowItem &item,
+                                             SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount) {
+    const unsigned int slot = atomicAdd(qcount, 1u);
+    queue[slot] = item;
+    if (c.side == 0) {
+        c.J->G[(size_t)c.i * c.p + j] = 2;
+        c.J->G[(size_t)j * c.p + c.i] = 2;
+    }
+}
+
 template <int L, int MODE>
 __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount,
                             unsigned long long &n_exec, unsigned long long &n_useful,
@@ -806,7 +820,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
     for (int q = c.q0; q < c.q1; ++q) {
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
-        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] == 0) continue;
+        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] != 1) continue;  // removed, or pending in the SLOW kernel
         if constexpr (MODE == 1) {
             __syncthreads();  // the previous pair is done with the row
             stage_pair_row(c, j, c.prow, tid, 256);
@@ -863,8 +877,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
                 if (tid == 0) {
                     const float zb = z_of_rho(fminf(fminf(s_r[0], s_r[1]), fminf(s_r[2], s_r[3])));
                     if (any_near) {
-                        const unsigned int slot = atomicAdd(qcount, 1u);
-                        queue[slot] = SlowItem{iter * 256, c.rowoff[c.i] + q, c.job, zb, 0};
+                        enqueue_slow(c, j, SlowItem{iter * 256, c.rowoff[c.i] + q, c.job, zb, 0}, queue, qcount);
                     } else {
                         update_zmin(*c.J, c.p, c.i, j, zb * c.zscale);
                     }
@@ -893,7 +906,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
         if (q >= c.q1) break;
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
-        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] == 0) continue;
+        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] != 1) continue;  // removed, or pending in the SLOW kernel
         float *myrow = c.prow + (size_t)(threadIdx.x / team) * c.d;
         if constexpr (MODE == 1) {
             // the team's lanes stage C[j, adj(i)]; LDS accesses of one wave
@@ -919,10 +932,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
             }
             if (__ballot(near) & gmask) {
                 for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
-                if (gl == 0) {
-                    const unsigned int slot = atomicAdd(qcount, 1u);
-                    queue[slot] = SlowItem{iter * team, c.rowoff[c.i] + q, c.job, z_of_rho(rl), 0};
-                }
+                if (gl == 0) enqueue_slow(c, j, SlowItem{iter * team, c.rowoff[c.i] + q, c.job, z_of_rho(rl), 0}, queue, qcount);
                 break;
             }
             if (act) ++n_exec;
@@ -1176,22 +1186,27 @@ __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
 }
 
 // ----------------------------------------------------------------------------
-// level l >= 1, SLOW phase: finishes the pairs the FAST kernel queued, from the
-// queued subset rank on, rechecking near-threshold FP32 tests in FP64 with the
-// reference's operation order.  Work: a static share per lane group, then 8
-// striped counters chosen by block index (no single hot atomic).
+// level l >= 1, SLOW phase, after both FAST sides: finishes the pairs the FAST
+// kernels queued, from the queued subset rank on, rechecking near-threshold
+// FP32 tests in FP64 with the reference's operation order.  A side-A pair
+// (pending, G = 2) that no side-A test removes goes on with its side B from
+// rank 0 in the same lane group and gets G = 1 back.  Items: queue A, then
+// queue B.  Work: a static share per lane group, then 8 striped counters
+// chosen by block index (no single hot atomic).
 // ----------------------------------------------------------------------------
 template <int L, int GS>
 __global__ __launch_bounds__(256) void gauss_slow_kernel(
-    const Job *__restrict__ jobs, int side, float margin_k, unsigned int *__restrict__ stripe_counters,
-    const SlowItem *__restrict__ queue, const unsigned int *__restrict__ qcount,
+    const Job *__restrict__ jobs, float margin_k, unsigned int *__restrict__ stripe_counters,
+    const SlowItem *__restrict__ queueA, const unsigned int *__restrict__ qcountA,
+    const SlowItem *__restrict__ queueB, const unsigned int *__restrict__ qcountB,
     unsigned long long *__restrict__ bstats, unsigned int *__restrict__ stats_done,
     unsigned long long *__restrict__ st) {
     const int lane = __lane_id();
     const int gbase = lane & ~(GS - 1);
     const int gl = lane & (GS - 1);
     const uint64_t gmask = (GS == 64) ? ~0ull : (((1ull << GS) - 1ull) << gbase);
-    const int total = (int)*qcount;
+    const int totalA = (int)*qcountA;
+    const int total = totalA + (int)*qcountB;
 
     constexpr int GPB = 256 / GS;
     const int groups = gridDim.x * GPB;
@@ -1206,13 +1221,13 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
     int k_static = 0;
 
     const Job *J = nullptr;
-    int p = 0, i = 0, j = 0, jpos = 0, Nn = 0;
+    int p = 0, i = 0, j = 0, jpos = 0, Nn = 0, side = 0;
     const int32_t *gnb = nullptr;
     double thr = 0.0;
     float rthr = 0.0f;
     uint64_t count = 0, next = 0;
     float zl = __builtin_inff();
-    bool have = false, done = false;
+    bool have = false, done = false, restore = false;
     unsigned long long n_exec = 0, n_re = 0, n_flip = 0, n_useful = 0;
     float mr = 0.0f;
 
@@ -1236,7 +1251,8 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
             if (t >= total) {
                 done = true;
             } else {
-                const SlowItem qi = queue[t];
+                side = t < totalA ? 0 : 1;
+                const SlowItem qi = side == 0 ? queueA[t] : queueB[t - totalA];
                 J = &jobs[qi.job];
                 p = J->p;
                 JobWs jw = carve_job(J->ws, p);
@@ -1334,6 +1350,40 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 if (zs < *zp) *zp = zs;
             }
             have = false;
+            if (side == 0 && !pm) {
+                // side A kept the pending pair: its side B, row j, partner i
+                // (one of the row's neighbours below j, found by bisection)
+                const int row = j, partner = i;
+                JobWs jw = carve_job(J->ws, p);
+                const int d = jw.deg[row];
+                gnb = jw.nbr + (size_t)row * p;
+                int lo = 0, hi = d;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (gnb[mid] <= partner) lo = mid; else hi = mid;
+                }
+                i = row;
+                j = partner;
+                jpos = lo;
+                side = 1;
+                Nn = d - 1;
+                count = binom(Nn, L);
+                next = 0;
+                zl = __builtin_inff();
+                have = count > 0;
+                if (!have && gl == 0) {
+                    J->G[(size_t)i * p + j] = 1;
+                    J->G[(size_t)j * p + i] = 1;
+                }
+                restore = have;
+            } else if (restore) {
+                // a continued pair: side B done; G = 0 was written if it passed
+                if (!pm && gl == 0) {
+                    J->G[(size_t)i * p + j] = 1;
+                    J->G[(size_t)j * p + i] = 1;
+                }
+                restore = false;
+            }
         }
     }
     for (int off = 32; off >= 1; off >>= 1) {
@@ -1512,6 +1562,8 @@ static const char *const kPhaseSlowName[] = {
     "", "gauss_level1_slow", "gauss_level2_slow", "gauss_level3_slow", "gauss_level4_slow",
     "gauss_level5_slow", "gauss_level6_slow", "gauss_level7_slow", "gauss_level8_slow"};
 
+// Side 0: the FAST kernel.  Side 1: the FAST kernel, then the SLOW kernel for
+// the pairs both sides queued (and the level's statistics).
 template <int L>
 static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int gs_min, float margin_k,
                          unsigned long long *st, hipStream_t s) {
@@ -1519,7 +1571,6 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
     const WorkItem *items = side == 0 ? w.itemsA : w.itemsB;
     SlowItem *queue = side == 0 ? w.queueA : w.queueB;
     unsigned int *qcount = w.counters + CNT_QUEUE + side;
-    unsigned int *slow_ctr = w.counters + (2 * side + 1) * STRIPES;
     const int lds = g_tuning[L].fast_lds_bytes;
     if (g_tuning[L].wave_items)
         GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_wave_kernel<L>, dim3(fast_grid(L)), dim3(256), 0, s, jd, nj,
@@ -1527,8 +1578,10 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
     else
         GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj,
                       base, items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
-    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 16>), dim3(slow_grid()), dim3(256), 0, s, jd, side,
-                  margin_k, slow_ctr, queue, qcount, w.bstats, w.counters + CNT_STATS_DONE, side == 1 ? st : nullptr);
+    if (side == 0) return;
+    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 16>), dim3(slow_grid()), dim3(256), 0, s, jd, margin_k,
+                  w.counters + STRIPES, w.queueA, w.counters + CNT_QUEUE, w.queueB, w.counters + CNT_QUEUE + 1,
+                  w.bstats, w.counters + CNT_STATS_DONE, st);
 }
 
 int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float margin_k, int group_size,
