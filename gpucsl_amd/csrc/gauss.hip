@@ -232,9 +232,6 @@ struct PlanArgs {
 __device__ void jobprefix_body(const Job *__restrict__ jobs, int nj, int32_t *__restrict__ baseA,
                                int32_t *__restrict__ baseB, unsigned int *__restrict__ counters,
                                unsigned long long *__restrict__ bstats, int *sA, int *sB);
-__device__ void items_body(const Job *__restrict__ jobs, int nj, int level, const Team &team,
-                           const int32_t *__restrict__ baseA, const int32_t *__restrict__ baseB,
-                           WorkItem *__restrict__ itemsA, WorkItem *__restrict__ itemsB);
 
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = __lane_id();
@@ -452,8 +449,8 @@ __global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jo
 }
 
 // Per job (one block each) scan_job; in a batch with a plan, the block that
-// finishes last plans the level for every job: job bases, counter resets and
-// the work-item lists.
+// finishes last computes the job bases and resets the launch counters (the
+// work-item lists follow in items_kernel, one block per job).
 __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs, int level, Team team,
                                                     unsigned long long *__restrict__ st, PlanArgs plan) {
     const Job &J = jobs[blockIdx.x];
@@ -471,8 +468,26 @@ __global__ __launch_bounds__(1024) void scan_kernel(const Job *__restrict__ jobs
     if (!s_last) return;
     __threadfence();
     jobprefix_body(jobs, plan.nj, plan.baseA, plan.baseB, plan.counters, plan.bstats, sA, sB);
-    __syncthreads();
-    items_body(jobs, plan.nj, level, team, plan.baseA, plan.baseB, plan.itemsA, plan.itemsB);
+}
+
+// Work-item lists of a batch: block b writes job b's items {job, row, first
+// pair} at the job's base plus the row's item offset.
+__global__ __launch_bounds__(256) void items_kernel(const Job *__restrict__ jobs, int level, Team team, PlanArgs plan) {
+    const int b = blockIdx.x;
+    const Job &J = jobs[b];
+    if (level > J.max_level) return;
+    const int p = J.p;
+    JobWs w = carve_job(J.ws, p);
+    const int32_t ba = plan.baseA[b], bb = plan.baseB[b];
+    for (int r = threadIdx.x; r < p; r += blockDim.x) {
+        const uint64_t count = binom_level(w.deg[r] - 1, level);
+        if (!count) continue;
+        const int ppi = items_pairs(count, team);
+        const int ia = w.itemoffA[r], ib = w.itemoffB[r];
+        const int na = w.itemoffA[r + 1] - ia, nb = w.itemoffB[r + 1] - ib;
+        for (int k = 0; k < na; ++k) plan.itemsA[ba + ia + k] = WorkItem{b, r, k * ppi, 0};
+        for (int k = 0; k < nb; ++k) plan.itemsB[bb + ib + k] = WorkItem{b, r, k * ppi, 0};
+    }
 }
 
 // Work-item bases of every job of the launch (exclusive prefix over jobs of
@@ -522,28 +537,6 @@ __device__ void jobprefix_body(const Job *__restrict__ jobs, int nj, int32_t *__
     }
 }
 
-// Work-item lists of both sides, one entry per item {job, row, first pair}
-// at the job's base plus the row's item offset; rows strided over the block.
-__device__ void items_body(const Job *__restrict__ jobs, int nj, int level, const Team &team,
-                           const int32_t *__restrict__ baseA, const int32_t *__restrict__ baseB,
-                           WorkItem *__restrict__ itemsA, WorkItem *__restrict__ itemsB) {
-    for (int b = 0; b < nj; ++b) {
-        const Job &J = jobs[b];
-        if (level > J.max_level) continue;
-        const int p = J.p;
-        JobWs w = carve_job(J.ws, p);
-        for (int r = threadIdx.x; r < p; r += blockDim.x) {
-            const uint64_t count = binom_level(w.deg[r] - 1, level);
-            if (!count) continue;
-            const int ppi = items_pairs(count, team);
-            const int na = w.itemoffA[r + 1] - w.itemoffA[r], nb = w.itemoffB[r + 1] - w.itemoffB[r];
-            WorkItem *oa = itemsA + baseA[b] + w.itemoffA[r];
-            WorkItem *ob = itemsB + baseB[b] + w.itemoffB[r];
-            for (int k = 0; k < na; ++k) oa[k] = WorkItem{b, r, k * ppi, 0};
-            for (int k = 0; k < nb; ++k) ob[k] = WorkItem{b, r, k * ppi, 0};
-        }
-    }
-}
 
 // ----------------------------------------------------------------------------
 // FP32 partial-correlation test on the (l+2) x (l+2) submatrix ordered
@@ -1547,6 +1540,8 @@ static int compact_and_plan(const Job *jh, const Job *jd, int nj, int level, int
                                gs_min);
     GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd, level, team, st, plan);
     GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, team, st, plan);
+    if (plan.done && nj > 1)
+        GPUCSL_LAUNCH("compact_items", items_kernel, dim3(nj), dim3(256), 0, s, jd, level, team, plan);
     return check_hip(hipGetLastError(), "compact launch");
 }
 
