@@ -1539,7 +1539,10 @@ static int compact_and_plan(const Job *jh, const Job *jd, int nj, int level, int
     const Team team = team_for(level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
                                gs_min);
     GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd, level, team, st, plan);
-    GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, team, st, plan);
+    // the scan only when a plan or the level statistics are wanted (the
+    // neighbour lists and degrees alone come from the compaction)
+    if (plan.done || st)
+        GPUCSL_LAUNCH("compact_scan", scan_kernel, dim3(nj), dim3(1024), 0, s, jd, level, team, st, plan);
     if (plan.done && nj > 1)
         GPUCSL_LAUNCH("compact_items", items_kernel, dim3(nj), dim3(256), 0, s, jd, level, team, plan);
     return check_hip(hipGetLastError(), "compact launch");
