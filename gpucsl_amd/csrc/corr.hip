@@ -18,10 +18,17 @@ constexpr int LDA = CT + 4;
 
 __global__ void write_job_kernel(Job *dst, Job src) { *dst = src; }
 
-__host__ __device__ inline int upper_tiles(int p) {
-    const int t = (p + CT - 1) / CT;
+__host__ __device__ inline int upper_tiles(int p, int ct = CT) {
+    const int t = (p + ct - 1) / ct;
     return t * (t + 1) / 2;
 }
+
+// Datasets with p <= GRAM_SMALL_P take their Gram matrix in 32 x 32 tiles of
+// one wave each (at p = 200, 70 % of the MFMA work of the upper tiles is
+// needed, against 49 % with 64 x 64 tiles).  Each entry accumulates the same
+// k-slabs in the same order either way, so the bits do not depend on the tile.
+constexpr int GRAM_SMALL_P = 256;
+__host__ __device__ inline int gram_tile(int p) { return p <= GRAM_SMALL_P ? 32 : CT; }
 
 // Row splits of the column sums: a function of the job's own n, so a job's
 // means are the same bits alone and in any batch.
@@ -86,27 +93,29 @@ __global__ __launch_bounds__(256) void colsum_one_kernel(Job one, Job *dst) {
 // KT rows are double-buffered in LDS: the global loads of slab s+1 are in
 // flight while the MFMAs of slab s run, and centring (x - mean, in FP64)
 // happens when a fragment is read.
-template <typename T>
-__global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ jobs) {
-    __shared__ T As[2][KT][LDA];
-    __shared__ T Bs[2][KT][LDA];
+template <typename T, int CTT>
+__global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const Job *__restrict__ jobs) {
+    constexpr int NT = CTT == CT ? 256 : 64;  // 4 waves of 32 x 32, or one
+    constexpr int RPP = NT / CTT;             // slab rows per loader pass
+    __shared__ T As[2][KT][CTT + 4];
+    __shared__ T Bs[2][KT][CTT + 4];
 
     const Job &J = jobs[blockIdx.z];
     const int p = J.p;
     const int64_t n = J.n;
-    if ((int)blockIdx.x >= upper_tiles(p) || (int)blockIdx.y >= J.k_splits) return;
+    if (gram_tile(p) != CTT || (int)blockIdx.x >= upper_tiles(p, CTT) || (int)blockIdx.y >= J.k_splits) return;
     const T *__restrict__ X = (const T *)J.X;
     CorrWs cw = carve_corr(J.corr_ws, p);
 
     // upper-triangle tile id -> (ta, tb), ta <= tb, row-major over the triangle
-    const int tiles = (p + CT - 1) / CT;
+    const int tiles = (p + CTT - 1) / CTT;
     int t = blockIdx.x, ta = 0;
     while (t >= tiles - ta) {
         t -= tiles - ta;
         ++ta;
     }
     const int tb = ta + t;
-    const int a0 = ta * CT, b0 = tb * CT;
+    const int a0 = ta * CTT, b0 = tb * CTT;
     const int split = blockIdx.y;
     int64_t k_per_split = (n + J.k_splits - 1) / J.k_splits;
     k_per_split = (k_per_split + KT - 1) / KT * KT;
@@ -118,8 +127,8 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wr = (wave >> 1) * 32;  // wave row offset in tile
-    const int wc = (wave & 1) * 32;   // wave col offset in tile
+    const int wr = CTT == CT ? (wave >> 1) * 32 : 0;  // wave row offset in tile
+    const int wc = CTT == CT ? (wave & 1) * 32 : 0;   // wave col offset in tile
     const int ci = lane & 15;
 
     f64x4 acc[2][2];
@@ -130,9 +139,9 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
 
     // column means of the two tile edges: mean[a] = (sum of the row-split
     // partial sums, in split order) / n
-    __shared__ double s_mean[2][CT];
-    if (tid < 2 * CT) {
-        const int col = (tid < CT ? a0 : b0) + (tid & (CT - 1));
+    __shared__ double s_mean[2][CTT];
+    if (tid < 2 * CTT) {
+        const int col = (tid < CTT ? a0 : b0) + (tid & (CTT - 1));
         double m = 0.0;
         if (col < p) {
             const int rs = corr_row_splits(n);
@@ -149,7 +158,7 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
             for (; q < rs; ++q) m += pc[(size_t)q * p];
             m = m / (double)n;
         }
-        s_mean[tid >> 6][tid & (CT - 1)] = m;
+        s_mean[tid / CTT][tid & (CTT - 1)] = m;
     }
     __syncthreads();
     double ma[2], mb[2];
@@ -159,17 +168,17 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
         mb[i] = s_mean[1][wc + i * 16 + ci];
     }
 
-    // loader: 256 threads cover KT rows x 64 cols as KT/4 passes of 4 rows
-    const int lcol = tid & 63;
-    const int lrow = tid >> 6;
+    // loader: NT threads cover KT rows x CTT cols as KT/RPP passes of RPP rows
+    const int lcol = tid & (CTT - 1);
+    const int lrow = tid / CTT;
     const int ga = a0 + lcol, gb = b0 + lcol;
-    T ra[KT / 4], rb[KT / 4];
+    T ra[KT / RPP], rb[KT / RPP];
 
     auto load = [&](int s) {
         const int64_t kk = k0 + (int64_t)s * KT;
 #pragma unroll
-        for (int pass = 0; pass < KT / 4; ++pass) {
-            const int64_t k = kk + pass * 4 + lrow;
+        for (int pass = 0; pass < KT / RPP; ++pass) {
+            const int64_t k = kk + pass * RPP + lrow;
             const bool kv = k < k1;
             ra[pass] = (kv && ga < p) ? X[k * p + ga] : T(0);
             rb[pass] = (kv && gb < p) ? X[k * p + gb] : T(0);
@@ -177,9 +186,9 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(const Job *__restrict__ 
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int pass = 0; pass < KT / 4; ++pass) {
-            As[buf][pass * 4 + lrow][lcol] = ra[pass];
-            Bs[buf][pass * 4 + lrow][lcol] = rb[pass];
+        for (int pass = 0; pass < KT / RPP; ++pass) {
+            As[buf][pass * RPP + lrow][lcol] = ra[pass];
+            Bs[buf][pass * RPP + lrow][lcol] = rb[pass];
         }
     };
 
@@ -355,7 +364,8 @@ __global__ void c64_to_c32_kernel(const Job *__restrict__ jobs) {
 // the device only, so a dataset gets the same ks, and the same bits, alone and
 // in any batch.
 int corr_k_splits(int64_t n, int p) {
-    const int64_t tu = upper_tiles(p);
+    // work per split in 4-wave units: a 32 x 32 tile is one wave
+    const int64_t tu = gram_tile(p) == CT ? upper_tiles(p) : (upper_tiles(p, 32) + 3) / 4;
     const int64_t cus = device_cus();
     int64_t kmax = (n + 255) / 256;
     if (kmax > 16) kmax = 16;
@@ -376,7 +386,7 @@ int corr_k_splits(int64_t n, int p) {
 // With `one` (nj == 1), jobs_dev is where the first launch stores *one.
 static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, const Job *one, hipStream_t s) {
     if (nj < 1) return OK;
-    int maxp = 0, maxks = 1, maxtiles = 1, dtype = jobs_host[0].x_dtype;
+    int maxp = 0, maxks = 1, maxtiles = 0, maxtiles32 = 0, maxnt = 1, dtype = jobs_host[0].x_dtype;
     int64_t maxn = 0;
     for (int b = 0; b < nj; ++b) {
         const Job &J = jobs_host[b];
@@ -386,7 +396,11 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
         maxp = J.p > maxp ? J.p : maxp;
         maxn = J.n > maxn ? J.n : maxn;
         maxks = J.k_splits > maxks ? J.k_splits : maxks;
-        maxtiles = upper_tiles(J.p) > maxtiles ? upper_tiles(J.p) : maxtiles;
+        // Gram tiles by tile size; normalisation tiles are always 64 x 64
+        const int ut = upper_tiles(J.p, gram_tile(J.p));
+        int &mt = gram_tile(J.p) == CT ? maxtiles : maxtiles32;
+        mt = ut > mt ? ut : mt;
+        maxnt = upper_tiles(J.p) > maxnt ? upper_tiles(J.p) : maxnt;
     }
     if (dtype != 0 && dtype != 1) return set_error(ERR_ARG, "corr: dtype must be 0 (f32) or 1 (f64)");
     const int rs = corr_row_splits(maxn);
@@ -400,13 +414,23 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
         GPUCSL_LAUNCH("corr_colsum", colsum_kernel<float>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
     else
         GPUCSL_LAUNCH("corr_colsum", colsum_kernel<double>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
-    if (dtype == 0)
-        GPUCSL_LAUNCH("corr_gram_mfma_f64", gram_mfma_kernel<float>, dim3(maxtiles, maxks, nj), dim3(256), 0, s,
-                      jobs_dev);
-    else
-        GPUCSL_LAUNCH("corr_gram_mfma_f64", gram_mfma_kernel<double>, dim3(maxtiles, maxks, nj), dim3(256), 0, s,
-                      jobs_dev);
-    GPUCSL_LAUNCH("corr_normalize", normalize_tile_kernel, dim3(maxtiles * NQ, 1, nj), dim3(256), 0, s, jobs_dev);
+    if (maxtiles > 0) {
+        if (dtype == 0)
+            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0, s,
+                          jobs_dev);
+        else
+            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0,
+                          s, jobs_dev);
+    }
+    if (maxtiles32 > 0) {
+        if (dtype == 0)
+            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, 32>), dim3(maxtiles32, maxks, nj), dim3(64), 0,
+                          s, jobs_dev);
+        else
+            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, 32>), dim3(maxtiles32, maxks, nj), dim3(64),
+                          0, s, jobs_dev);
+    }
+    GPUCSL_LAUNCH("corr_normalize", normalize_tile_kernel, dim3(maxnt * NQ, 1, nj), dim3(256), 0, s, jobs_dev);
     return check_hip(hipGetLastError(), "corr launch");
 }
 
