@@ -106,7 +106,9 @@ typedef struct gpucsl_gauss_job {
 } gpucsl_gauss_job;
 
 /* Per-job workspaces of the batched entry points, and the split-K factor the
- * correlation GEMM uses for an n x p dataset. */
+ * correlation GEMM uses for an n x p dataset (1..16, chosen from n, p and the
+ * current device's compute-unit count; the same for a dataset alone and in
+ * any batch on that device, which keeps its correlation bits the same). */
 size_t gpucsl_gauss_job_workspace_bytes(int p);
 int gpucsl_corr_k_splits(int64_t n, int p);
 size_t gpucsl_corr_job_workspace_bytes(int p, int k_splits);

@@ -104,6 +104,7 @@ def test_compact_random_large():
 
 @pytest.mark.parametrize("p,n,dtype", [(50, 2000, np.float64), (64, 1000, np.float32),
                                        (65, 333, np.float32), (130, 2049, np.float64),
+                                       (256, 3001, np.float32), (257, 3001, np.float32),
                                        (1000, 10000, np.float32)])
 def test_corr_matches_numpy(p, n, dtype):
     X = syn.gaussian_sem(p, n, 3.0, seed=p + n).data.astype(dtype)
