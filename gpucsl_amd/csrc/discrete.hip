@@ -328,10 +328,12 @@ __global__ __launch_bounds__(256) void disc_onehot_gram_kernel(const uint8_t *__
 // read from the Gram matrix, integer marginals, chi^2 summed over the cells in
 // (a, b) order with the per-cell term of chi2_term, Perlman's pchisq, then pmax
 // and G as disc_level0_kernel writes them.  MD >= max_dim bounds the
-// marginal arrays.
+// marginal arrays.  Each cell is zeroed after its last read: the cells never
+// read are those of categories without observations (never added to) and the
+// diagonal blocks (never read), so the next level-0 run starts from zero.
 template <int MD>
 __global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__restrict__ dims, int p, int D, int W,
-                                                               const int32_t *__restrict__ N, double alpha,
+                                                               int32_t *__restrict__ N, double alpha,
                                                                uint8_t *__restrict__ Gm, float *__restrict__ pmax,
                                                                unsigned long long *__restrict__ st) {
 #pragma clang fp contract(off)
@@ -352,7 +354,7 @@ __global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__
     const int vj = (int)(t - (long long)vi * (vi - 1) / 2);
     const int di = dims[vi], dj = dims[vj];
     // N[a][b] = Gram[vj D + b][vi D + a] (upper triangle: vj D + b < vi D)
-    const int32_t *T = N + (size_t)vj * D * W + (size_t)vi * D;
+    int32_t *T = N + (size_t)vj * D * W + (size_t)vi * D;
     uint32_t Ni[MD], Nj[MD], Ns = 0;
 #pragma unroll
     for (int a = 0; a < MD; ++a) Ni[a] = Nj[a] = 0;
@@ -372,7 +374,9 @@ __global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__
 #pragma unroll
         for (int b = 0; b < MD; ++b)
             if (a < di && b < dj) {
-                const uint32_t o = (uint32_t)T[(size_t)b * W + a];
+                int32_t *cell = T + (size_t)b * W + a;
+                const uint32_t o = (uint32_t)*cell;
+                *cell = 0;  // leave the pair's table zero for the next level-0 run
                 const double e = Ni[a] * (double)Nj[b] / Ns;
                 double term = 0.0;
                 if (!(Ns == 0 || e == 0.0)) {

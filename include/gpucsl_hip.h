@@ -238,8 +238,9 @@ int gpucsl_disc_init(int p, int max_level, int max_dim, uint8_t *G, int32_t *sep
  * discrete_ci_level_0 (gpucsl/pc/cuda/discrete_ci.cu:37-97).  Runs after
  * gpucsl_disc_init on the same workspace.  With max_dim <= 16 and
  * p * max_dim <= 4096 every pair's table comes from one int8-MFMA Gram matrix
- * of the one-hot codes; otherwise tables are counted pair by pair.  Results
- * are the same either way. */
+ * of the one-hot codes (zeroed by gpucsl_disc_init and left zero by each run,
+ * so level 0 may run again on the same workspace); otherwise tables are
+ * counted pair by pair.  Results are the same either way. */
 int gpucsl_disc_level0(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
                        int max_level, int max_dim, double alpha, uint8_t *G, float *pmax,
                        uint64_t *stats_level0, void *ws, size_t ws_bytes, void *stream);

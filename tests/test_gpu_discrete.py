@@ -161,6 +161,23 @@ def test_level0_table_paths(p, card, n, label):
     _compare(res, ref, label)
 
 
+def test_level0_runs_again_on_the_same_workspace():
+    """The level-0 table matrix is left zero by each run, so level 0 can run
+    again after gpucsl_disc_init without double counting."""
+    d = syn.discrete_bn(20, 3000, 2.0, seed=12)
+    eng, dims = _engine(d.data, 2, 0.05)
+    eng.start()
+    torch.cuda.synchronize()
+    G1, pm1 = eng.G.cpu().numpy().copy(), eng.pmax.cpu().numpy().copy()
+    s = nat.stream_handle()
+    nat.call("gpucsl_disc_level0", eng.data.data_ptr(), eng.n, eng.stride, eng.dims.data_ptr(), eng.p,
+             eng.max_level, eng.max_dim, eng.alpha, eng.G.data_ptr(), eng.pmax.data_ptr(),
+             eng.stats.data_ptr(), eng.ws.data_ptr(), eng.ws_bytes, s)
+    torch.cuda.synchronize()
+    assert (eng.G.cpu().numpy() == G1).all()
+    assert np.array_equal(eng.pmax.cpu().numpy().view(np.uint32), pm1.view(np.uint32))
+
+
 def test_constant_column_and_unaligned_n():
     d = syn.discrete_bn(10, 1237, 2.0, seed=5)
     codes = np.concatenate([d.data, np.zeros((1237, 1), np.uint8)], axis=1)
