@@ -105,6 +105,8 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
 int gauss_batch_finish(const Job *jh, const Job *jd, int nj, hipStream_t s);
 size_t disc_workspace_bytes(int p, int max_level, int max_dim);
 size_t disc_min_workspace_bytes(int p, int max_level, int max_dim);
+int disc_encode_launch(const void *values, int dtype, int64_t n, int p, int64_t ld, int row_major, uint8_t *codes,
+                       int64_t stride, uint8_t *dims, int32_t *status, hipStream_t s);
 int disc_init_launch(int p, int max_level, int max_dim, uint8_t *Gm, int32_t *sep, float *pmax, void *ws,
                      unsigned long long *st, int nst, hipStream_t s);
 int disc_level0_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int p,
@@ -408,6 +410,11 @@ int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const u
         [&](int lv, unsigned long long *st) { return compact_launch(G, p, lv, ws, st, S(stream)); }, S(stream));
     if (rc) return rc;
     return lower_minus_one_launch(pmax, p, S(stream));
+}
+
+int gpucsl_disc_encode(const void *values, int dtype, int64_t n, int p, int64_t ld, int row_major, uint8_t *codes,
+                       int64_t stride, uint8_t *dims, int32_t *status, void *stream) {
+    return disc_encode_launch(values, dtype, n, p, ld, row_major, codes, stride, dims, status, S(stream));
 }
 
 int gpucsl_disc_finish(const uint8_t *G, int p, float *pmax, uint16_t *skeleton, void *stream) {

@@ -788,6 +788,126 @@ int disc_level_launch(int level, const uint8_t *data, int64_t n, int64_t stride,
     return check_hip(hipGetLastError(), "disc_level launch");
 }
 
+// ----------------------------------------------------------------------------
+// ordinal encoding on the device (gpucsl/pc/discover_skeleton_discrete.py:38-39,
+// sklearn's OrdinalEncoder): a value's code is its rank among the column's
+// sorted distinct values.  One block per column: the distinct values are found
+// in increasing order, each as the block-wide minimum of the values above the
+// previous one (one pass over the column per category), kept in LDS, and every
+// value is then coded by bisection.  At most 255 categories; NaN is reported.
+// ----------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void disc_encode_kernel(const T *__restrict__ values, int64_t n, int64_t ld,
+                                                          int row_major, uint8_t *__restrict__ codes, int64_t stride,
+                                                          uint8_t *__restrict__ dims, int32_t *__restrict__ status) {
+    const int v = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ T s_val[256];
+    __shared__ T s_m[4];
+    __shared__ int s_h[4], s_nan[4];
+    __shared__ int s_cnt, s_more;
+    auto at = [&](int64_t k) -> T { return row_major ? values[k * ld + v] : values[(size_t)v * ld + k]; };
+    int cnt = 0, any_nan = 0;
+    T prev = T(0);
+    for (;;) {
+        T m = T(0);
+        int has = 0;
+        for (int64_t k = tid; k < n; k += 256) {
+            const T x = at(k);
+            if (!(x == x)) {
+                any_nan = 1;
+                continue;
+            }
+            if ((cnt == 0 || x > prev) && (!has || x < m)) {
+                m = x;
+                has = 1;
+            }
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const T om = __shfl_xor(m, off);
+            const int oh = __shfl_xor(has, off);
+            if (oh && (!has || om < m)) {
+                m = om;
+                has = 1;
+            }
+        }
+        const int wn = __any(any_nan) ? 1 : 0;
+        if (lane == 0) {
+            s_m[wave] = m;
+            s_h[wave] = has;
+            s_nan[wave] = wn;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            T bm = T(0);
+            int bh = 0;
+            for (int w = 0; w < 4; ++w)
+                if (s_h[w] && (!bh || s_m[w] < bm)) {
+                    bm = s_m[w];
+                    bh = 1;
+                }
+            s_more = bh;
+            if (bh && cnt < 255) s_val[cnt] = bm;
+            s_nan[0] = s_nan[0] | s_nan[1] | s_nan[2] | s_nan[3];
+        }
+        __syncthreads();
+        any_nan = s_nan[0];
+        if (!s_more) break;
+        if (cnt == 255) {  // a 256th distinct value
+            cnt = 256;
+            break;
+        }
+        prev = s_val[cnt];
+        ++cnt;
+        __syncthreads();  // s_m / s_h / s_more are reused
+    }
+    if (tid == 0) {
+        dims[v] = cnt <= 255 ? (uint8_t)cnt : 0;
+        status[v] = cnt > 255 ? 1 : (any_nan ? 2 : 0);
+    }
+    if (cnt > 255 || any_nan) return;
+    for (int64_t k = tid; k < stride; k += 256) {
+        uint8_t code = 0;
+        if (k < n) {
+            const T x = at(k);
+            int lo = 0, hi = cnt;  // s_val[lo] <= x < s_val[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_val[mid] <= x) lo = mid; else hi = mid;
+            }
+            code = (uint8_t)lo;
+        }
+        codes[(size_t)v * stride + k] = code;
+    }
+}
+
+int disc_encode_launch(const void *values, int dtype, int64_t n, int p, int64_t ld, int row_major, uint8_t *codes,
+                       int64_t stride, uint8_t *dims, int32_t *status, hipStream_t s) {
+    if (!values || !codes || !dims || !status || n < 1 || p < 1 || stride < n || stride % 16 != 0)
+        return set_error(ERR_ARG, "disc_encode: bad arguments");
+    if (ld < (row_major ? p : n)) return set_error(ERR_ARG, "disc_encode: leading dimension too small");
+    switch (dtype) {
+    case 0:
+        GPUCSL_LAUNCH("disc_encode", disc_encode_kernel<int32_t>, dim3(p), dim3(256), 0, s,
+                      (const int32_t *)values, n, ld, row_major, codes, stride, dims, status);
+        break;
+    case 1:
+        GPUCSL_LAUNCH("disc_encode", disc_encode_kernel<long long>, dim3(p), dim3(256), 0, s,
+                      (const long long *)values, n, ld, row_major, codes, stride, dims, status);
+        break;
+    case 2:
+        GPUCSL_LAUNCH("disc_encode", disc_encode_kernel<float>, dim3(p), dim3(256), 0, s, (const float *)values,
+                      n, ld, row_major, codes, stride, dims, status);
+        break;
+    case 3:
+        GPUCSL_LAUNCH("disc_encode", disc_encode_kernel<double>, dim3(p), dim3(256), 0, s, (const double *)values,
+                      n, ld, row_major, codes, stride, dims, status);
+        break;
+    default:
+        return set_error(ERR_ARG, "disc_encode: dtype must be 0 (int32), 1 (int64), 2 (float32) or 3 (float64)");
+    }
+    return check_hip(hipGetLastError(), "disc_encode launch");
+}
+
 int disc_single_test_launch(const uint8_t *data, int64_t n, int64_t stride, const uint8_t *dims, int vi, int vj,
                             const int32_t *S, int l, int sequential, uint32_t *table_out, double *out,
                             hipStream_t s) {

@@ -34,6 +34,54 @@ def ordinal_encode(data: np.ndarray):
     return codes, dims.astype(np.uint8)
 
 
+# numpy dtype -> (dtype code of gpucsl_disc_encode, numpy dtype it is passed as)
+def _encode_dtype(a: np.ndarray):
+    k, size = a.dtype.kind, a.dtype.itemsize
+    if k == "b" or (k in "iu" and size <= 2) or (k == "i" and size == 4):
+        return 0, np.int32
+    if (k == "i" and size == 8) or (k == "u" and size == 4):
+        return 1, np.int64
+    if k == "u" and size == 8 and (a.size == 0 or int(a.max()) < (1 << 63)):
+        return 1, np.int64
+    if k == "f" and size <= 4:
+        return 2, np.float32
+    if k == "f" and size == 8:
+        return 3, np.float64
+    return None
+
+
+def ordinal_encode_device(data, device=None):
+    """ordinal_encode on the GPU for numeric data: the values go to the device
+    as they are and gpucsl_disc_encode codes them there.  Returns (codes (p,
+    stride) uint8 tensor, padded column-major; dims (p,) uint8 tensor; dims as a
+    host array), or None for data it does not take (non-numeric, or NaN, which
+    the host encoder treats as a category)."""
+    torch = nat.require_gpu()
+    a = np.asarray(data)
+    if a.ndim != 2 or a.size == 0:
+        return None
+    kind = _encode_dtype(a)
+    if kind is None:
+        return None
+    code, np_dtype = kind
+    n, p = a.shape
+    dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+    stride = (n + COLUMN_ALIGN - 1) // COLUMN_ALIGN * COLUMN_ALIGN
+    vals = torch.from_numpy(np.ascontiguousarray(a, dtype=np_dtype)).to(dev)
+    codes = torch.empty((p, stride), dtype=torch.uint8, device=dev)
+    dims = torch.empty((p,), dtype=torch.uint8, device=dev)
+    status = torch.empty((p,), dtype=torch.int32, device=dev)
+    nat.call("gpucsl_disc_encode", vals.data_ptr(), code, n, p, p, 1, codes.data_ptr(), stride, dims.data_ptr(),
+             status.data_ptr(), nat.stream_handle())
+    st = status.cpu().numpy()
+    if (st == 1).any():
+        v = int(np.flatnonzero(st == 1)[0])
+        raise GpucslError(f"variable {v} has more than 255 categories; at most 255 are supported")
+    if (st == 2).any():
+        return None
+    return codes, dims, dims.cpu().numpy()
+
+
 @dataclass
 class DiscreteResult:
     skeleton: np.ndarray          # (p, p) uint16

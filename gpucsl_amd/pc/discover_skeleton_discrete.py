@@ -24,8 +24,9 @@ def discover_skeleton_gpu_discrete(
 ) -> SkeletonResult:
     """Signature and result of gpucsl/pc/discover_skeleton_discrete.py:24-128.
 
-    The data are ordinal-encoded on the host, uploaded once in a padded
-    column-major layout and every level runs on the GPU.  ``memory_restriction``
+    Numeric data are uploaded once and ordinal-encoded on the GPU into a padded
+    column-major layout (other data are encoded on the host); every level runs
+    on the GPU.  ``memory_restriction``
     bounds the workspace in bytes: below the full size fewer CI tests run at
     once, and a RuntimeError is raised if it cannot hold one test of max_level
     (the reference raises in the same situation,
@@ -33,11 +34,15 @@ def discover_skeleton_gpu_discrete(
     """
     import torch
 
-    from gpucsl_amd.discrete import DiscreteEngine, ordinal_encode
+    from gpucsl_amd.discrete import DiscreteEngine, ordinal_encode, ordinal_encode_device
 
     if skeleton is not None and tuple(np.shape(skeleton)) != (num_variables, num_variables):
         raise GpucslError(f"skeleton must be {num_variables}x{num_variables}")
-    codes, dims = ordinal_encode(data)
+    dev_codes = ordinal_encode_device(data)
+    if dev_codes is not None:
+        codes_t, dims_t, dims = dev_codes
+    else:
+        codes, dims = ordinal_encode(data)
     max_dim = int(dims.max()) if dims.size else 1
     ci = getattr(kernels, "ci_test", None)
     if memory_restriction is None and ci is not None and getattr(ci, "max_memory_size", None):
@@ -46,7 +51,10 @@ def discover_skeleton_gpu_discrete(
     # minimum and the full size fewer tests run at once
     engine = DiscreteEngine(num_variables, num_observations, max_level, alpha, max_dim,
                             ws_limit=memory_restriction)
-    engine.set_codes(codes, dims)
+    if dev_codes is not None:
+        engine.set_codes_device(codes_t, dims_t)
+    else:
+        engine.set_codes(codes, dims)
     torch.cuda.synchronize()
     start = timer()
     engine.run()

@@ -228,6 +228,17 @@ size_t gpucsl_disc_workspace_bytes(int p, int max_level, int max_dim);
  * parallelism under a memory restriction. */
 size_t gpucsl_disc_min_workspace_bytes(int p, int max_level, int max_dim);
 
+/* Ordinal codes on the device, as sklearn's OrdinalEncoder: a value's code is
+ * its rank among its column's sorted distinct values (replaces the encoding of
+ * gpucsl/pc/discover_skeleton_discrete.py:38-39).  values: device n x p array
+ * with element (k, v) at values[k*ld + v] (row_major = 1) or values[v*ld + k]
+ * (row_major = 0); dtype 0 int32, 1 int64, 2 float32, 3 float64.  codes: p x
+ * stride uint8 (stride >= n, a multiple of 16; padding set to 0); dims: p
+ * category counts; status: p int32, 0 ok, 1 more than 255 distinct values
+ * (dims 0, codes unset), 2 NaN present (codes unset). */
+int gpucsl_disc_encode(const void *values, int dtype, int64_t n, int p, int64_t ld, int row_major, uint8_t *codes,
+                       int64_t stride, uint8_t *dims, int32_t *status, void *stream);
+
 /* G = 1, sepsets = -1, pmax = 0, stats = 0, removal keys cleared and the
  * level-0 table matrix zeroed, in one launch.  Replaces the buffer set-up of
  * gpucsl/pc/discover_skeleton_discrete.py:41-55. */

@@ -223,3 +223,40 @@ def test_memory_restriction_runs_fewer_tests_at_once_same_result():
     say(f"workspace full {full} min {least}")
     assert (a.skeleton == b.skeleton).all() and (a.seperation_sets == b.seperation_sets).all()
     assert np.array_equal(a.pmax.view(np.uint32), b.pmax.view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", [np.int64, np.int32, np.int16, np.uint8, np.bool_, np.float64, np.float32])
+def test_device_ordinal_encoding_matches_host(dtype):
+    """gpucsl_disc_encode gives the host encoder's codes and category counts
+    (sorted distinct values -> 0..k-1) for each numeric dtype."""
+    from gpucsl_amd.discrete import ordinal_encode, ordinal_encode_device
+
+    rng = np.random.default_rng(7)
+    n, p = 1237, 9
+    if dtype == np.bool_:
+        raw = rng.integers(0, 2, size=(n, p)).astype(np.bool_)
+    elif np.issubdtype(dtype, np.floating):
+        raw = (rng.integers(-3, 4, size=(n, p)) * 0.37 + 1e-3).astype(dtype)
+    elif dtype == np.uint8:
+        raw = (rng.integers(0, 6, size=(n, p)) * 40 + 3).astype(dtype)
+    else:
+        raw = (rng.integers(-4, 5, size=(n, p)) * 1000 - 17).astype(dtype)
+    raw[:, 3] = raw[0, 3]                      # a constant column
+    want_codes, want_dims = ordinal_encode(raw)
+    codes_t, dims_t, dims = ordinal_encode_device(raw)
+    codes = codes_t.cpu().numpy()
+    assert (dims == want_dims).all()
+    assert (codes[:, :n].T == want_codes).all()
+    assert (codes[:, n:] == 0).all()
+
+
+def test_device_ordinal_encoding_limits():
+    from gpucsl_amd.discrete import ordinal_encode_device
+
+    wide = np.tile(np.arange(300, dtype=np.int64)[:, None], (1, 3))
+    with pytest.raises(GpucslError):
+        ordinal_encode_device(wide)
+    with_nan = np.ones((50, 2))
+    with_nan[4, 1] = np.nan
+    assert ordinal_encode_device(with_nan) is None        # left to the host encoder
+    assert ordinal_encode_device(np.array([["a", "b"], ["b", "a"]])) is None
