@@ -14,7 +14,6 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CT = 64;   // output tile edge
 constexpr int KT = 16;   // k-slab per LDS stage
-constexpr int LDA = CT + 4;
 
 __global__ void write_job_kernel(Job *dst, Job src) { *dst = src; }
 

@@ -804,7 +804,7 @@ __global__ __launch_bounds__(256) void disc_encode_kernel(const T *__restrict__ 
     __shared__ T s_val[256];
     __shared__ T s_m[4];
     __shared__ int s_h[4], s_nan[4];
-    __shared__ int s_cnt, s_more;
+    __shared__ int s_more;
     auto at = [&](int64_t k) -> T { return row_major ? values[k * ld + v] : values[(size_t)v * ld + k]; };
     int cnt = 0, any_nan = 0;
     T prev = T(0);

@@ -29,7 +29,6 @@
 namespace gpucsl {
 
 constexpr float EPS32 = 5.9604645e-8f;  // 2^-24
-constexpr int PHASE_BLOCKS_PER_CU = 8;
 // FAST kernel: 256-thread blocks with an LDS staging area of at most 64 KB
 constexpr int FB_MAX_LDS_BYTES = 65536;
 
