@@ -260,3 +260,26 @@ def test_device_ordinal_encoding_limits():
     with_nan[4, 1] = np.nan
     assert ordinal_encode_device(with_nan) is None        # left to the host encoder
     assert ordinal_encode_device(np.array([["a", "b"], ["b", "a"]])) is None
+
+
+def test_device_ordinal_encoding_column_major_input():
+    """gpucsl_disc_encode on column-major values with a padded leading
+    dimension gives the same codes as on the row-major layout."""
+    from gpucsl_amd.discrete import ordinal_encode
+
+    rng = np.random.default_rng(11)
+    n, p, ld = 999, 5, 1024
+    raw = rng.integers(-2, 3, size=(n, p)).astype(np.int64) * 7
+    cm = np.zeros((p, ld), np.int64)
+    cm[:, :n] = raw.T
+    vals = torch.from_numpy(cm).cuda()
+    stride = 1024
+    codes = torch.empty((p, stride), dtype=torch.uint8, device="cuda")
+    dims = torch.empty((p,), dtype=torch.uint8, device="cuda")
+    status = torch.empty((p,), dtype=torch.int32, device="cuda")
+    nat.call("gpucsl_disc_encode", vals.data_ptr(), 1, n, p, ld, 0, codes.data_ptr(), stride, dims.data_ptr(),
+             status.data_ptr(), nat.stream_handle())
+    want_codes, want_dims = ordinal_encode(raw)
+    assert (status.cpu().numpy() == 0).all()
+    assert (dims.cpu().numpy() == want_dims).all()
+    assert (codes.cpu().numpy()[:, :n].T == want_codes).all()
