@@ -63,16 +63,29 @@ def cpu_baseline(X: np.ndarray, n: int, alpha: float, max_level: int):
     import oracle
 
     oracle.lib()
-    t0 = time.perf_counter()
-    C = oracle.corrcoef(X.astype(np.float64))
-    t1 = time.perf_counter()
-    ref = oracle.gauss_skeleton(C, n, alpha, max_level)
-    t2 = time.perf_counter()
-    tests = int(ref["executed"].sum())
-    return dict(value=tests / (t2 - t0), unit="CI tests/s", cores=oracle.num_threads(), kind="port",
-                sample=(f"full workload once: corrcoef {t1 - t0:.3f}s + skeleton {t2 - t1:.3f}s, "
+    X64 = X.astype(np.float64)
+
+    def once():
+        t0 = time.perf_counter()
+        C = oracle.corrcoef(X64)
+        t1 = time.perf_counter()
+        ref = oracle.gauss_skeleton(C, n, alpha, max_level)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, int(ref["executed"].sum())
+
+    tc, ts, tests = once()
+    cores = oracle.num_threads()
+    oracle.set_threads(1)
+    try:
+        tc1, ts1, _ = once()
+    finally:
+        oracle.set_threads(0)
+    return dict(value=tests / (tc + ts), unit="CI tests/s", cores=cores, kind="port",
+                sample=(f"full workload once: corrcoef {tc:.3f}s + skeleton {ts:.3f}s, "
                         f"{tests} tests (FP64 oracle, OpenMP)"),
-                seconds=t2 - t0, tests=tests)
+                seconds=tc + ts, tests=tests, skeleton_only=tests / ts,
+                single_thread={"value": tests / (tc1 + ts1), "skeleton_only": tests / ts1,
+                               "corrcoef_s": tc1, "skeleton_s": ts1})
 
 
 def roofline(timings, profile_steps, stats, shapes, workload):
@@ -184,8 +197,16 @@ def bench_discrete(args):
             reps += 1
         cs_ = (time.perf_counter() - t0) / reps
         tests = int(ref["executed"].sum())
+        oracle.set_threads(1)
+        try:
+            t1 = time.perf_counter()
+            oracle.disc_skeleton(colmajor, dims, alpha, L)
+            cs1 = time.perf_counter() - t1
+        finally:
+            oracle.set_threads(0)
         cpu = dict(value=tests / cs_, unit="CI tests/s", cores=oracle.num_threads(), kind="port",
-                   sample=f"full workload, {reps} runs of {cs_:.3f}s, {tests} tests each (C oracle, OpenMP)")
+                   sample=f"full workload, {reps} runs of {cs_:.3f}s, {tests} tests each (C oracle, OpenMP)",
+                   single_thread={"value": tests / cs1, "seconds": cs1})
 
     out = {
         "metric": METRIC,

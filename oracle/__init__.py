@@ -67,6 +67,8 @@ def lib():
         L.or_disc_skeleton.argtypes = [_P, _i64, ct.c_int, _P, ct.c_double, ct.c_int, _P, _P, _P,
                                        _P, _P]
         L.or_num_threads.restype = ct.c_int
+        L.or_set_threads.restype = None
+        L.or_set_threads.argtypes = [ct.c_int]
         _lib = L
     return _lib
 
@@ -208,6 +210,11 @@ def disc_skeleton(data_colmajor: np.ndarray, dims: np.ndarray, alpha: float, max
 
 def num_threads() -> int:
     return int(lib().or_num_threads())
+
+
+def set_threads(k: int) -> None:
+    """Threads of the following oracle calls (for timing); 0 restores the default."""
+    lib().or_set_threads(int(k))
 
 
 # --------------------------------------------------------------------------

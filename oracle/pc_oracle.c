@@ -677,3 +677,14 @@ OR_EXPORT int or_num_threads(void) {
     return 1;
 #endif
 }
+
+/* threads of the following calls (timing runs); 0 restores OpenMP's default */
+OR_EXPORT void or_set_threads(int k) {
+#ifdef _OPENMP
+    static int dflt = 0;
+    if (!dflt) dflt = omp_get_max_threads();
+    omp_set_num_threads(k > 0 ? k : dflt);
+#else
+    (void)k;
+#endif
+}
