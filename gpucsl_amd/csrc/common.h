@@ -464,16 +464,15 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a -
 // Compute units of the current device, read once per process (256 when no
 // device answers, e.g. on a host without a GPU, where only sizes are asked).
 inline int device_cus() {
-    static int cus = 0;
-    if (cus == 0) {
+    static const int cus = [] {
         int dev = 0, nn = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&nn, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nn <= 0) {
             (void)hipGetLastError();
             nn = 256;
         }
-        cus = nn;
-    }
+        return nn;
+    }();
     return cus;
 }
 

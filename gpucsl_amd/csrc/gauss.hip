@@ -26,6 +26,8 @@
 // up to it.  No host sync happens between levels.
 #include "job.h"
 
+#include <mutex>
+
 namespace gpucsl {
 
 constexpr float EPS32 = 5.9604645e-8f;  // 2^-24
@@ -75,33 +77,41 @@ __host__ __device__ __forceinline__ uint64_t binom_level(int m, int level) {
 
 // per-level launch shapes of the FAST kernel (settable at run time, e.g. from
 // an autotuning table); index 0 is unused
-static gpucsl_level_tuning g_tuning[GPUCSL_MAX_LEVEL_TEMPLATE + 1];
-static bool g_tuning_init = false;
-
-static void tuning_defaults() {
-    if (g_tuning_init) return;
-    for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = gpucsl_level_tuning{8, 8, 4, 32768, 4096, 0};
-    g_tuning_init = true;
+// (process-wide, defaults on first use; reads and writes under one mutex)
+struct TuningTable {
+    gpucsl_level_tuning lv[GPUCSL_MAX_LEVEL_TEMPLATE + 1];
+};
+static TuningTable &tuning_table() {
+    static TuningTable t = [] {
+        TuningTable x;
+        for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) x.lv[l] = gpucsl_level_tuning{8, 8, 4, 32768, 4096, 0};
+        return x;
+    }();
+    return t;
+}
+static std::mutex g_tuning_mu;
+static gpucsl_level_tuning tuning_of(int level) {
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    return tuning_table().lv[level];
 }
 
 static Team team_for(int level, int gs_min) {
-    tuning_defaults();
-    const gpucsl_level_tuning &t = g_tuning[level];
+    const gpucsl_level_tuning t = tuning_of(level);
     if (t.wave_items) return Team{gs_min, gs_min * t.small_team_iters, 0x7fffffff, t.item_tests, 1};
     return Team{gs_min, gs_min * t.small_team_iters, 64 * t.wave_team_iters, t.item_tests, 0};
 }
 
 int set_level_tuning(int level, const gpucsl_level_tuning *t) {
-    tuning_defaults();
     if (!t) return set_error(ERR_ARG, "tuning: null");
     if (t->small_team_iters < 1 || t->wave_team_iters < 1 || t->fast_blocks_per_cu < 1 || t->fast_blocks_per_cu > 32 ||
         t->fast_lds_bytes < 4096 || t->fast_lds_bytes > FB_MAX_LDS_BYTES || t->fast_lds_bytes % 1024 ||
         t->item_tests < 64 || t->item_tests > (1 << 24) || t->wave_items < 0 || t->wave_items > 1)
         return set_error(ERR_ARG, "tuning: a field is out of range");
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
     if (level == 0) {
-        for (int l = 1; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) g_tuning[l] = *t;
+        for (int l = 1; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) tuning_table().lv[l] = *t;
     } else if (level >= 1 && level <= GPUCSL_MAX_LEVEL_TEMPLATE) {
-        g_tuning[level] = *t;
+        tuning_table().lv[level] = *t;
     } else {
         return set_error(ERR_ARG, "tuning: level %d outside 0..%d", level, GPUCSL_MAX_LEVEL_TEMPLATE);
     }
@@ -109,9 +119,8 @@ int set_level_tuning(int level, const gpucsl_level_tuning *t) {
 }
 
 int get_level_tuning(int level, gpucsl_level_tuning *t) {
-    tuning_defaults();
     if (!t || level < 1 || level > GPUCSL_MAX_LEVEL_TEMPLATE) return set_error(ERR_ARG, "tuning: bad level or null");
-    *t = g_tuning[level];
+    *t = tuning_of(level);
     return OK;
 }
 
@@ -1492,8 +1501,7 @@ __global__ void to_u16_kernel(const uint8_t *__restrict__ g, size_t count, uint1
 // ----------------------------------------------------------------------------
 static int slow_grid() { return device_cus() / 2; }
 static int fast_grid(int level) {
-    tuning_defaults();
-    return device_cus() * g_tuning[level].fast_blocks_per_cu;
+    return device_cus() * tuning_of(level).fast_blocks_per_cu;
 }
 
 size_t gauss_batch_ws_bytes(const Job *jobs_host, int nj) {
@@ -1568,8 +1576,9 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
     const WorkItem *items = side == 0 ? w.itemsA : w.itemsB;
     SlowItem *queue = side == 0 ? w.queueA : w.queueB;
     unsigned int *qcount = w.counters + CNT_QUEUE + side;
-    const int lds = g_tuning[L].fast_lds_bytes;
-    if (g_tuning[L].wave_items)
+    const gpucsl_level_tuning tl = tuning_of(L);
+    const int lds = tl.fast_lds_bytes;
+    if (tl.wave_items)
         GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_wave_kernel<L>, dim3(fast_grid(L)), dim3(256), 0, s, jd, nj,
                       base, items, side, team_for(L, gs_min), margin_k, queue, qcount, w.bstats);
     else

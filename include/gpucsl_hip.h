@@ -182,7 +182,8 @@ int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss
  * size the grid and its LDS staging; a work item holds about item_tests
  * (64..2^24) subset tests; wave_items = 1 lets every wave take whole items on
  * its own (64 lanes per pair, reads from L2, no LDS staging).  Results do not
- * depend on these values.  level 0 sets every level.  Defaults:
+ * depend on these values.  level 0 sets every level.  The table is
+ * process-wide; launches read it at launch time, so set it before they run.  Defaults:
  * {8, 8, 4, 32768, 4096, 0}. */
 typedef struct gpucsl_level_tuning {
     int32_t small_team_iters;
