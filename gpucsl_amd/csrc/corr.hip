@@ -55,14 +55,14 @@ __device__ __forceinline__ void colsum_body(const Job &J) {
     const int64_t r1 = r0 + per < n ? r0 + per : n;
     double acc = 0.0;
     if (a < p) {
+        // rows g, g+4, g+8, ... added in that order; eight loads in flight
         int64_t k = r0 + g;
-        for (; k + 12 < r1; k += 16) {
-            const double x0 = (double)X[k * p + a], x1 = (double)X[(k + 4) * p + a];
-            const double x2 = (double)X[(k + 8) * p + a], x3 = (double)X[(k + 12) * p + a];
-            acc += x0;
-            acc += x1;
-            acc += x2;
-            acc += x3;
+        for (; k + 28 < r1; k += 32) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = (double)X[(k + 4 * u) * p + a];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += x[u];
         }
         for (; k < r1; k += 4) acc += (double)X[k * p + a];
     }
