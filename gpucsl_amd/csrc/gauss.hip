@@ -681,6 +681,7 @@ struct ItemCtx {
     const float *sf;   // LDS
     const uint64_t *tab;  // LDS binomial table C(m, k), k = 1..L, m = 0..Nn; null: computed
     float *prow;          // LDS, ROW mode: C[j, adj(i)] of the pair each team works on (d floats per team)
+    int prow_slot0;       // team slot (threadIdx.x / team) of prow's first row
 };
 
 template <int L, int MODE>
@@ -908,7 +909,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
         if (c.side == 1 && Gm[(size_t)c.i * c.p + j] != 1) continue;  // removed, or pending in the SLOW kernel
-        float *myrow = c.prow + (size_t)(threadIdx.x / team) * c.d;
+        float *myrow = c.prow + (size_t)(threadIdx.x / team - c.prow_slot0) * c.d;
         if constexpr (MODE == 1) {
             // the team's lanes stage C[j, adj(i)]; LDS accesses of one wave
             // are ordered, the fences keep the compiler from moving them
@@ -1010,6 +1011,7 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         c.snb = snb;
         c.sf = sf;
         c.prow = sf + 2 * d;
+        c.prow_slot0 = 0;
         // binomial table after the staged data, when it fits
         const int tab_off = ((d + (mode == 0 ? D * D : (2 + teams) * d)) + 1) & ~1;
         const bool use_tab = L >= 2 && mode != 2 && c.Nn >= L && tab_off + 2 * L * (c.Nn + 1) <= WORDS;
@@ -1119,13 +1121,16 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
 
 // Level l >= 1, FAST phase, one wave per work item: every wave of a block
 // takes whole items on its own and its teams (gs_min lanes, or 64) pull the
-// item's pairs, reading C from global memory (L2); no LDS staging and no block
-// barriers, so a wave never waits for another.
+// item's pairs; no block barriers, so a wave never waits for another.  A wave
+// stages the row in its own LDS region of WAVE_WORDS words when it fits (ROW
+// mode: neighbour ids, C[i, adj(i)], the diagonal, and one pair row per team),
+// and otherwise reads C from global memory (L2).
 template <int L>
 __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
     const Job *__restrict__ jobs, int nj, const int32_t *__restrict__ itembase, const WorkItem *__restrict__ items,
-    int side, Team tm, float margin_k, SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount,
-    unsigned long long *__restrict__ bstats) {
+    int side, Team tm, int WAVE_WORDS, float margin_k, SlowItem *__restrict__ queue,
+    unsigned int *__restrict__ qcount, unsigned long long *__restrict__ bstats) {
+    extern __shared__ int lds_words[];
     __shared__ int s_next[4];
     __shared__ unsigned long long s_e[4], s_u[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1160,10 +1165,39 @@ __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
         c.sf = nullptr;
         c.tab = nullptr;
         c.prow = nullptr;
-        if (lane == 0) s_next[wave] = c.q0;
+        c.prow_slot0 = 0;
+        const int team = team_lanes(c.count, tm);
+        const int teams_w = 64 / team;
+        const int d = c.d;
+        // the previous item's LDS reads and s_next are done before they change
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        team_pairs<L, 2>(c, team_lanes(c.count, tm), &s_next[wave], queue, qcount, n_exec, n_useful);
+        if (lane == 0) s_next[wave] = c.q0;
+        if ((3 + teams_w) * d <= WAVE_WORDS) {
+            int *snb = lds_words + (size_t)wave * WAVE_WORDS;
+            float *sf = reinterpret_cast<float *>(snb + d);
+            for (int k = lane; k < d; k += 64) snb[k] = ldg(c.gnb + k);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int k = lane; k < d; k += 64) {
+                const int v = snb[k];
+                sf[k] = ldg(c.C32 + (size_t)c.i * c.p + v);
+                sf[d + k] = ldg(c.C32 + (size_t)v * c.p + v);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            c.snb = snb;
+            c.sf = sf;
+            c.prow = sf + 2 * d;
+            c.prow_slot0 = wave * teams_w;
+            team_pairs<L, 1>(c, team, &s_next[wave], queue, qcount, n_exec, n_useful);
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            team_pairs<L, 2>(c, team, &s_next[wave], queue, qcount, n_exec, n_useful);
+        }
         __builtin_amdgcn_wave_barrier();  // every team is done with s_next before lane 0 resets it
     }
     for (int off = 32; off >= 1; off >>= 1) {
@@ -1579,8 +1613,8 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
     const gpucsl_level_tuning tl = tuning_of(L);
     const int lds = tl.fast_lds_bytes;
     if (tl.wave_items)
-        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_wave_kernel<L>, dim3(fast_grid(L)), dim3(256), 0, s, jd, nj,
-                      base, items, side, team_for(L, gs_min), margin_k, queue, qcount, w.bstats);
+        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_wave_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj,
+                      base, items, side, team_for(L, gs_min), lds / 16, margin_k, queue, qcount, w.bstats);
     else
         GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj,
                       base, items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
