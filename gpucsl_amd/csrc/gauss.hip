@@ -1619,7 +1619,7 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
         GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj,
                       base, items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
     if (side == 0) return;
-    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 16>), dim3(slow_grid()), dim3(256), 0, s, jd, margin_k,
+    GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 64>), dim3(slow_grid()), dim3(256), 0, s, jd, margin_k,
                   w.counters + STRIPES, w.queueA, w.counters + CNT_QUEUE, w.queueB, w.counters + CNT_QUEUE + 1,
                   w.bstats, w.counters + CNT_STATS_DONE, st);
 }
