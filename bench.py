@@ -24,6 +24,9 @@ sys.path.insert(0, str(ROOT))
 METRIC = "CI tests/s + skeleton time, Gaussian p=1000 n=10k; p50/p99 request latency"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_INT8_TOPS = 5000.0
+GRAM_SMALL_P = 256      # float32 datasets above this take the integer Gram path
+I8_DIGIT_PAIRS = 26     # digit-plane products of that path (pairs a + b <= 6 of six digits)
 WORKLOADS = {
     # name: (p, n, degree, seed, alpha, max_level, dtype)
     "gaussian_p1000_n10000": (1000, 10000, 4.0, 100, 0.01, 4, np.float32),
@@ -88,16 +91,27 @@ def cpu_baseline(X: np.ndarray, n: int, alpha: float, max_level: int):
                                "corrcoef_s": tc1, "skeleton_s": ts1})
 
 
+def int8_gram(n: int, p: int, f32: bool) -> bool:
+    return f32 and p > GRAM_SMALL_P
+
+
 def roofline(timings, profile_steps, stats, shapes, workload):
     """Roofline object for the kernel with the most time per step.  shapes:
-    the (n, p) of every dataset one launch covers."""
+    the (n, p, float32?) of every dataset one launch covers."""
     dominant = max(timings, key=lambda k: timings[k][1])
     cnt, tot, _, _ = timings[dominant]
     avg_s = tot / cnt / 1e3
     launches_per_step = cnt / profile_steps
-    if dominant.startswith("corr_gram"):
+    if dominant == "corr_gram_i8":
+        # p(p+1)/2 distinct entries x n rows x 26 digit products x 2 int8 ops
+        ops = sum(I8_DIGIT_PAIRS * n * p * (p + 1.0) for n, p, f in shapes if int8_gram(n, p, f)) / launches_per_step
+        achieved = ops / avg_s / 1e12
+        fp64_eq = sum(1.0 * n * p * (p + 1) for n, p, f in shapes if int8_gram(n, p, f)) / launches_per_step
+        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_INT8_TOPS, unit="TOP/s (int8)",
+                    frac=achieved / PEAK_INT8_TOPS, fp64_equivalent_tflops=fp64_eq / avg_s / 1e12)
+    elif dominant.startswith("corr_gram"):
         # symmetric rank-n update: n * p * (p + 1) flops for the p(p+1)/2 distinct entries
-        flops = sum(1.0 * n * p * (p + 1) for n, p in shapes) / launches_per_step
+        flops = sum(1.0 * n * p * (p + 1) for n, p, f in shapes if not int8_gram(n, p, f)) / launches_per_step
         achieved = flops / avg_s / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS)
@@ -272,13 +286,13 @@ def bench_batched(args):
         j = GaussJobState(5000, 200, L, alpha=alpha)
         j.set_data(base_dev[torch.from_numpy(r).cuda()])
         jobs.append(j)
-        shapes.append((5000, 200))
+        shapes.append((5000, 200, True))
     for X in sets:
         n, p = X.shape
         j = GaussJobState(n, p, L, alpha=alpha)
         j.set_data(X)
         jobs.append(j)
-        shapes.append((n, p))
+        shapes.append((n, p, True))
     torch.cuda.synchronize()
     hbm = sum(j.nbytes for j in jobs)
     log(f"[bench] {len(jobs)} jobs, {hbm / 2**30:.2f} GiB of job state")
@@ -623,7 +637,7 @@ def main():
         torch.cuda.synchronize()
     timings = kt.timings
     per_step_kernel_ms = {k: v[1] / args.profile_steps for k, v in timings.items()}
-    roof = roofline(timings, args.profile_steps, stats, [(n, p)], args.workload)
+    roof = roofline(timings, args.profile_steps, stats, [(n, p, dtype == np.float32)], args.workload)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -646,7 +660,8 @@ def main():
         "config": {"workload": f"{args.workload}: ER-DAG linear SEM degree {degree}, seed {seed}, "
                                f"{np.dtype(dtype).name} data, alpha={alpha}, max_level={L}",
                    "p": p, "n": n, "max_level": L, "alpha": alpha,
-                   "correlation": "FP64 MFMA GEMM (C32 rounded from C64)",
+                   "correlation": ("exact int8-digit Gram matrix on MFMA, FP64 accuracy" if int8_gram(n, p, dtype == np.float32)
+                                   else "FP64 MFMA Gram matrix") + " (C32 rounded from C64)",
                    "ci_tests": "FP32 with FP64 recheck near the threshold",
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -33,8 +33,8 @@ SIGNATURES = {
     "gpucsl_gauss_init": (_I, [_I, _I, _P, _P, _P, _P, _P]),
     "gpucsl_gauss_level0": (_I, [_P, _P, _I, _D, _F, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_gauss_job_workspace_bytes": (_SZ, [_I]),
-    "gpucsl_corr_k_splits": (_I, [_I64, _I]),
-    "gpucsl_corr_job_workspace_bytes": (_SZ, [_I, _I]),
+    "gpucsl_corr_k_splits": (_I, [_I64, _I, _I]),
+    "gpucsl_corr_job_workspace_bytes": (_SZ, [_I64, _I, _I]),
     "gpucsl_gauss_batch_workspace_bytes": (_SZ, [_P, _I]),
     "gpucsl_gauss_batch_corr": (_I, [_P, _P, _I, _P]),
     "gpucsl_gauss_batch_init": (_I, [_P, _P, _I, _P]),

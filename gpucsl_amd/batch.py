@@ -76,7 +76,7 @@ def job_layout(n: int, p: int, max_level: int, x_itemsize: int = 4, k_splits: in
     + 2 skeleton) + 4 p^2 L (sepsets) + level and correlation workspaces."""
     lib = nat.load()
     if k_splits is None:
-        k_splits = int(lib.gpucsl_corr_k_splits(n, p))
+        k_splits = int(lib.gpucsl_corr_k_splits(n, p, 0 if x_itemsize == 4 else 1))
     L = max(max_level, 1)
     pp = p * p
     sizes = [
@@ -84,7 +84,7 @@ def job_layout(n: int, p: int, max_level: int, x_itemsize: int = 4, k_splits: in
         ("C32", 4 * pp), ("C64", 8 * pp), ("G", pp), ("sepsets", 4 * pp * L),
         ("zmin", 4 * pp), ("pmax", 4 * pp), ("skeleton", 2 * pp),
         ("ws", int(lib.gpucsl_gauss_job_workspace_bytes(p))),
-        ("corr_ws", int(lib.gpucsl_corr_job_workspace_bytes(p, k_splits))),
+        ("corr_ws", int(lib.gpucsl_corr_job_workspace_bytes(n, p, k_splits))),
     ]
     off, out = 0, {}
     for name, sz in sizes:
@@ -117,7 +117,7 @@ class GaussJobState:
         # job's correlation matrix (and everything after it) is the same bits
         # whatever batch it runs in, and the same as the single-dataset path's
         if k_splits is None:
-            k_splits = int(nat.load().gpucsl_corr_k_splits(self.n, self.p))
+            k_splits = int(nat.load().gpucsl_corr_k_splits(self.n, self.p, 0 if x_dtype == "float32" else 1))
         self.k_splits = max(1, int(k_splits))
         # pcalg conventions: thresholds sqrt(n - l - 3), pMax over every test,
         # one-way sepsets (GPUCSL_JOB_PCALG_COMPAT)

@@ -72,7 +72,7 @@ void record_stop(const char *name, hipStream_t s) {
 
 // defined in corr.hip / gauss.hip
 size_t corr_workspace_bytes(int64_t n, int p);
-int corr_k_splits(int64_t n, int p);
+int corr_k_splits(int64_t n, int p, int dtype);
 int corr_launch(const void *X, int dtype, int64_t n, int p, double *C64, float *C32, void *ws,
                 size_t ws_bytes, hipStream_t s);
 int c64_to_c32_launch(const double *C64, int p, float *C32, hipStream_t s);
@@ -217,10 +217,12 @@ size_t gpucsl_gauss_workspace_bytes(int p, int max_level) {
 
 size_t gpucsl_gauss_job_workspace_bytes(int p) { return p < 1 ? 0 : job_ws_bytes(p); }
 
-int gpucsl_corr_k_splits(int64_t n, int p) { return corr_k_splits(n, p); }
+int gpucsl_corr_k_splits(int64_t n, int p, int dtype) {
+    return n < 2 || p < 1 || (dtype != 0 && dtype != 1) ? 0 : corr_k_splits(n, p, dtype);
+}
 
-size_t gpucsl_corr_job_workspace_bytes(int p, int k_splits) {
-    return p < 1 || k_splits < 1 ? 0 : corr_job_ws_bytes(p, k_splits);
+size_t gpucsl_corr_job_workspace_bytes(int64_t n, int p, int k_splits) {
+    return n < 2 || p < 1 || k_splits < 1 ? 0 : corr_job_ws_bytes(n, p, k_splits);
 }
 
 int gpucsl_gauss_init(int p, int max_level, uint8_t *G, int32_t *sepsets, float *zmin,

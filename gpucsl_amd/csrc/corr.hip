@@ -1,11 +1,20 @@
-// Correlation matrix on MFMA: C = corrcoef(X) for X (n x p, row-major), for a
-// table of jobs (blockIdx.z selects the job).
+// Correlation matrix on the matrix cores: C = corrcoef(X) for X (n x p,
+// row-major), for a table of jobs (blockIdx.z selects the job).
 //
-// Replaces gpucsl/pc/helpers.py:58-59 (np.corrcoef on the host).  The Gram
-// matrix of the centred data is computed with v_mfma_f64_16x16x4_f64 in FP64 so
-// that C64 serves the FP64 recheck and C32 (correctly rounded from C64) feeds the
-// FP32 CI tests.  Split-K partial Gram matrices are summed in a fixed order, so
-// results do not depend on scheduling.
+// Replaces gpucsl/pc/helpers.py:58-59 (np.corrcoef on the host).  C64 serves
+// the FP64 recheck and C32 (correctly rounded from C64) feeds the FP32 CI
+// tests, so the Gram matrix of the centred data is formed to FP64 accuracy:
+//  - float32 data: each centred value, scaled by a power of two per column, is
+//    rounded to a 42-bit integer and split into six signed base-128 digits;
+//    the Gram matrix is the sum of the 26 digit-plane products whose weight is
+//    at least 128^-6 of the leading one, on v_mfma_i32_16x16x64_i8 with exact
+//    int32 accumulation (the splitting idea of the Ozaki scheme, Ozaki, Ogita,
+//    Oishi and Rump, Numer. Algorithms 59, 2012, on integer units as in
+//    Ootomo, Ozaki and Yokota, "DGEMM on integer matrix multiplication unit",
+//    IJHPCA 2024; here with one fixed-point scale per column);
+//  - float64 data: v_mfma_f64_16x16x4_f64 on the centred values.
+// Split-K partial Gram matrices are summed in a fixed order, so results do not
+// depend on scheduling.
 #include "job.h"
 
 namespace gpucsl {
@@ -28,6 +37,9 @@ __host__ __device__ inline int upper_tiles(int p, int ct = CT) {
 // k-slabs in the same order either way, so the bits do not depend on the tile.
 constexpr int GRAM_SMALL_P = 256;
 __host__ __device__ inline int gram_tile(int p) { return p <= GRAM_SMALL_P ? 32 : CT; }
+// Float32 datasets above GRAM_SMALL_P take the integer Gram path; the others
+// (and all float64 datasets) the FP64 one.
+__host__ __device__ inline bool gram_int8(int dtype, int p) { return dtype == 0 && p > GRAM_SMALL_P; }
 
 // Row splits of the column sums: a function of the job's own n, so a job's
 // means are the same bits alone and in any batch.
@@ -37,9 +49,9 @@ __host__ __device__ inline int corr_row_splits(int64_t n) {
     return rs < 1 ? 1 : (int)rs;
 }
 
-// partial[s][a] = sum of X[k][a] over the rows k of split s: 64 columns x 4
-// row groups per block (group g takes rows g, g+4, ... of the split), the four
-// group sums added in order.
+// partial[s][a] = sum of X[k][a] over the rows k of split s (cmin, cmax: the
+// extremes over the same rows): 64 columns x 4 row groups per block (group g
+// takes rows g, g+4, ... of the split), the four group sums added in order.
 template <typename T>
 __device__ __forceinline__ void colsum_body(const Job &J) {
     const int p = J.p;
@@ -53,7 +65,7 @@ __device__ __forceinline__ void colsum_body(const Job &J) {
     const int64_t per = (n + rs - 1) / rs;
     const int64_t r0 = s * per;
     const int64_t r1 = r0 + per < n ? r0 + per : n;
-    double acc = 0.0;
+    double acc = 0.0, mn = INFINITY, mx = -INFINITY;
     if (a < p) {
         // rows g, g+4, g+8, ... added in that order; eight loads in flight
         int64_t k = r0 + g;
@@ -62,15 +74,49 @@ __device__ __forceinline__ void colsum_body(const Job &J) {
 #pragma unroll
             for (int u = 0; u < 8; ++u) x[u] = (double)X[(k + 4 * u) * p + a];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc += x[u];
+            for (int u = 0; u < 8; ++u) {
+                acc += x[u];
+                mn = fmin(mn, x[u]);
+                mx = fmax(mx, x[u]);
+            }
         }
-        for (; k < r1; k += 4) acc += (double)X[k * p + a];
+        for (; k < r1; k += 4) {
+            const double x = (double)X[k * p + a];
+            acc += x;
+            mn = fmin(mn, x);
+            mx = fmax(mx, x);
+        }
     }
-    __shared__ double part[4][64];
-    part[g][c] = acc;
+    __shared__ double part[3][4][64];
+    part[0][g][c] = acc;
+    part[1][g][c] = mn;
+    part[2][g][c] = mx;
     __syncthreads();
-    if (g == 0 && a < p)
-        carve_corr(J.corr_ws, p).partial[(size_t)s * p + a] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+    if (g == 0 && a < p) {
+        const CorrWs cw = carve_corr(J);
+        const size_t at = (size_t)s * p + a;
+        cw.partial[at] = ((part[0][0][c] + part[0][1][c]) + part[0][2][c]) + part[0][3][c];
+        cw.cmin[at] = fmin(fmin(part[1][0][c], part[1][1][c]), fmin(part[1][2][c], part[1][3][c]));
+        cw.cmax[at] = fmax(fmax(part[2][0][c], part[2][1][c]), fmax(part[2][2][c], part[2][3][c]));
+    }
+}
+
+// mean of column col: the row-split partial sums added in split order, / n
+__device__ __forceinline__ double column_mean(const CorrWs &cw, int64_t n, int p, int col) {
+    const int rs = corr_row_splits(n);
+    const double *pc = cw.partial + col;
+    double m = 0.0;
+    int q = 0;
+    for (; q + 4 <= rs; q += 4) {
+        const double x0 = pc[(size_t)q * p], x1 = pc[(size_t)(q + 1) * p];
+        const double x2 = pc[(size_t)(q + 2) * p], x3 = pc[(size_t)(q + 3) * p];
+        m += x0;
+        m += x1;
+        m += x2;
+        m += x3;
+    }
+    for (; q < rs; ++q) m += pc[(size_t)q * p];
+    return m / (double)n;
 }
 
 template <typename T>
@@ -102,9 +148,11 @@ __global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const J
     const Job &J = jobs[blockIdx.z];
     const int p = J.p;
     const int64_t n = J.n;
-    if (gram_tile(p) != CTT || (int)blockIdx.x >= upper_tiles(p, CTT) || (int)blockIdx.y >= J.k_splits) return;
+    if (gram_int8(J.x_dtype, p) || gram_tile(p) != CTT || (int)blockIdx.x >= upper_tiles(p, CTT) ||
+        (int)blockIdx.y >= J.k_splits)
+        return;
     const T *__restrict__ X = (const T *)J.X;
-    CorrWs cw = carve_corr(J.corr_ws, p);
+    CorrWs cw = carve_corr(J);
 
     // upper-triangle tile id -> (ta, tb), ta <= tb, row-major over the triangle
     const int tiles = (p + CTT - 1) / CTT;
@@ -141,23 +189,7 @@ __global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const J
     __shared__ double s_mean[2][CTT];
     if (tid < 2 * CTT) {
         const int col = (tid < CTT ? a0 : b0) + (tid & (CTT - 1));
-        double m = 0.0;
-        if (col < p) {
-            const int rs = corr_row_splits(n);
-            const double *pc = cw.partial + col;
-            int q = 0;
-            for (; q + 4 <= rs; q += 4) {
-                const double x0 = pc[(size_t)q * p], x1 = pc[(size_t)(q + 1) * p];
-                const double x2 = pc[(size_t)(q + 2) * p], x3 = pc[(size_t)(q + 3) * p];
-                m += x0;
-                m += x1;
-                m += x2;
-                m += x3;
-            }
-            for (; q < rs; ++q) m += pc[(size_t)q * p];
-            m = m / (double)n;
-        }
-        s_mean[tid / CTT][tid & (CTT - 1)] = m;
+        s_mean[tid / CTT][tid & (CTT - 1)] = col < p ? column_mean(cw, n, p, col) : 0.0;
     }
     __syncthreads();
     double ma[2], mb[2];
@@ -237,6 +269,283 @@ __global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const J
         }
 }
 
+// ---- integer Gram path (float32 data) --------------------------------------
+// Digits of the centred column a: z = fl(x - mean_a) is scaled by 2^(41 - e_a),
+// where 2^e_a exceeds max |z| over the column (from the column extremes:
+// rounding is monotone, so |fl(x - mean)| <= max(fl(max - mean), fl(mean - min))),
+// rounded to an integer Z with |Z| <= 2^41, and written as
+// Z = sum_a D_a 128^(5-a) with D_5..D_1 in [-64, 63] and |D_0| <= 64.  The Gram matrix of the Z is
+// then sum over digit pairs (a, b) of D_a^T D_b 128^(10-a-b); the pairs with
+// a + b <= 6 are kept.  A dropped pair weighs at most 128^-7 of the leading
+// one, and the only dropped pairs that do not average out over the rows, the
+// squares D_a^T D_a of a diagonal entry, at most 128^-8; keeping a + b <= 5
+// instead leaves the square of D_3 out of G_aa, a bias of about 2^-42 n that
+// shows as 1e-12 in C.  With a + b <= 6, C agrees with numpy's FP64
+// corrcoef to a few 1e-15, as does the FP64 Gram matrix.  The per-column scale
+// 2^(41 - e_a) enters G_ab as a factor 2^(41-e_a) 2^(41-e_b) and cancels
+// exactly in G_ab / sqrt(G_aa G_bb), so the planes carry no scale.  A column
+// whose mean or extremes are not finite (NaN or Inf in the data) gets zero
+// digits and hence NaN correlations, as with the FP64 arithmetic.
+//
+// Block: 64 columns x I8_SLICE_STEPS steps of 64 rows; 256 threads.  Thread
+// (column c = tid % 64, quarter q = tid / 64) converts rows 16 q .. 16 q + 15 of
+// its column in each step and stores them as bytes 64 c + 16 q .. + 15 of the
+// step's tile of every digit.  The column centre and scale come from the row
+// splits' sums and extremes, the splits taken four at a time by the four waves
+// and their partial sums added in wave order.
+constexpr int I8_SLICE_STEPS = 4;
+__global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ jobs) {
+    const Job &J = jobs[blockIdx.z];
+    if (!gram_int8(J.x_dtype, J.p)) return;
+    const int p = J.p;
+    const int64_t n = J.n;
+    const int64_t steps = i8_rows(n) / 64;
+    const int CB = i8_cols(p) / 64;
+    const int cb = blockIdx.y;
+    const int64_t st0 = (int64_t)blockIdx.x * I8_SLICE_STEPS;
+    if (st0 >= steps || cb >= CB) return;
+    const CorrWs cw = carve_corr(J);
+    const int tid = threadIdx.x;
+    const int c = tid & 63, q = tid >> 6;
+    const int col = cb * 64 + c;
+    __shared__ double s_part[3][4][64];
+    __shared__ double s_mu[64], s_sc[64];
+    {
+        // splits q, q + 4, q + 8, ... in that order, eight loads of each kind in flight
+        const int rs = corr_row_splits(n);
+        double sm = 0.0, mn = INFINITY, mx = -INFINITY;
+        if (col < p)
+            for (int r0 = q; r0 < rs; r0 += 32) {
+                double vs[8], vn[8], vx[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int r = r0 + 4 * u;
+                    const size_t at = (size_t)(r < rs ? r : q) * p + col;
+                    vs[u] = cw.partial[at];
+                    vn[u] = cw.cmin[at];
+                    vx[u] = cw.cmax[at];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (r0 + 4 * u < rs) {
+                        sm += vs[u];
+                        mn = fmin(mn, vn[u]);
+                        mx = fmax(mx, vx[u]);
+                    }
+            }
+        s_part[0][q][c] = sm;
+        s_part[1][q][c] = mn;
+        s_part[2][q][c] = mx;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const double mu = (((s_part[0][0][c] + s_part[0][1][c]) + s_part[0][2][c]) + s_part[0][3][c]) / (double)n;
+        const double mn = fmin(fmin(s_part[1][0][c], s_part[1][1][c]), fmin(s_part[1][2][c], s_part[1][3][c]));
+        const double mx = fmax(fmax(s_part[2][0][c], s_part[2][1][c]), fmax(s_part[2][2][c], s_part[2][3][c]));
+        const double bound = fmax(mx - mu, mu - mn);
+        double sc = 0.0;
+        if (col < p && isfinite(mu) && isfinite(bound) && bound > 0.0) {
+            int e;
+            frexp(bound, &e);
+            sc = ldexp(1.0, 41 - e);
+        }
+        s_mu[c] = mu;
+        s_sc[c] = sc;
+    }
+    __syncthreads();
+    const float *X = (const float *)J.X;
+    const double mu = s_mu[c], sc = s_sc[c];
+    const bool live = sc != 0.0;
+    const int64_t st1 = st0 + I8_SLICE_STEPS < steps ? st0 + I8_SLICE_STEPS : steps;
+    // rows of the next step are loaded while the current step is converted
+    float xn[16];
+    auto load = [&](int64_t st) {
+        const int64_t k0 = st * 64 + 16 * q;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) xn[j] = live && k0 + j < n ? X[(k0 + j) * p + col] : 0.0f;
+    };
+    load(st0);
+    for (int64_t st = st0; st < st1; ++st) {
+        const int64_t k0 = st * 64 + 16 * q;
+        float x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = xn[j];
+        if (st + 1 < st1) load(st + 1);
+        // Z' = Z + B with B = 64 (128^5 - 1) / 127: the five lower 7-bit fields of
+        // Z' minus 64 are the lower digits and the field above them is D_0, so
+        // no carries are needed.  Z' = zh 2^21 + zl with 0 <= zl < 2^21, both
+        // exact in FP64.  Four rows' fields go into one word per digit and the
+        // "- 64" is applied to the four bytes at once.
+        int w[I8_PLANES][4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            int zh[4], zl[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int j = 4 * g + e;
+                const double z = live && k0 + j < n ? rint(((double)x[j] - mu) * sc) + 17315143744.0 : 17315143744.0;
+                const double hi = floor(z * (1.0 / 2097152.0));
+                zh[e] = (int)hi;
+                zl[e] = (int)(z - hi * 2097152.0);
+            }
+            auto pack = [&](const int (&v)[4], int shift, int width) {
+                const unsigned m = (1u << width) - 1u;
+                return (int)((((unsigned)v[0] >> shift) & m) | ((((unsigned)v[1] >> shift) & m) << 8) |
+                             ((((unsigned)v[2] >> shift) & m) << 16) | ((((unsigned)v[3] >> shift) & m) << 24));
+            };
+            auto rebias = [](int v) {
+                return (int)((((unsigned)v | 0x80808080u) - 0x40404040u) ^ 0x80808080u);
+            };
+            w[0][g] = pack(zh, 14, 8);
+            w[1][g] = rebias(pack(zh, 7, 7));
+            w[2][g] = rebias(pack(zh, 0, 7));
+            w[3][g] = rebias(pack(zl, 14, 7));
+            w[4][g] = rebias(pack(zl, 7, 7));
+            w[5][g] = rebias(pack(zl, 0, 7));
+        }
+#pragma unroll
+        for (int a = 0; a < I8_PLANES; ++a) {
+            int4 v;
+            v.x = w[a][0];
+            v.y = w[a][1];
+            v.z = w[a][2];
+            v.w = w[a][3];
+            *(int4 *)(cw.planes + i8_tile(a, cb, st, CB, steps) + 64 * c + 16 * q) = v;
+        }
+    }
+}
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+constexpr int I8_MAXU = 6;        // digit pairs (a, b) with a + b <= I8_MAXU
+constexpr int I8_COLB = 80;        // LDS bytes per column and digit: 64 rows + 16 pad
+
+// Split-K partial Gram tile (ta <= tb) of 64 x 64 from the digit planes: the
+// 26 digit pairs (a, b) with a + b <= 6, D_a^T D_b accumulated exactly in int32
+// per weight u = a + b (at most 6 pairs x 64 rows x 64^2 per step, so a split
+// of at most I8_MAX_STEPS steps stays below 2^31), then combined as
+// sum_u T_u 128^-u in FP64 from the smallest weight up.  Block: 4 waves in 2 x 2,
+// each wave a 32 x 32 quarter (2 x 2 MFMA blocks of 16 x 16).  A 64-row step of
+// both column panels (6 digits x 64 columns x 64 bytes each) is staged in LDS;
+// the next step's panels are loaded into registers while the current one is
+// multiplied.  On a diagonal tile (ta == tb) the two panels are the same and
+// the wave below the diagonal, whose entries are never read, only loads.
+__global__ __launch_bounds__(256, 2) void gram_i8_kernel(const Job *__restrict__ jobs) {
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][I8_PLANES][64][I8_COLB];
+    const Job &J = jobs[blockIdx.z];
+    const int p = J.p;
+    const int64_t n = J.n;
+    if (!gram_int8(J.x_dtype, p) || (int)blockIdx.x >= upper_tiles(p) || (int)blockIdx.y >= J.k_splits) return;
+    const CorrWs cw = carve_corr(J);
+    const int tiles = (p + CT - 1) / CT;
+    int t = blockIdx.x, ta = 0;
+    while (t >= tiles - ta) {
+        t -= tiles - ta;
+        ++ta;
+    }
+    const int tb = ta + t;
+    const int a0 = ta * CT, b0 = tb * CT;
+    const bool diag = ta == tb;
+    const int split = blockIdx.y;
+    const int64_t npad = i8_rows(n);
+    const int ppad = i8_cols(p);
+    const int64_t steps = npad / 64;
+    const int64_t per = (steps + J.k_splits - 1) / J.k_splits;
+    const int64_t s0 = split * per;
+    const int64_t s1 = s0 + per < steps ? s0 + per : steps;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int ci = lane & 15, kq = lane >> 4;
+    const bool idle = diag && wr > wc;
+
+    // loader: thread -> bytes 16 tid .. 16 tid + 15 of each 4 KB panel tile,
+    // i.e. column tid / 4, 16-byte chunk tid % 4
+    const int CB = ppad / 64;
+    const size_t plane_stride = (size_t)CB * steps * I8_TILE;
+    const int8_t *srcA = cw.planes + i8_tile(0, ta, 0, CB, steps) + 16 * tid;
+    const int8_t *srcB = cw.planes + i8_tile(0, tb, 0, CB, steps) + 16 * tid;
+    i32x4 ra[I8_PLANES], rb[I8_PLANES];
+    auto load = [&](int64_t st) {
+        const size_t off = (size_t)st * I8_TILE;
+#pragma unroll
+        for (int a = 0; a < I8_PLANES; ++a) ra[a] = *(const i32x4 *)(srcA + a * plane_stride + off);
+        if (!diag)
+#pragma unroll
+            for (int a = 0; a < I8_PLANES; ++a) rb[a] = *(const i32x4 *)(srcB + a * plane_stride + off);
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[0][a][tid >> 2][16 * (tid & 3)] = ra[a];
+        if (!diag)
+#pragma unroll
+            for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[1][a][tid >> 2][16 * (tid & 3)] = rb[a];
+    };
+
+    i32x4 acc[I8_MAXU + 1][2][2];
+#pragma unroll
+    for (int u = 0; u <= I8_MAXU; ++u)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[u][i][j] = i32x4{0, 0, 0, 0};
+
+    const int bp = diag ? 0 : 1;
+    if (s0 < s1) {
+        load(s0);
+        store();
+    }
+    __syncthreads();
+    for (int64_t st = s0; st < s1; ++st) {
+        if (st + 1 < s1) load(st + 1);
+        if (!idle) {
+            i32x4 bf[I8_PLANES][2];
+#pragma unroll
+            for (int b = 0; b < I8_PLANES; ++b)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bf[b][j] = *(const i32x4 *)&lds[bp][b][wc + 16 * j + ci][16 * kq];
+#pragma unroll
+            for (int a = 0; a < I8_PLANES; ++a) {
+                i32x4 af[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) af[i] = *(const i32x4 *)&lds[0][a][wr + 16 * i + ci][16 * kq];
+#pragma unroll
+                for (int b = 0; b < I8_PLANES && a + b <= I8_MAXU; ++b)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+                            acc[a + b][i][j] =
+                                __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[b][j], acc[a + b][i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (st + 1 < s1) {
+            store();
+            __syncthreads();
+        }
+    }
+    if (idle) return;
+
+    // D layout of v_mfma_i32_16x16x64_i8 (checked on gfx950): accumulator item v
+    // of a lane holds column (lane & 15) and row 4 * (lane >> 4) + v.
+    double *out = cw.gram + (size_t)split * p * p;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = b0 + wc + j * 16 + ci;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = a0 + wr + i * 16 + 4 * (lane >> 4) + v;
+                double h = (double)acc[I8_MAXU][i][j][v];
+#pragma unroll
+                for (int u = I8_MAXU - 1; u >= 0; --u) h = h * (1.0 / 128.0) + (double)acc[u][i][j][v];
+                if (row < p && col < p) out[(size_t)row * p + col] = h;
+            }
+        }
+}
+
 // C64 and C32 for a quarter (16 rows x 64 columns) of one upper tile
 // (ta <= tb) of 64 x 64 and for its mirror: G_ab = sum of the split-K partial
 // Gram matrices in split order, sd[a] = sqrt(G_aa / (n-1)), and
@@ -258,7 +567,7 @@ __global__ __launch_bounds__(256) void normalize_tile_kernel(const Job *__restri
     }
     const int tb = ta + t;
     const int a0 = ta * CT + quarter * QR, b0 = tb * CT;
-    const CorrWs cw = carve_corr(J.corr_ws, p);
+    const CorrWs cw = carve_corr(J);
     const int ks = J.k_splits;
     const double nm1 = (double)(J.n - 1);
     const size_t pp = (size_t)p * p;
@@ -356,25 +665,34 @@ __global__ void c64_to_c32_kernel(const Job *__restrict__ jobs) {
 // host side
 // ----------------------------------------------------------------------------
 // Split-K factor of one dataset's Gram matrix: ks in 1..16 (at least 256 rows
-// per split) that minimises the busiest compute unit's work when the
+// per split; more when float32 data has more than 16 x 65536 rows) that minimises the busiest compute unit's work when the
 // tiles x ks blocks spread evenly over the device, ceil(tiles ks / CUs) blocks
-// of ceil(n / ks / KT) slabs each, plus the split-K partial tiles (written
-// once, read once; about four slabs' time per block).  Depends on (n, p) and
-// the device only, so a dataset gets the same ks, and the same bits, alone and
-// in any batch.
-int corr_k_splits(int64_t n, int p) {
-    // work per split in 4-wave units: a 32 x 32 tile is one wave
-    const int64_t tu = gram_tile(p) == CT ? upper_tiles(p) : (upper_tiles(p, 32) + 3) / 4;
+// of ceil(steps / ks) row steps each, plus the split-K partial tiles (written
+// once, read once; about `over` steps' time per block).  The integer path
+// (64 x 64 tiles, 64-row steps) and the FP64 path (32 x 32 tiles for
+// p <= GRAM_SMALL_P, 16-row slabs) have their own models.  Depends on
+// (n, p, dtype) and the device only, so a dataset gets the same ks, and the
+// same bits, alone and in any batch.
+int corr_k_splits(int64_t n, int p, int dtype) {
+    const bool i8 = gram_int8(dtype, p);
+    // work per split in 4-wave units: a 32 x 32 FP64 tile is one wave
+    const int64_t tu = i8 || gram_tile(p) == CT ? upper_tiles(p) : (upper_tiles(p, 32) + 3) / 4;
+    const int64_t rows_per_step = i8 ? 64 : KT;
+    const double over = i8 ? 2.0 : 4.0;
     const int64_t cus = device_cus();
     int64_t kmax = (n + 255) / 256;
     if (kmax > 16) kmax = 16;
+    // integer path: a split holds at most I8_MAX_STEPS steps
+    const int64_t kmin = i8 ? (i8_rows(n) / 64 + I8_MAX_STEPS - 1) / I8_MAX_STEPS : 1;
+    if (kmax < kmin) kmax = kmin;
     int best = 1;
     double best_cost = 0.0;
-    for (int64_t ks = 1; ks <= (kmax < 1 ? 1 : kmax); ++ks) {
+    for (int64_t ks = kmin; ks <= (kmax < 1 ? 1 : kmax); ++ks) {
         const int64_t rows = (n + ks - 1) / ks;
-        const int64_t slabs = (rows + KT - 1) / KT;
-        const double cost = (double)((tu * ks + cus - 1) / cus) * (double)slabs + 4.0 * (double)(tu * ks) / (double)cus;
-        if (ks == 1 || cost < best_cost) {
+        const int64_t steps = (rows + rows_per_step - 1) / rows_per_step;
+        const double cost =
+            (double)((tu * ks + cus - 1) / cus) * (double)steps + over * (double)(tu * ks) / (double)cus;
+        if (ks == kmin || cost < best_cost) {
             best = (int)ks;
             best_cost = cost;
         }
@@ -386,19 +704,29 @@ int corr_k_splits(int64_t n, int p) {
 static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, const Job *one, hipStream_t s) {
     if (nj < 1) return OK;
     int maxp = 0, maxks = 1, maxtiles = 0, maxtiles32 = 0, maxnt = 1, dtype = jobs_host[0].x_dtype;
-    int64_t maxn = 0;
+    int maxi8 = 0, maxi8p = 0;  // integer path: upper tiles and columns
+    int64_t maxn = 0, maxn8 = 0;
     for (int b = 0; b < nj; ++b) {
         const Job &J = jobs_host[b];
         if (!J.X || !J.C32 || !J.corr_ws || J.n < 2 || J.p < 1 || J.k_splits < 1)
             return set_error(ERR_ARG, "corr: job %d has bad arguments", b);
         if (J.x_dtype != dtype) return set_error(ERR_ARG, "corr: jobs of one launch must share x_dtype");
+        if (gram_int8(dtype, J.p) && (i8_rows(J.n) / 64 + J.k_splits - 1) / J.k_splits > I8_MAX_STEPS)
+            return set_error(ERR_ARG, "corr: job %d: k_splits %d leaves more than %d rows per split", b, J.k_splits,
+                             (int)(64 * I8_MAX_STEPS));
         maxp = J.p > maxp ? J.p : maxp;
         maxn = J.n > maxn ? J.n : maxn;
         maxks = J.k_splits > maxks ? J.k_splits : maxks;
-        // Gram tiles by tile size; normalisation tiles are always 64 x 64
-        const int ut = upper_tiles(J.p, gram_tile(J.p));
-        int &mt = gram_tile(J.p) == CT ? maxtiles : maxtiles32;
-        mt = ut > mt ? ut : mt;
+        // Gram tiles by path and tile size; normalisation tiles are always 64 x 64
+        if (gram_int8(dtype, J.p)) {
+            maxi8 = upper_tiles(J.p) > maxi8 ? upper_tiles(J.p) : maxi8;
+            maxi8p = J.p > maxi8p ? J.p : maxi8p;
+            maxn8 = J.n > maxn8 ? J.n : maxn8;
+        } else {
+            const int ut = upper_tiles(J.p, gram_tile(J.p));
+            int &mt = gram_tile(J.p) == CT ? maxtiles : maxtiles32;
+            mt = ut > mt ? ut : mt;
+        }
         maxnt = upper_tiles(J.p) > maxnt ? upper_tiles(J.p) : maxnt;
     }
     if (dtype != 0 && dtype != 1) return set_error(ERR_ARG, "corr: dtype must be 0 (f32) or 1 (f64)");
@@ -413,10 +741,15 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
         GPUCSL_LAUNCH("corr_colsum", colsum_kernel<float>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
     else
         GPUCSL_LAUNCH("corr_colsum", colsum_kernel<double>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
+    if (maxi8 > 0) {
+        const unsigned rb = (unsigned)((i8_rows(maxn8) / 64 + I8_SLICE_STEPS - 1) / I8_SLICE_STEPS);
+        GPUCSL_LAUNCH("corr_slice_i8", slice_i8_kernel, dim3(rb, (maxi8p + 63) / 64, nj), dim3(256), 0, s, jobs_dev);
+        GPUCSL_LAUNCH("corr_gram_i8", gram_i8_kernel, dim3(maxi8, maxks, nj), dim3(256), 0, s, jobs_dev);
+    }
     if (maxtiles > 0) {
         if (dtype == 0)
-            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0, s,
-                          jobs_dev);
+            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0,
+                          s, jobs_dev);
         else
             GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0,
                           s, jobs_dev);
@@ -452,7 +785,10 @@ int c64_to_c32_batch_launch(const Job *jobs_host, const Job *jobs_dev, int nj, h
 
 // ---- single dataset: the workspace starts with one job descriptor ----------
 size_t corr_workspace_bytes(int64_t n, int p) {
-    return JOB_DESC_BYTES + corr_job_ws_bytes(p, corr_k_splits(n, p));
+    int ks = corr_k_splits(n, p, 0);
+    const int ks1 = corr_k_splits(n, p, 1);
+    ks = ks1 > ks ? ks1 : ks;
+    return JOB_DESC_BYTES + corr_job_ws_bytes(n, p, ks);
 }
 
 int corr_launch(const void *X, int dtype, int64_t n, int p, double *C64, float *C32, void *ws, size_t ws_bytes,
@@ -464,7 +800,7 @@ int corr_launch(const void *X, int dtype, int64_t n, int p, double *C64, float *
     j.p = p;
     j.n = n;
     j.x_dtype = dtype;
-    j.k_splits = corr_k_splits(n, p);
+    j.k_splits = corr_k_splits(n, p, dtype);
     j.X = X;
     j.C32 = C32;
     j.C64 = C64;

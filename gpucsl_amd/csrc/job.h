@@ -55,30 +55,53 @@ inline size_t job_ws_bytes(int p) {
     return off;
 }
 
-// per-job correlation workspace
+// Digit planes of the integer Gram path (float32 data): the centred, scaled
+// value of column c at row k is sum_a D_a[c][k] 128^(5-a), every digit in
+// [-64, 64].  Rows and columns are padded to multiples of 64 (padding is zero)
+// and each digit plane is stored as 64 x 64 tiles of 4 KB, tile (column block
+// cb, row step st) at ((a * CB + cb) * steps + st) * 4096 with CB = columns / 64
+// and steps = rows / 64, and inside a tile column c at byte 64 c, row r at
+// byte r: a column panel of one step is one contiguous 4 KB read.
+constexpr int I8_PLANES = 6;
+constexpr int64_t I8_MAX_STEPS = 1024;  // 64-row steps per split-K block (int32 range)
+constexpr int I8_TILE = 4096;
+__host__ __device__ inline int64_t i8_rows(int64_t n) { return (n + 63) / 64 * 64; }
+__host__ __device__ inline int i8_cols(int p) { return (p + 63) / 64 * 64; }
+__host__ __device__ inline size_t i8_tile(int a, int cb, int64_t st, int CB, int64_t steps) {
+    return (((size_t)a * CB + cb) * (size_t)steps + (size_t)st) * I8_TILE;
+}
+
+// per-job correlation workspace: per row split s and column a the sum, minimum
+// and maximum of X[., a] over the split; the digit planes; the ks split-K
+// partial Gram matrices (p x p each, upper tiles only)
 struct CorrWs {
-    double *partial, *mean, *sd, *gram;
+    double *partial, *cmin, *cmax;
+    int8_t *planes;
+    double *gram;
 };
 
-__host__ __device__ inline CorrWs carve_corr(void *ws, int p) {
+__host__ __device__ inline CorrWs carve_corr(void *ws, int64_t n, int p) {
     CorrWs c;
     char *b = (char *)ws;
     size_t off = 0;
+    const size_t split_bytes = align_up(sizeof(double) * (size_t)CORR_ROW_SPLITS_MAX * p, 256);
     c.partial = (double *)(b + off);
-    off = align_up(off + sizeof(double) * (size_t)CORR_ROW_SPLITS_MAX * p, 256);
-    c.mean = (double *)(b + off);
-    off = align_up(off + sizeof(double) * (size_t)p, 256);
-    c.sd = (double *)(b + off);
-    off = align_up(off + sizeof(double) * (size_t)p, 256);
+    off += split_bytes;
+    c.cmin = (double *)(b + off);
+    off += split_bytes;
+    c.cmax = (double *)(b + off);
+    off += split_bytes;
+    c.planes = (int8_t *)(b + off);
+    off = align_up(off + (size_t)I8_PLANES * i8_cols(p) * (size_t)i8_rows(n), 256);
     c.gram = (double *)(b + off);
     return c;
 }
 
-inline size_t corr_job_ws_bytes(int p, int k_splits) {
-    size_t off = 0;
-    off = align_up(off + sizeof(double) * (size_t)CORR_ROW_SPLITS_MAX * p, 256);
-    off = align_up(off + sizeof(double) * (size_t)p, 256);
-    off = align_up(off + sizeof(double) * (size_t)p, 256);
+__host__ __device__ inline CorrWs carve_corr(const Job &J) { return carve_corr(J.corr_ws, J.n, J.p); }
+
+inline size_t corr_job_ws_bytes(int64_t n, int p, int k_splits) {
+    size_t off = 3 * align_up(sizeof(double) * (size_t)CORR_ROW_SPLITS_MAX * p, 256);
+    off = align_up(off + (size_t)I8_PLANES * i8_cols(p) * (size_t)i8_rows(n), 256);
     off = align_up(off + sizeof(double) * (size_t)k_splits * p * p, 256);
     return off;
 }

@@ -72,7 +72,7 @@ int gpucsl_corr_c64_to_c32(const double *C64, int p, float *C32, void *stream);
 /* One Gaussian skeleton job (one dataset).  Every pointer is device memory
  * owned by the caller: ws is a per-job level workspace of
  * gpucsl_gauss_job_workspace_bytes(p) bytes and corr_ws a correlation
- * workspace of gpucsl_corr_job_workspace_bytes(p, k_splits) bytes.  X (n x p
+ * workspace of gpucsl_corr_job_workspace_bytes(n, p, k_splits) bytes.  X (n x p
  * row-major, x_dtype 0 = float32, 1 = float64) is only read by the correlation
  * step and may be NULL when C64 is supplied.  The single-dataset entry points
  * below build one of these internally; the batched entry points take an array
@@ -106,12 +106,16 @@ typedef struct gpucsl_gauss_job {
 } gpucsl_gauss_job;
 
 /* Per-job workspaces of the batched entry points, and the split-K factor the
- * correlation GEMM uses for an n x p dataset (1..16, chosen from n, p and the
- * current device's compute-unit count; the same for a dataset alone and in
- * any batch on that device, which keeps its correlation bits the same). */
+ * correlation GEMM uses for an n x p dataset of x_dtype dtype (1..16, chosen
+ * from n, p, dtype and the current device's compute-unit count; the same for a
+ * dataset alone and in any batch on that device, which keeps its correlation
+ * bits the same; 0 for bad arguments).  The correlation workspace holds the
+ * column sums, the integer digit planes of float32 data (6 bytes per element,
+ * rows and columns padded to multiples of 64) and k_splits partial Gram
+ * matrices. */
 size_t gpucsl_gauss_job_workspace_bytes(int p);
-int gpucsl_corr_k_splits(int64_t n, int p);
-size_t gpucsl_corr_job_workspace_bytes(int p, int k_splits);
+int gpucsl_corr_k_splits(int64_t n, int p, int dtype);
+size_t gpucsl_corr_job_workspace_bytes(int64_t n, int p, int k_splits);
 /* Workspace of the single-dataset entry points (level lists, work queues). */
 size_t gpucsl_gauss_workspace_bytes(int p, int max_level);
 

@@ -122,6 +122,74 @@ def test_corr_matches_numpy(p, n, dtype):
     assert (C64 == C64.T).all()
 
 
+def _corr_engine(X):
+    from gpucsl_amd.gaussian import GaussianEngine
+
+    eng = GaussianEngine(X.shape[1], X.shape[0], 1, [0.1, 0.1])
+    eng.set_data(X)
+    return eng.C64.cpu().numpy(), eng.C32.cpu().numpy()
+
+
+def test_corr_float32_nonfinite_and_constant_columns():
+    """A column holding NaN or Inf, or a constant column, gets NaN correlations
+    in its row and column, as with numpy's FP64 corrcoef; the rest match it."""
+    n, p = 3000, 40
+    X = syn.gaussian_sem(p, n, 3.0, seed=5).data.astype(np.float32)
+    X[17, 3] = np.nan
+    X[:, 5] = 2.5
+    X[400, 7] = np.inf
+    C64, C32 = _corr_engine(X)
+    with np.errstate(all="ignore"):
+        want = np.corrcoef(X.astype(np.float64), rowvar=False)
+    bad = np.zeros(p, bool)
+    bad[[3, 5, 7]] = True
+    assert np.isnan(C64[bad]).all() and np.isnan(C64[:, bad]).all()
+    good = np.ix_(~bad, ~bad)
+    assert not np.isnan(C64[good]).any()
+    err = float(np.abs(C64[good] - want[good]).max())
+    say(f"corr with NaN/Inf/constant columns: max|C64 - numpy| over the rest = {err:.3g}")
+    assert err < 1e-12
+    assert (C32[good] == C64[good].astype(np.float32)).all()
+
+
+def test_corr_float32_offsets_and_scales():
+    """Columns with a large offset and a small spread, and columns scaled by
+    1e-30 and 1e30: each column is scaled by its own power of two before the
+    integer Gram matrix, so the correlation keeps FP64 accuracy."""
+    n, p = 5000, 24
+    X = syn.gaussian_sem(p, n, 3.0, seed=6).data
+    X[:, :4] = 1e4 + 1e-2 * X[:, :4]
+    X[:, 4:8] *= 1e-30
+    X[:, 8:12] *= 1e30
+    X = X.astype(np.float32)
+    C64, _ = _corr_engine(X)
+    want = np.corrcoef(X.astype(np.float64), rowvar=False)
+    err = float(np.abs(C64 - want).max())
+    say(f"corr offsets/scales: max|C64 - numpy| = {err:.3g}")
+    assert err < 1e-12
+
+
+def test_corr_float32_many_rows_and_split_limit():
+    """n = 70 000 rows, p = 300 (float32 data above 256 columns takes the
+    integer Gram path): it needs at least two row splits (at most 65 536 rows
+    each keeps the int32 sums exact); a job that asks for one is refused."""
+    from gpucsl_amd.batch import GaussJobState, GaussianBatch
+    from gpucsl_amd.cli.cli_util import GpucslError
+
+    n, p = 70000, 300
+    assert nat.load().gpucsl_corr_k_splits(n, p, 0) >= 2
+    X = syn.gaussian_sem(p, n, 3.0, seed=8).data.astype(np.float32)
+    C64, _ = _corr_engine(X)
+    err = float(np.abs(C64 - np.corrcoef(X.astype(np.float64), rowvar=False)).max())
+    say(f"corr n={n}: max|C64 - numpy| = {err:.3g}")
+    assert err < 1e-12
+    job = GaussJobState(n, p, 1, alpha=0.01, k_splits=1)
+    job.set_data(torch.from_numpy(X).cuda())
+    b = GaussianBatch([job])
+    with pytest.raises(GpucslError, match="rows per split"):
+        nat.call("gpucsl_gauss_batch_corr", *b._args(), nat.stream_handle())
+
+
 @pytest.mark.parametrize("seed", range(10))
 def test_c1_parity(seed):
     """Config C1: p=50, n=2000 FP64 samples, degree 3, alpha=0.01, L=3."""
