@@ -417,7 +417,10 @@ __global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ j
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 constexpr int I8_MAXU = 6;        // digit pairs (a, b) with a + b <= I8_MAXU
-constexpr int I8_COLB = 80;        // LDS bytes per column and digit: 64 rows + 16 pad
+// LDS panel layout: 64 bytes per column and digit, the four 16-byte chunks of
+// column c in the order q ^ ((c / 4) % 4), so the 16 lanes of a fragment read
+// (16 consecutive columns, one chunk) hit 16 different groups of four banks.
+constexpr int I8_COLB = 64;
 
 // Split-K partial Gram tile (ta <= tb) of 64 x 64 from the digit planes: the
 // 26 digit pairs (a, b) with a + b <= 6, D_a^T D_b accumulated exactly in int32
@@ -466,6 +469,8 @@ __global__ __launch_bounds__(256, 2) void gram_i8_kernel(const Job *__restrict__
     const int8_t *srcA = cw.planes + i8_tile(0, ta, 0, CB, steps) + 16 * tid;
     const int8_t *srcB = cw.planes + i8_tile(0, tb, 0, CB, steps) + 16 * tid;
     i32x4 ra[I8_PLANES], rb[I8_PLANES];
+    const int wchunk = (tid & 3) ^ ((tid >> 4) & 3);  // column tid / 4
+    const int rchunk = kq ^ ((ci >> 2) & 3);          // column 16 m + ci
     auto load = [&](int64_t st) {
         const size_t off = (size_t)st * I8_TILE;
 #pragma unroll
@@ -476,10 +481,10 @@ __global__ __launch_bounds__(256, 2) void gram_i8_kernel(const Job *__restrict__
     };
     auto store = [&]() {
 #pragma unroll
-        for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[0][a][tid >> 2][16 * (tid & 3)] = ra[a];
+        for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[0][a][tid >> 2][16 * wchunk] = ra[a];
         if (!diag)
 #pragma unroll
-            for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[1][a][tid >> 2][16 * (tid & 3)] = rb[a];
+            for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[1][a][tid >> 2][16 * wchunk] = rb[a];
     };
 
     i32x4 acc[I8_MAXU + 1][2][2];
@@ -503,12 +508,12 @@ __global__ __launch_bounds__(256, 2) void gram_i8_kernel(const Job *__restrict__
 #pragma unroll
             for (int b = 0; b < I8_PLANES; ++b)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) bf[b][j] = *(const i32x4 *)&lds[bp][b][wc + 16 * j + ci][16 * kq];
+                for (int j = 0; j < 2; ++j) bf[b][j] = *(const i32x4 *)&lds[bp][b][wc + 16 * j + ci][16 * rchunk];
 #pragma unroll
             for (int a = 0; a < I8_PLANES; ++a) {
                 i32x4 af[2];
 #pragma unroll
-                for (int i = 0; i < 2; ++i) af[i] = *(const i32x4 *)&lds[0][a][wr + 16 * i + ci][16 * kq];
+                for (int i = 0; i < 2; ++i) af[i] = *(const i32x4 *)&lds[0][a][wr + 16 * i + ci][16 * rchunk];
 #pragma unroll
                 for (int b = 0; b < I8_PLANES && a + b <= I8_MAXU; ++b)
 #pragma unroll
