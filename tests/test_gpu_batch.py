@@ -52,8 +52,10 @@ def _same(a, b, label):
     return ok
 
 
+# (p, n, max_level, alpha, seed); the p = 300 and p = 420 jobs take the integer Gram path
 SPECS = [(20, 300, 3, 0.01, 1), (57, 2000, 3, 0.01, 2), (80, 1500, 2, 0.05, 3), (33, 5000, 0, 0.01, 4),
-         (120, 3000, 3, 0.01, 5), (64, 999, 1, 0.01, 6), (200, 5000, 3, 0.01, 7), (45, 700, 3, 0.05, 8)]
+         (120, 3000, 3, 0.01, 5), (64, 999, 1, 0.01, 6), (200, 5000, 3, 0.01, 7), (45, 700, 3, 0.05, 8),
+         (300, 3000, 3, 0.01, 9), (420, 2500, 2, 0.01, 10)]
 
 
 def test_batch_equals_alone_and_oracle():
@@ -87,15 +89,16 @@ def test_batch_membership_does_not_change_results():
     """A job's result is the same bits in two different batches."""
     from gpucsl_amd.batch import GaussianBatch
 
-    a_specs = SPECS[:3] + [(150, 4000, 3, 0.01, 11)]
-    b_specs = [(150, 4000, 3, 0.01, 11)] + SPECS[5:]
+    a_specs = SPECS[:3] + [(150, 4000, 3, 0.01, 11), (330, 3500, 3, 0.01, 12)]
+    b_specs = [(330, 3500, 3, 0.01, 12), (150, 4000, 3, 0.01, 11)] + SPECS[5:]
     ja, _ = _jobs(a_specs)
     jb, _ = _jobs(b_specs)
     ba, bb = GaussianBatch(ja), GaussianBatch(jb)
     ba.launch()
     bb.launch()
     torch.cuda.synchronize()
-    assert _same(ba.results()[3], bb.results()[0], "job in two batches")
+    assert _same(ba.results()[3], bb.results()[1], "job in two batches")
+    assert _same(ba.results()[4], bb.results()[0], "integer-Gram job in two batches")
 
 
 def test_graph_replay_matches_eager():
