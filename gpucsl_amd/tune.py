@@ -27,8 +27,8 @@ from gpucsl_amd import _native as nat
 SPACE = {
     "small_team_iters": (1, 2, 4, 8),
     "wave_team_iters": (2, 4, 8),
-    "fast_blocks_per_cu": (2, 4, 8),
-    "fast_lds_bytes": (16384, 32768, 65536),
+    "fast_blocks_per_cu": (2, 4, 8, 16, 24, 32),
+    "fast_lds_bytes": (4096, 6144, 8192, 16384, 32768, 65536),
     "item_tests": (256, 1024, 4096, 16384),
     "wave_items": (0, 1),
 }
@@ -100,7 +100,7 @@ def sweep(p: int, n: int, degree: float, max_level: int, alpha: float = 0.01, se
             ms = _level_ms(kt.timings, lv, reps)
             if ms < best[lv][0]:
                 best[lv] = (ms, shape)
-        if k % 12 == 0:
+        if k % 100 == 0:
             log(f"[tune] {k + 1}/{len(combos)} shapes, {time.perf_counter() - t0:.1f}s")
     return {"workload": {"p": p, "n": n, "degree": degree, "max_level": max_level, "alpha": alpha, "seed": seed},
             "device": torch.cuda.get_device_name(0),
