@@ -504,24 +504,42 @@ __global__ __launch_bounds__(256, 2) void gram_i8_kernel(const Job *__restrict__
     for (int64_t st = s0; st < s1; ++st) {
         if (st + 1 < s1) load(st + 1);
         if (!idle) {
-            i32x4 bf[I8_PLANES][2];
+            // B fragments of digits 0..3 are held for the whole step; those of
+            // digits 4 and 5 (paired only with a <= 2 and a <= 1) are read where
+            // used, which leaves the registers to read the next digit's A
+            // fragments while the current digit's products run.
+            auto frag = [&](int buf, int digit, int col) {
+                return *(const i32x4 *)&lds[buf][digit][col][16 * rchunk];
+            };
+            i32x4 bf[4][2];
 #pragma unroll
-            for (int b = 0; b < I8_PLANES; ++b)
+            for (int b = 0; b < 4; ++b)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) bf[b][j] = *(const i32x4 *)&lds[bp][b][wc + 16 * j + ci][16 * rchunk];
+                for (int j = 0; j < 2; ++j) bf[b][j] = frag(bp, b, wc + 16 * j + ci);
+            i32x4 af[2], an[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) af[i] = frag(0, 0, wr + 16 * i + ci);
 #pragma unroll
             for (int a = 0; a < I8_PLANES; ++a) {
-                i32x4 af[2];
+                if (a + 1 < I8_PLANES)
 #pragma unroll
-                for (int i = 0; i < 2; ++i) af[i] = *(const i32x4 *)&lds[0][a][wr + 16 * i + ci][16 * rchunk];
+                    for (int i = 0; i < 2; ++i) an[i] = frag(0, a + 1, wr + 16 * i + ci);
+                __builtin_amdgcn_sched_barrier(0);  // keep those reads ahead of the products
 #pragma unroll
-                for (int b = 0; b < I8_PLANES && a + b <= I8_MAXU; ++b)
+                for (int b = 0; b < I8_PLANES && a + b <= I8_MAXU; ++b) {
+                    i32x4 bb[2];
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) bb[j] = b < 4 ? bf[b < 4 ? b : 0][j] : frag(bp, b, wc + 16 * j + ci);
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
                         for (int j = 0; j < 2; ++j)
                             acc[a + b][i][j] =
-                                __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[b][j], acc[a + b][i][j], 0, 0, 0);
+                                __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bb[j], acc[a + b][i][j], 0, 0, 0);
+                }
+                if (a + 1 < I8_PLANES)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) af[i] = an[i];
             }
         }
         __syncthreads();
