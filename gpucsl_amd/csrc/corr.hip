@@ -232,22 +232,40 @@ __global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const J
         const int cur = s & 1;
         if (s + 1 < nslab) load(s + 1);
         const int64_t kk = k0 + (int64_t)s * KT;
+        // the next quarter-slab's values are read from LDS while the current
+        // quarter's products run (the scheduling barrier keeps them there)
+        T xa[2], xb[2], na[2], nb[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) xa[i] = As[cur][lane >> 4][wr + i * 16 + ci];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) xb[j] = Bs[cur][lane >> 4][wc + j * 16 + ci];
 #pragma unroll
         for (int kq = 0; kq < KT / 4; ++kq) {
             const int kr = kq * 4 + (lane >> 4);
             const bool kv = kk + kr < k1;
+            if (kq + 1 < KT / 4) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) na[i] = As[cur][kr + 4][wr + i * 16 + ci];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) nb[j] = Bs[cur][kr + 4][wc + j * 16 + ci];
+            }
+            __builtin_amdgcn_sched_barrier(0);
             double af[2], bf[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-                af[i] = kv ? (double)As[cur][kr][wr + i * 16 + ci] - ma[i] : 0.0;
+            for (int i = 0; i < 2; ++i) af[i] = kv ? (double)xa[i] - ma[i] : 0.0;
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-                bf[j] = kv ? (double)Bs[cur][kr][wc + j * 16 + ci] - mb[j] : 0.0;
+            for (int j = 0; j < 2; ++j) bf[j] = kv ? (double)xb[j] - mb[j] : 0.0;
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+            if (kq + 1 < KT / 4) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) xa[i] = na[i];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) xb[j] = nb[j];
+            }
         }
         if (s + 1 < nslab) store(cur ^ 1);
         __syncthreads();
