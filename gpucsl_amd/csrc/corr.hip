@@ -307,8 +307,8 @@ __global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const J
 //
 // Block: 64 columns x I8_SLICE_STEPS steps of 64 rows; 256 threads.  Thread
 // (column c = tid % 64, quarter q = tid / 64) converts rows 16 q .. 16 q + 15 of
-// its column in each step and stores them as bytes 64 c + 16 q .. + 15 of the
-// step's tile of every digit.  The column centre and scale come from the row
+// its column in each step; they are bytes 64 c + 16 q .. + 15 of the step's
+// tile of every digit.  The column centre and scale come from the row
 // splits' sums and extremes, the splits taken four at a time by the four waves
 // and their partial sums added in wave order.
 constexpr int I8_SLICE_STEPS = 4;
@@ -328,6 +328,7 @@ __global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ j
     const int col = cb * 64 + c;
     __shared__ double s_part[3][4][64];
     __shared__ double s_mu[64], s_sc[64];
+    __shared__ int4 s_tile[I8_PLANES][256 + 64];
     {
         // splits q, q + 4, q + 8, ... in that order, eight loads of each kind in flight
         const int rs = corr_row_splits(n);
@@ -421,6 +422,9 @@ __global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ j
             w[4][g] = rebias(pack(zl, 7, 7));
             w[5][g] = rebias(pack(zl, 0, 7));
         }
+        // through LDS, so that each wave stores 1 KB of a tile contiguously: chunk
+        // (c, q) sits at 16-byte slot 4 c + q + c / 4 (the pad keeps the 16 lanes
+        // of a wave quarter on different banks when writing)
 #pragma unroll
         for (int a = 0; a < I8_PLANES; ++a) {
             int4 v;
@@ -428,8 +432,13 @@ __global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ j
             v.y = w[a][1];
             v.z = w[a][2];
             v.w = w[a][3];
-            *(int4 *)(cw.planes + i8_tile(a, cb, st, CB, steps) + 64 * c + 16 * q) = v;
+            s_tile[a][4 * c + q + (c >> 2)] = v;
         }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < I8_PLANES; ++a)
+            *(int4 *)(cw.planes + i8_tile(a, cb, st, CB, steps) + 16 * tid) = s_tile[a][tid + (tid >> 4)];
+        __syncthreads();
     }
 }
 
