@@ -93,3 +93,19 @@ def test_level_tuning_roundtrip_and_validation():
     finally:
         for lv, sh in before.items():
             set_level(lv, sh)
+
+
+def test_corr_job_workspace_holds_digit_planes():
+    """Host-only sizing: three 64-split column statistics arrays, the six int8
+    digit planes (rows and columns padded to 64) and k_splits FP64 partial Gram
+    matrices, each section 256-byte aligned."""
+    from gpucsl_amd import _native
+
+    lib = _native.load()
+    al = lambda v: (v + 255) // 256 * 256
+    for n, p, k in [(10000, 1000, 7), (333, 65, 1), (70000, 300, 2), (2, 1, 1)]:
+        rows, cols = (n + 63) // 64 * 64, (p + 63) // 64 * 64
+        want = al(3 * al(8 * 64 * p) + 6 * cols * rows) + 8 * k * p * p
+        assert lib.gpucsl_corr_job_workspace_bytes(n, p, k) == al(want), (n, p, k)
+    assert lib.gpucsl_corr_job_workspace_bytes(1, 10, 1) == 0      # n < 2
+    assert lib.gpucsl_corr_job_workspace_bytes(100, 10, 0) == 0    # k_splits < 1
