@@ -684,6 +684,13 @@ struct ItemCtx {
     int prow_slot0;       // team slot (threadIdx.x / team) of prow's first row
 };
 
+// Level 1, ROW mode: a pair's C[j, k] is read from global memory (L2) by the
+// lane whose test needs it instead of staging the whole row C[j, adj(i)] in
+// LDS first; most level-1 pairs end within their first team iteration, so
+// staging would gather up to deg values to use a few dozen.
+template <int L>
+constexpr bool lazy_pair_row() { return L == 1; }
+
 template <int L, int MODE>
 __device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, const int (&sp)[L], bool &near,
                                           const float *rowJ_lds) {
@@ -708,7 +715,7 @@ __device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, con
 #pragma unroll
         for (int a = 0; a < L; ++a) {
             ci[a] = rowI[sp[a]];
-            cj[a] = rowJ_lds[sp[a]];
+            cj[a] = lazy_pair_row<L>() ? ldg(c.C32 + (size_t)j * c.p + S[a]) : rowJ_lds[sp[a]];
 #pragma unroll
             for (int b = 0; b < a; ++b) cm[a][b] = ldg(c.C32 + (size_t)S[a] * c.p + S[b]);
             cm[a][a] = diag[sp[a]];
@@ -823,7 +830,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
         if (c.side == 1 && Gm[(size_t)c.i * c.p + j] != 1) continue;  // removed, or pending in the SLOW kernel
-        if constexpr (MODE == 1) {
+        if constexpr (MODE == 1 && !lazy_pair_row<L>()) {
             __syncthreads();  // the previous pair is done with the row
             stage_pair_row(c, j, c.prow, tid, 256);
             __syncthreads();
@@ -910,7 +917,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
         const int j = nbr_at<L, MODE>(c, jpos);
         if (c.side == 1 && Gm[(size_t)c.i * c.p + j] != 1) continue;  // removed, or pending in the SLOW kernel
         float *myrow = c.prow + (size_t)(threadIdx.x / team - c.prow_slot0) * c.d;
-        if constexpr (MODE == 1) {
+        if constexpr (MODE == 1 && !lazy_pair_row<L>()) {
             // the team's lanes stage C[j, adj(i)]; LDS accesses of one wave
             // are ordered, the fences keep the compiler from moving them
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1004,7 +1011,8 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         const int d = c.d, D = c.D, i = c.i, p = c.p;
         const int teams = 256 / team;
         // ROW mode: neighbour ids, C[i, adj], diag, and one pair row per team
-        const int mode = (L >= 2 && D * D + d <= WORDS) ? 0 : ((3 + teams) * d <= WORDS ? 1 : 2);
+        const int mode =
+            (L >= 2 && D * D + d <= WORDS) ? 0 : ((lazy_pair_row<L>() ? 3 : 3 + teams) * d <= WORDS ? 1 : 2);
         c.gnb = jw.nbr + (size_t)i * p;
         int *snb = lds_words;
         float *sf = reinterpret_cast<float *>(lds_words + d);
@@ -1173,7 +1181,7 @@ __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (lane == 0) s_next[wave] = c.q0;
-        if ((3 + teams_w) * d <= WAVE_WORDS) {
+        if ((lazy_pair_row<L>() ? 3 : 3 + teams_w) * d <= WAVE_WORDS) {
             int *snb = lds_words + (size_t)wave * WAVE_WORDS;
             float *sf = reinterpret_cast<float *>(snb + d);
             for (int k = lane; k < d; k += 64) snb[k] = ldg(c.gnb + k);
