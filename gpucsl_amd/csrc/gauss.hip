@@ -684,12 +684,12 @@ struct ItemCtx {
     int prow_slot0;       // team slot (threadIdx.x / team) of prow's first row
 };
 
-// Level 1, ROW mode: a pair's C[j, k] is read from global memory (L2) by the
-// lane whose test needs it instead of staging the whole row C[j, adj(i)] in
-// LDS first; most level-1 pairs end within their first team iteration, so
-// staging would gather up to deg values to use a few dozen.
+// Levels 1 and 2, ROW mode: a pair's C[j, k] is read from global memory (L2)
+// by the lane whose test needs it instead of staging the whole row
+// C[j, adj(i)] in LDS first; most pairs end within their first team
+// iterations, so staging would gather up to deg values to use a few dozen.
 template <int L>
-constexpr bool lazy_pair_row() { return L == 1; }
+constexpr bool lazy_pair_row() { return L <= 2; }
 
 template <int L, int MODE>
 __device__ __forceinline__ float rho_test(const ItemCtx &c, int jpos, int j, const int (&sp)[L], bool &near,
