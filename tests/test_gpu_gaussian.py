@@ -221,6 +221,20 @@ def test_c2_parity():
         assert res.stats[l]["useful"] == ref["executed"][l]
 
 
+@pytest.mark.parametrize("seed", range(5))
+def test_integer_gram_parity(seed):
+    """float32 data with p = 400 (the integer-digit Gram path): skeleton,
+    sepsets and the useful test counts identical to the FP64 oracle on the
+    same float32 values, over five seeds."""
+    n = 4000
+    X = syn.gaussian_sem(400, n, 4.0, seed=500 + seed).data.astype(np.float32)
+    _, res = _engine_run(X, n, 0.01, 3)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), n, 0.01, 3)
+    _compare(res, ref, f"integer Gram seed {seed}", pmax_atol=5e-5)
+    for l in range(4):
+        assert res.stats[l]["useful"] == ref["executed"][l]
+
+
 def test_user_correlation_matrix_path():
     d = syn.gaussian_sem(120, 1500, 3.0, seed=21)
     C = oracle.corrcoef(d.data)
