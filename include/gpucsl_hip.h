@@ -107,7 +107,9 @@ typedef struct gpucsl_gauss_job {
 
 /* Per-job workspaces of the batched entry points, and the split-K factor the
  * correlation GEMM uses for an n x p dataset of x_dtype dtype (1..16, chosen
- * from n, p, dtype and the current device's compute-unit count; the same for a
+ * from n, p, dtype and the current device's compute-unit count, and at least
+ * n / 65536 for float32 data with p > 256, whose integer sums are exact up to
+ * 65 536 rows per split; a batched job with fewer splits is refused; the same for a
  * dataset alone and in any batch on that device, which keeps its correlation
  * bits the same; 0 for bad arguments).  The correlation workspace holds the
  * column sums, the integer digit planes of float32 data (6 bytes per element,
