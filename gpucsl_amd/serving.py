@@ -72,8 +72,9 @@ class Completed:
         return self.done - self.arrival
 
     def release(self) -> None:
-        """Zero-copy results only: the arrays are views of the server's pinned
-        pool and stay valid until this call."""
+        """Zero-copy and device results only: the arrays are views of the
+        server's pinned pool (or, with results on the device, tensors in its
+        device pool) and stay valid until this call."""
         if self._release is not None:
             self._release()
             self._release = None
@@ -157,7 +158,7 @@ class SkeletonServer:
 
     def __init__(self, hbm_budget: int | None = None, host_budget: int | None = None,
                  margin: float = DEFAULT_MARGIN, group_size: int = DEFAULT_GROUP_SIZE, discrete_streams: int = 2,
-                 ticks_in_flight: int = 2, copy_results: bool = True):
+                 ticks_in_flight: int = 2, copy_results: bool = True, results_on_device: bool = False):
         torch = nat.require_gpu()
         self.torch = torch
         self.lib = nat.load()
@@ -174,8 +175,12 @@ class SkeletonServer:
         self._ws = None
         self._copying: list = []
         # copy_results=False hands out views of the pinned pool (no host copy);
-        # the caller gives them back with Completed.release()
+        # the caller gives them back with Completed.release().  With
+        # results_on_device the skeleton, sepsets, pmax and zmin stay in the
+        # job's device range (torch tensors, nothing crosses to the host but
+        # the discrete statistics) until Completed.release().
         self.copy_results = bool(copy_results)
+        self.results_on_device = bool(results_on_device)
         self._workers = ThreadPoolExecutor(max_workers=4, thread_name_prefix="gpucsl-results")
         self.gstream = torch.cuda.Stream()
         self.dstreams = [torch.cuda.Stream() for _ in range(max(1, discrete_streams))]
@@ -402,6 +407,14 @@ class SkeletonServer:
         torch = self.torch
         st: GaussJobState = j.state
         p, L = st.p, max(st.max_level, 1)
+        if self.results_on_device:
+            j.host = {"skeleton": st.view("skeleton", torch.int16, (p, p)),
+                      "sepsets": st.view("sepsets", torch.int32, (p, p, L)),
+                      "pmax": st.view("pmax", torch.float32, (p, p)),
+                      "zmin": st.view("zmin", torch.float32, (p, p))}
+            j.done_event = torch.cuda.Event()
+            j.done_event.record(self.gstream)
+            return
         h = self._host_views(j, torch)
         h["skeleton"].copy_(st.view("skeleton", torch.int16, (p, p)), non_blocking=True)
         h["sepsets"].copy_(st.view("sepsets", torch.int32, (p, p, L)), non_blocking=True)
@@ -422,10 +435,13 @@ class SkeletonServer:
             else:
                 eng.finish()
                 h = self._host_views(j, torch)
-                h["skeleton"].copy_(eng.skel16, non_blocking=True)
-                h["sepsets"].copy_(eng.sep, non_blocking=True)
-                h["pmax"].copy_(eng.pmax, non_blocking=True)
                 h["stats"].copy_(eng.stats, non_blocking=True)
+                if self.results_on_device:
+                    h["skeleton"], h["sepsets"], h["pmax"] = eng.skel16, eng.sep, eng.pmax
+                else:
+                    h["skeleton"].copy_(eng.skel16, non_blocking=True)
+                    h["sepsets"].copy_(eng.sep, non_blocking=True)
+                    h["pmax"].copy_(eng.pmax, non_blocking=True)
                 j.host = h
                 j.done_event = torch.cuda.Event()
                 j.done_event.record(j.stream)
@@ -436,10 +452,20 @@ class SkeletonServer:
         """Jobs whose device work and result copy are complete give back their
         device range at once; their results are copied out of the pinned pool
         by worker threads (large fresh host arrays cost page faults, which must
-        not stall the launches), and the request completes when its copy has."""
+        not stall the launches), and the request completes when its copy has.
+        With results on the device a finished job completes at once and keeps
+        its device range until the caller releases the result."""
         still, zero_copy = [], []
         for j in self.active:
             if j.done_event is not None and j.done_event.query():
+                if self.results_on_device:
+                    # the results are the job's device state: both ranges go
+                    # back when the caller releases them
+                    c = self._result(j, copy=False)
+                    c._release = (lambda dr=j.dev_range, hr=j.host_range:
+                                  (self.dev_pool.release(*dr), self.host_pool.release(*hr)))
+                    zero_copy.append(c)
+                    continue
                 self.dev_pool.release(*j.dev_range)
                 if self.copy_results:
                     self._copying.append((j, self._workers.submit(self._result, j)))
@@ -468,12 +494,18 @@ class SkeletonServer:
     def _result(self, j, copy: bool = True) -> Completed:
         req, h = j.req, j.host
         L = req.max_level
-        own = (lambda a: a.copy()) if copy else (lambda a: a)
-        skel = own(h["skeleton"].numpy().view(np.uint16))
-        sep = own(h["sepsets"].numpy()[:, :, :L])
-        pmax = own(h["pmax"].numpy())
+        if self.results_on_device:
+            # torch tensors on the device; the skeleton as int16 (uint16 bits)
+            skel, sep, pmax = h["skeleton"], h["sepsets"][:, :, :L], h["pmax"]
+            zmin = h.get("zmin")
+        else:
+            own = (lambda a: a.copy()) if copy else (lambda a: a)
+            skel = own(h["skeleton"].numpy().view(np.uint16))
+            sep = own(h["sepsets"].numpy()[:, :, :L])
+            pmax = own(h["pmax"].numpy())
+            zmin = own(h["zmin"].numpy()) if req.kind == "gaussian" else None
         if req.kind == "gaussian":
-            res = GaussianResult(skel, sep, pmax, own(h["zmin"].numpy()), [])
+            res = GaussianResult(skel, sep, pmax, zmin, [])
             stats = []
         else:
             stats = nat.decode_stats(h["stats"].numpy().view(np.uint64), L)

@@ -78,6 +78,34 @@ def test_served_equals_alone_under_continuous_batching():
         assert _same(c.result, _alone(r), r.kind), f"request {r.rid} ({r.kind} p={r.p} n={r.n}) differs"
 
 
+def test_results_on_device_equal_alone_and_hold_their_range():
+    """results_on_device: results are device tensors of the job's own range,
+    the same bits as the request alone, and the range stays in use until the
+    caller releases the result."""
+    from gpucsl_amd.serving import SkeletonServer
+
+    reqs = _requests(9, 6, 3)
+    srv = SkeletonServer(results_on_device=True)
+    for r in reqs:
+        srv.submit(r)
+    done = srv.drain()
+    assert len(done) == len(reqs)
+    held = srv.used
+    assert held > 0                      # every finished job still holds its device range
+    for c in done:
+        r = next(x for x in reqs if x.rid == c.rid)
+        res = c.result
+        assert res.skeleton.is_cuda and res.pmax.is_cuda
+        a = _alone(r)
+        assert (res.skeleton.cpu().numpy().view(np.uint16) == a.skeleton).all()
+        assert (res.separation_sets.cpu().numpy() == a.separation_sets).all()
+        assert np.array_equal(res.pmax.cpu().numpy().view(np.uint32), a.pmax.view(np.uint32))
+        if r.kind == "gaussian":
+            assert np.array_equal(res.zmin.cpu().numpy().view(np.uint32), a.zmin.view(np.uint32))
+        c.release()
+    assert srv.used == 0
+
+
 def test_served_matches_oracle_small():
     """Served Gaussian and discrete results against the FP64 oracle."""
     from gpucsl_amd.serving import Request, SkeletonServer
