@@ -399,11 +399,7 @@ __device__ void scan_job(const Job &J, int level, const Team &team, unsigned lon
 // the scan launch: item counts in itemoffA/itemoffB and pairs per item in
 // rowoffA, until the scans replace them with offsets.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jobs, int level, Team team,
-                                                      unsigned long long *__restrict__ st, PlanArgs plan) {
-    const bool pre = plan.done != nullptr;
-    if (pre && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *plan.done = 0;
-    const Job &J = jobs[blockIdx.y];
+__device__ __forceinline__ void compact_rows(const Job &J, int level, const Team &team, bool pre) {
     const int p = J.p;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     JobWs w = carve_job(J.ws, p);
@@ -454,6 +450,19 @@ __global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jo
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jobs, int level, Team team,
+                                                      unsigned long long *__restrict__ st, PlanArgs plan) {
+    const bool pre = plan.done != nullptr;
+    if (pre && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *plan.done = 0;
+    compact_rows(jobs[blockIdx.y], level, team, pre);
+}
+
+// One dataset passed by value: a compaction that no scan follows needs no
+// descriptor in device memory.
+__global__ __launch_bounds__(256) void compact_one_kernel(Job one, int level, Team team) {
+    compact_rows(one, level, team, false);
 }
 
 // Per job (one block each) scan_job; in a batch with a plan, the block that
@@ -1804,6 +1813,12 @@ int gauss_skeleton_launch(const float *C32, const double *C64, int p, int max_le
 
 int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats, hipStream_t s) {
     Job j = single_job(p, level, nullptr, nullptr, const_cast<uint8_t *>(Gm), nullptr, nullptr, nullptr, 0, ws);
+    if (!stats) {
+        GPUCSL_LAUNCH("compact", compact_one_kernel, dim3((p + 3) / 4, 1), dim3(256), 0, s, j, level,
+                      team_for(level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
+                               16));
+        return check_hip(hipGetLastError(), "compact launch");
+    }
     int rc = single_write_desc(j, ws, s);
     if (rc) return rc;
     return gauss_batch_compact(&j, carve_single(ws, p).desc, 1, level, 16, stats, s);
