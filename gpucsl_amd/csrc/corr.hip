@@ -796,14 +796,11 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
         GPUCSL_LAUNCH("corr_slice_i8", slice_i8_kernel, dim3(rb, (maxi8p + 63) / 64, nj), dim3(256), 0, s, jobs_dev);
         GPUCSL_LAUNCH("corr_gram_i8", gram_i8_kernel, dim3(maxi8, maxks, nj), dim3(256), 0, s, jobs_dev);
     }
-    if (maxtiles > 0) {
-        if (dtype == 0)
-            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0,
-                          s, jobs_dev);
-        else
-            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0,
-                          s, jobs_dev);
-    }
+    // 64 x 64 FP64 tiles: float64 data above GRAM_SMALL_P (float32 data there
+    // takes the integer path)
+    if (maxtiles > 0)
+        GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0, s,
+                      jobs_dev);
     if (maxtiles32 > 0) {
         if (dtype == 0)
             GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, 32>), dim3(maxtiles32, maxks, nj), dim3(64), 0,
