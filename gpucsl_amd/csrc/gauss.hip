@@ -369,6 +369,9 @@ __device__ void scan_job(const Job &J, int level, const Team &team, unsigned lon
         w.rowoffB[p] = s_tot[1];
         w.itemoffA[p] = s_tot[2];
         w.itemoffB[p] = s_tot[3];
+        // the level kernels read the |rho| threshold instead of evaluating
+        // the FP64 tanh once per work item
+        if (active && level < GPUCSL_MAX_LEVELS) w.rthr[level] = (float)tanh(J.thr[level]);
         if (st) {
             unsigned long long T = 0;
             int M = 0;
@@ -907,6 +910,30 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
     }
 }
 
+// Minimum over each team's lanes (team a power of two, 1..64, lanes aligned
+// to the team size) of non-negative floats, valid in every lane of the team.
+// Non-negative floats (+inf included) order like their bit patterns as
+// unsigned integers.  Within 16 lanes the steps are DPP moves (quad
+// permutes, half-row and row mirrors), which stay inside the team; wider
+// teams add lane swaps through ds_bpermute.
+__device__ __forceinline__ unsigned int umin_u32(unsigned int a, unsigned int b) { return a < b ? a : b; }
+
+template <int CTRL>
+__device__ __forceinline__ unsigned int dpp_u32(unsigned int u) {
+    return (unsigned int)__builtin_amdgcn_update_dpp((int)u, (int)u, CTRL, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ float team_min(float x, int team) {
+    unsigned int v = __float_as_uint(x);
+    if (team >= 2) v = umin_u32(v, dpp_u32<0xB1>(v));   // quad_perm [1,0,3,2]
+    if (team >= 4) v = umin_u32(v, dpp_u32<0x4E>(v));   // quad_perm [2,3,0,1]
+    if (team >= 8) v = umin_u32(v, dpp_u32<0x141>(v));  // row_half_mirror
+    if (team >= 16) v = umin_u32(v, dpp_u32<0x140>(v)); // row_mirror
+    if (team >= 32) v = umin_u32(v, (unsigned int)__shfl_xor((int)v, 16));
+    if (team >= 64) v = umin_u32(v, (unsigned int)__shfl_xor((int)v, 32));
+    return __uint_as_float(v);
+}
+
 // teams of `team` lanes inside a wave pull pairs from s_next
 template <int L, int MODE>
 __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__restrict__ queue,
@@ -949,7 +976,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
                 pass = ar < c.rthr;
             }
             if (__ballot(near) & gmask) {
-                for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
+                rl = team_min(rl, team);
                 if (gl == 0) enqueue_slow(c, j, SlowItem{iter * team, c.rowoff[c.i] + q, c.job, z_of_rho(rl), 0}, queue, qcount);
                 break;
             }
@@ -972,7 +999,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
                 finish = (iter + 1) * team >= c.count;
             }
             if (finish) {
-                for (int off = team / 2; off >= 1; off >>= 1) rl = fminf(rl, __shfl_xor(rl, off));
+                rl = team_min(rl, team);
                 if (gl == 0) update_zmin(*c.J, c.p, c.i, j, z_of_rho(rl) * c.zscale);
                 break;
             }
@@ -1005,7 +1032,7 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         JobWs jw = carve_job(J.ws, c.p);
         c.rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
         c.C32 = J.C32;
-        c.rthr = (float)tanh(J.thr[L]);
+        c.rthr = jw.rthr[L];
         c.margin_k = margin_k;
         c.zscale = level_zscale(J, L);
         c.d = jw.deg[c.i];
@@ -1166,7 +1193,7 @@ __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
         JobWs jw = carve_job(J.ws, c.p);
         c.rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
         c.C32 = J.C32;
-        c.rthr = (float)tanh(J.thr[L]);
+        c.rthr = jw.rthr[L];
         c.margin_k = margin_k;
         c.zscale = level_zscale(J, L);
         c.d = jw.deg[c.i];
@@ -1251,8 +1278,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
     const Job *__restrict__ jobs, float margin_k, unsigned int *__restrict__ stripe_counters,
     const SlowItem *__restrict__ queueA, const unsigned int *__restrict__ qcountA,
     const SlowItem *__restrict__ queueB, const unsigned int *__restrict__ qcountB,
-    unsigned long long *__restrict__ bstats, unsigned int *__restrict__ stats_done,
-    unsigned long long *__restrict__ st) {
+    const unsigned long long *__restrict__ bstats, unsigned long long *__restrict__ st) {
     const int lane = __lane_id();
     const int gbase = lane & ~(GS - 1);
     const int gl = lane & (GS - 1);
@@ -1455,46 +1481,28 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
         wstat[wave][BS_MAXRATIO] = __float_as_uint(mr);
     }
     __syncthreads();
-    if (threadIdx.x < 5) {
-        unsigned long long v = 0;
-        for (int w2 = 0; w2 < 4; ++w2) {
-            const unsigned long long x = wstat[w2][threadIdx.x];
-            v = threadIdx.x == BS_MAXRATIO ? (x > v ? x : v) : v + x;
-        }
-        unsigned long long *dst = bstats + (size_t)(blockIdx.x % BSTAT_STRIPES) * BSTAT_WORDS + threadIdx.x;
-        if (v) {
-            if (threadIdx.x == BS_MAXRATIO)
-                atomicMax(dst, v);
-            else
-                atomicAdd(dst, v);
-        }
+    if (!st || threadIdx.x >= 5) return;
+    // the block's totals and the FAST kernels' stripes blockIdx.x,
+    // blockIdx.x + gridDim.x, ... (complete: both FAST launches precede this
+    // one on the stream) go straight to the level's statistics, so no block
+    // waits for another
+    const int k = threadIdx.x;
+    const bool is_max = k == BS_MAXRATIO;
+    unsigned long long v = 0;
+    for (int w2 = 0; w2 < 4; ++w2) {
+        const unsigned long long x = wstat[w2][k];
+        v = is_max ? (x > v ? x : v) : v + x;
     }
-    if (!st) return;
-    // the block that finishes last adds the level's striped totals to st
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last = atomicAdd(stats_done, 1u) == gridDim.x - 1;
+    for (int sidx = blockIdx.x; sidx < BSTAT_STRIPES; sidx += gridDim.x) {
+        const unsigned long long x = bstats[(size_t)sidx * BSTAT_WORDS + k];
+        v = is_max ? (x > v ? x : v) : v + x;
     }
-    __syncthreads();
-    if (!s_last || threadIdx.x >= 64) return;
-    __threadfence();
-    const int t = threadIdx.x;
-    unsigned long long v[5];
-    for (int k = 0; k < 5; ++k) v[k] = __hip_atomic_load(bstats + (size_t)t * BSTAT_WORDS + k, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-    for (int off = 32; off >= 1; off >>= 1) {
-        for (int k = 0; k < 4; ++k) v[k] += __shfl_xor(v[k], off);
-        const unsigned long long o = __shfl_xor(v[4], off);
-        v[4] = o > v[4] ? o : v[4];
-    }
-    if (t == 0) {
-        st[ST_EXEC] += v[BS_EXEC];
-        st[ST_USEFUL] += v[BS_USEFUL];
-        st[ST_RECHECK] += v[BS_RECHECK];
-        st[ST_FLIP] += v[BS_FLIP];
-        st[ST_MAXRATIO] = v[BS_MAXRATIO] > st[ST_MAXRATIO] ? v[BS_MAXRATIO] : st[ST_MAXRATIO];
+    if (v) {
+        const int slot = k == BS_EXEC ? ST_EXEC : k == BS_USEFUL ? ST_USEFUL : k == BS_RECHECK ? ST_RECHECK : ST_FLIP;
+        if (is_max)
+            atomicMax(st + ST_MAXRATIO, v);
+        else
+            atomicAdd(st + slot, v);
     }
 }
 
@@ -1638,7 +1646,7 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
     if (side == 0) return;
     GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 64>), dim3(slow_grid()), dim3(256), 0, s, jd, margin_k,
                   w.counters + STRIPES, w.queueA, w.counters + CNT_QUEUE, w.queueB, w.counters + CNT_QUEUE + 1,
-                  w.bstats, w.counters + CNT_STATS_DONE, st);
+                  w.bstats, st);
 }
 
 int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float margin_k, int group_size,

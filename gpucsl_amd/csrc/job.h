@@ -28,6 +28,7 @@ struct WorkItem {
 // per-job level workspace
 struct JobWs {
     int32_t *nbr, *deg, *cntA, *cntB, *rowoffA, *rowoffB, *itemoffA, *itemoffB;
+    float *rthr;  // per level: (float) tanh(thr[level]), written by the level's scan
 };
 
 __host__ __device__ inline JobWs carve_job(void *ws, int p) {
@@ -44,6 +45,8 @@ __host__ __device__ inline JobWs carve_job(void *ws, int p) {
     w.rowoffB = w.rowoffA + (p + 1);
     w.itemoffA = w.rowoffB + (p + 1);
     w.itemoffB = w.itemoffA + (p + 1);
+    off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 4, 256);
+    w.rthr = (float *)(b + off);
     return w;
 }
 
@@ -52,6 +55,7 @@ inline size_t job_ws_bytes(int p) {
     off = align_up(off + sizeof(int32_t) * (size_t)p * p, 256);
     off = align_up(off + sizeof(int32_t) * (size_t)p * 3, 256);
     off = align_up(off + sizeof(int32_t) * (size_t)(p + 1) * 4, 256);
+    off = align_up(off + sizeof(float) * GPUCSL_MAX_LEVELS, 256);
     return off;
 }
 
@@ -110,7 +114,6 @@ inline size_t corr_job_ws_bytes(int64_t n, int p, int k_splits) {
 constexpr int NUM_COUNTERS = 64;
 constexpr int CNT_QUEUE = 32;       // 32, 33: queue lengths of sides A and B
 constexpr int CNT_PLAN_DONE = 40;   // scan blocks finished (level planning)
-constexpr int CNT_STATS_DONE = 41;  // side-B recheck blocks finished (level statistics)
 constexpr int STRIPES = 8;
 constexpr int BSTAT_WORDS = 8;
 constexpr int BSTAT_STRIPES = 64;
