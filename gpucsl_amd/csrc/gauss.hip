@@ -681,6 +681,14 @@ typedef const __attribute__((address_space(1))) float gfloat_t;
 typedef const __attribute__((address_space(1))) int gint_t;
 __device__ __forceinline__ float ldg(const float *ptr) { return *(gfloat_t *)(ptr); }
 __device__ __forceinline__ int ldg(const int *ptr) { return *(gint_t *)(ptr); }
+// Byte loads and the stores of the per-pair results likewise: flat accesses
+// also count against the LDS wait counter, so the next pair's LDS claim would
+// wait for them.
+typedef __attribute__((address_space(1))) uint8_t gu8_t;
+typedef __attribute__((address_space(1))) int32_t gi32_t;
+__device__ __forceinline__ uint8_t ldg(const uint8_t *ptr) { return *(const gu8_t *)(ptr); }
+__device__ __forceinline__ void stg(uint8_t *ptr, uint8_t v) { *(gu8_t *)(ptr) = v; }
+__device__ __forceinline__ void stg(int32_t *ptr, int32_t v) { *(gi32_t *)(ptr) = v; }
 
 struct ItemCtx {
     const Job *J;
@@ -766,23 +774,32 @@ template <int L, int MODE>
 __device__ __forceinline__ void write_sepset(const ItemCtx &c, int j, const int (&sp)[L]) {
     const Job &J = *c.J;
     const int p = c.p, i = c.i;
-    J.G[(size_t)i * p + j] = 0;
-    J.G[(size_t)j * p + i] = 0;
+    stg(J.G + (size_t)i * p + j, (uint8_t)0);
+    stg(J.G + (size_t)j * p + i, (uint8_t)0);
     int32_t *sep = J.sepsets;
     const int maxL = J.max_level;
     const bool both = !sepset_one_way(J);
 #pragma unroll
     for (int k = 0; k < L; ++k) {
         const int v = nbr_at<L, MODE>(c, sp[k]);
-        sep[((size_t)i * p + j) * maxL + k] = v;
-        if (both) sep[((size_t)j * p + i) * maxL + k] = v;
+        stg(sep + ((size_t)i * p + j) * maxL + k, v);
+        if (both) stg(sep + ((size_t)j * p + i) * maxL + k, v);
     }
+}
+
+// zmin[a, b] = min(zmin[a, b], z) for a non-negative z (NaN leaves it as it
+// is): non-negative floats order like their bits as unsigned integers, so one
+// atomic without a return value does it and the wave does not wait for a load
+// (through the global address space: a flat atomic would also count against
+// the LDS wait counter)
+__device__ __forceinline__ void zmin_update(float *zp, float z) {
+    typedef __attribute__((address_space(1))) unsigned int guint_t;
+    __hip_atomic_fetch_min((guint_t *)(zp), __float_as_uint(z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void update_zmin(const Job &J, int p, int i, int j, float z) {
     const int a = i < j ? i : j, b = i < j ? j : i;
-    float *zp = J.zmin + (size_t)a * p + b;
-    if (z < *zp) *zp = z;
+    zmin_update(J.zmin + (size_t)a * p + b, z);
 }
 
 // C[j, adj(i)] into LDS by `lanes` lanes (lane index `l`)
@@ -827,8 +844,8 @@ __device__ __forceinline__ void enqueue_slow(const ItemCtx &c, int j, const Slow
     const unsigned int slot = atomicAdd(qcount, 1u);
     queue[slot] = item;
     if (c.side == 0) {
-        c.J->G[(size_t)c.i * c.p + j] = 2;
-        c.J->G[(size_t)j * c.p + c.i] = 2;
+        stg(c.J->G + (size_t)c.i * c.p + j, (uint8_t)2);
+        stg(c.J->G + (size_t)j * c.p + c.i, (uint8_t)2);
     }
 }
 
@@ -841,7 +858,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
     for (int q = c.q0; q < c.q1; ++q) {
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
-        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] != 1) continue;  // removed, or pending in the SLOW kernel
+        if (c.side == 1 && ldg(Gm + (size_t)c.i * c.p + j) != 1) continue;  // removed, or pending in the SLOW kernel
         if constexpr (MODE == 1 && !lazy_pair_row<L>()) {
             __syncthreads();  // the previous pair is done with the row
             stage_pair_row(c, j, c.prow, tid, 256);
@@ -898,7 +915,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
                 if (tid == 0) {
                     const float zb = z_of_rho(fminf(fminf(s_r[0], s_r[1]), fminf(s_r[2], s_r[3])));
                     if (any_near) {
-                        enqueue_slow(c, j, SlowItem{iter * 256, c.rowoff[c.i] + q, c.job, zb, 0}, queue, qcount);
+                        enqueue_slow(c, j, SlowItem{iter * 256, ldg(c.rowoff + c.i) + q, c.job, zb, 0}, queue, qcount);
                     } else {
                         update_zmin(*c.J, c.p, c.i, j, zb * c.zscale);
                     }
@@ -951,7 +968,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
         if (q >= c.q1) break;
         const int jpos = c.jbase + q;
         const int j = nbr_at<L, MODE>(c, jpos);
-        if (c.side == 1 && Gm[(size_t)c.i * c.p + j] != 1) continue;  // removed, or pending in the SLOW kernel
+        if (c.side == 1 && ldg(Gm + (size_t)c.i * c.p + j) != 1) continue;  // removed, or pending in the SLOW kernel
         float *myrow = c.prow + (size_t)(threadIdx.x / team - c.prow_slot0) * c.d;
         if constexpr (MODE == 1 && !lazy_pair_row<L>()) {
             // the team's lanes stage C[j, adj(i)]; LDS accesses of one wave
@@ -977,7 +994,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
             }
             if (__ballot(near) & gmask) {
                 rl = team_min(rl, team);
-                if (gl == 0) enqueue_slow(c, j, SlowItem{iter * team, c.rowoff[c.i] + q, c.job, z_of_rho(rl), 0}, queue, qcount);
+                if (gl == 0) enqueue_slow(c, j, SlowItem{iter * team, ldg(c.rowoff + c.i) + q, c.job, z_of_rho(rl), 0}, queue, qcount);
                 break;
             }
             if (act) ++n_exec;
@@ -1032,11 +1049,11 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         JobWs jw = carve_job(J.ws, c.p);
         c.rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
         c.C32 = J.C32;
-        c.rthr = jw.rthr[L];
+        c.rthr = ldg(jw.rthr + L);
         c.margin_k = margin_k;
         c.zscale = level_zscale(J, L);
-        c.d = jw.deg[c.i];
-        c.cnt = c.rowoff[c.i + 1] - c.rowoff[c.i];
+        c.d = ldg(jw.deg + c.i);
+        c.cnt = ldg(c.rowoff + c.i + 1) - ldg(c.rowoff + c.i);
         c.Nn = c.d - 1;
         c.count = binom_k<L>(c.Nn);
         const int team = team_lanes(c.count, tm);
@@ -1193,11 +1210,11 @@ __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
         JobWs jw = carve_job(J.ws, c.p);
         c.rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
         c.C32 = J.C32;
-        c.rthr = jw.rthr[L];
+        c.rthr = ldg(jw.rthr + L);
         c.margin_k = margin_k;
         c.zscale = level_zscale(J, L);
-        c.d = jw.deg[c.i];
-        c.cnt = c.rowoff[c.i + 1] - c.rowoff[c.i];
+        c.d = ldg(jw.deg + c.i);
+        c.cnt = ldg(c.rowoff + c.i + 1) - ldg(c.rowoff + c.i);
         c.Nn = c.d - 1;
         c.count = binom_k<L>(c.Nn);
         c.q0 = it.q0;
@@ -1423,9 +1440,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
             for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
             if (gl == 0) {
                 const int a = i < j ? i : j, b = i < j ? j : i;
-                float *zp = J->zmin + (size_t)a * p + b;
-                const float zs = zl * level_zscale(*J, L);
-                if (zs < *zp) *zp = zs;
+                zmin_update(J->zmin + (size_t)a * p + b, zl * level_zscale(*J, L));
             }
             have = false;
             if (side == 0 && !pm) {
