@@ -25,8 +25,8 @@ import numpy as np
 from gpucsl_amd import _native as nat
 
 SPACE = {
-    "small_team_iters": (1, 2, 4, 8),
-    "wave_team_iters": (2, 4, 8),
+    "small_team_iters": (1, 2, 4, 8, 16, 32),
+    "wave_team_iters": (2, 4, 8, 16),
     "fast_blocks_per_cu": (2, 4, 8, 16, 24, 32),
     "fast_lds_bytes": (4096, 6144, 8192, 16384, 32768, 65536),
     "item_tests": (256, 1024, 4096, 16384),
