@@ -820,6 +820,30 @@ __device__ __forceinline__ void stage_pair_row(const ItemCtx &c, int j, float *d
     }
 }
 
+// Minimum over each team's lanes (team a power of two, 1..64, lanes aligned
+// to the team size) of non-negative floats, valid in every lane of the team.
+// Non-negative floats (+inf included) order like their bit patterns as
+// unsigned integers.  Within 16 lanes the steps are DPP moves (quad
+// permutes, half-row and row mirrors), which stay inside the team; wider
+// teams add lane swaps through ds_bpermute.
+__device__ __forceinline__ unsigned int umin_u32(unsigned int a, unsigned int b) { return a < b ? a : b; }
+
+template <int CTRL>
+__device__ __forceinline__ unsigned int dpp_u32(unsigned int u) {
+    return (unsigned int)__builtin_amdgcn_update_dpp((int)u, (int)u, CTRL, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ float team_min(float x, int team) {
+    unsigned int v = __float_as_uint(x);
+    if (team >= 2) v = umin_u32(v, dpp_u32<0xB1>(v));   // quad_perm [1,0,3,2]
+    if (team >= 4) v = umin_u32(v, dpp_u32<0x4E>(v));   // quad_perm [2,3,0,1]
+    if (team >= 8) v = umin_u32(v, dpp_u32<0x141>(v));  // row_half_mirror
+    if (team >= 16) v = umin_u32(v, dpp_u32<0x140>(v)); // row_mirror
+    if (team >= 32) v = umin_u32(v, (unsigned int)__shfl_xor((int)v, 16));
+    if (team >= 64) v = umin_u32(v, (unsigned int)__shfl_xor((int)v, 32));
+    return __uint_as_float(v);
+}
+
 template <int L>
 __device__ __forceinline__ void subset_of(const ItemCtx &c, uint64_t rank, int jpos, int (&sp)[L]) {
     int pos[L];
@@ -908,8 +932,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
                 }
             }
             if (finish) {
-                float m = rl;
-                for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+                const float m = team_min(rl, 64);
                 if (lane == 0) s_r[wave] = m;
                 __syncthreads();
                 if (tid == 0) {
@@ -925,30 +948,6 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
             if (finish) break;
         }
     }
-}
-
-// Minimum over each team's lanes (team a power of two, 1..64, lanes aligned
-// to the team size) of non-negative floats, valid in every lane of the team.
-// Non-negative floats (+inf included) order like their bit patterns as
-// unsigned integers.  Within 16 lanes the steps are DPP moves (quad
-// permutes, half-row and row mirrors), which stay inside the team; wider
-// teams add lane swaps through ds_bpermute.
-__device__ __forceinline__ unsigned int umin_u32(unsigned int a, unsigned int b) { return a < b ? a : b; }
-
-template <int CTRL>
-__device__ __forceinline__ unsigned int dpp_u32(unsigned int u) {
-    return (unsigned int)__builtin_amdgcn_update_dpp((int)u, (int)u, CTRL, 0xF, 0xF, false);
-}
-
-__device__ __forceinline__ float team_min(float x, int team) {
-    unsigned int v = __float_as_uint(x);
-    if (team >= 2) v = umin_u32(v, dpp_u32<0xB1>(v));   // quad_perm [1,0,3,2]
-    if (team >= 4) v = umin_u32(v, dpp_u32<0x4E>(v));   // quad_perm [2,3,0,1]
-    if (team >= 8) v = umin_u32(v, dpp_u32<0x141>(v));  // row_half_mirror
-    if (team >= 16) v = umin_u32(v, dpp_u32<0x140>(v)); // row_mirror
-    if (team >= 32) v = umin_u32(v, (unsigned int)__shfl_xor((int)v, 16));
-    if (team >= 64) v = umin_u32(v, (unsigned int)__shfl_xor((int)v, 32));
-    return __uint_as_float(v);
 }
 
 // teams of `team` lanes inside a wave pull pairs from s_next
@@ -1364,11 +1363,11 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 gnb = jw.nbr + (size_t)i * p;
                 j = gnb[jpos];
                 Nn = d - 1;
-                count = binom(Nn, L);
+                count = binom_k<L>(Nn);
                 next = qi.start;
                 zl = gl == 0 ? qi.zl : __builtin_inff();
                 thr = J->thr[L];
-                rthr = (float)tanh(thr);
+                rthr = ldg(jw.rthr + L);
                 have = count > 0;
             }
         }
@@ -1436,8 +1435,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
             finish = next >= count;
         }
         if (finish) {
-#pragma unroll
-            for (int off = GS / 2; off >= 1; off >>= 1) zl = fminf(zl, __shfl_xor(zl, off));
+            zl = team_min(zl, GS);
             if (gl == 0) {
                 const int a = i < j ? i : j, b = i < j ? j : i;
                 zmin_update(J->zmin + (size_t)a * p + b, zl * level_zscale(*J, L));
@@ -1460,7 +1458,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 jpos = lo;
                 side = 1;
                 Nn = d - 1;
-                count = binom(Nn, L);
+                count = binom_k<L>(Nn);
                 next = 0;
                 zl = __builtin_inff();
                 have = count > 0;
