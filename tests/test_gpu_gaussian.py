@@ -223,6 +223,30 @@ def test_c2_parity(seed):
         assert res.stats[l]["useful"] == ref["executed"][l]
 
 
+def test_high_levels_match_oracle():
+    """Levels 5-8, the deepest kernels built: a dense random SEM whose skeleton
+    still has tests at level 7, and an equicorrelated block (every correlation
+    0.5, n = 100 000) whose partial correlations stay far above the threshold,
+    so that every subset of every pair runs up to level 8.  Skeleton, sepsets,
+    pmax and the useful counts against the FP64 oracle."""
+    d = syn.gaussian_sem(100, 10000, 8.0, seed=11)
+    _, res = _engine_run(d.data, 10000, 0.01, 8)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), 10000, 0.01, 8)
+    assert ref["executed"][7] > 0
+    _compare(res, ref, "dense SEM, max level 8", pmax_atol=5e-5)  # FP32 z at n = 10 000, as for C2
+    for l in range(9):
+        assert res.stats[l]["useful"] == ref["executed"][l]
+    p, n = 12, 100000
+    C = np.full((p, p), 0.5)
+    np.fill_diagonal(C, 1.0)
+    _, res = _engine_run(None, n, 0.01, 8, C=C)
+    ref = oracle.gauss_skeleton(C, n, 0.01, 8)
+    assert ref["executed"][8] == 66 * 2 * 45  # both sides of every pair, C(10, 8) subsets
+    _compare(res, ref, "equicorrelated, max level 8")
+    for l in range(9):
+        assert res.stats[l]["useful"] == ref["executed"][l]
+
+
 @pytest.mark.parametrize("seed", range(5))
 def test_integer_gram_parity(seed):
     """float32 data with p = 400 (the integer-digit Gram path): skeleton,
