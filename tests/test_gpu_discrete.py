@@ -147,6 +147,15 @@ def test_c3_parity():
     _compare(res, ref, "C3")
 
 
+def test_high_levels_parity():
+    """Levels 4-8: a binary network (p=16, n=50 000, degree 5) whose skeleton
+    still has tests at level 8."""
+    d = syn.discrete_bn(16, 50000, 5.0, seed=2, card_low=2, card_high=2)
+    res, ref = _run(d.data, 0.05, 8)
+    assert ref["executed"][8] > 0
+    _compare(res, ref, "discrete, max level 8")
+
+
 @pytest.mark.parametrize("p,card,n,label", [
     (14, (14, 16), 2500, "one-hot Gram matrix, 16 categories"),
     (12, (17, 20), 2500, "pair-by-pair tables, more than 16 categories"),
