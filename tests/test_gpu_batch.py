@@ -85,6 +85,47 @@ def test_batch_equals_alone_and_oracle():
     assert [d["useful"] for d in st.levels] == oracle_exec
 
 
+def test_batch_with_deep_levels_equals_alone_and_oracle():
+    """Jobs that stop at different levels up to 8 (dense SEMs, degree 8, with
+    tests up to level 7) in one batch: each job bit-identical to its run alone,
+    skeleton and sepsets equal to the FP64 oracle, and the batch's useful
+    counts per level equal to the oracle's."""
+    from gpucsl_amd.batch import GaussianBatch, GaussJobState, run_plain
+
+    specs = [(100, 10000, 8, 11), (60, 5000, 6, 3), (70, 6000, 3, 9)]
+
+    def make():
+        jobs, data = [], []
+        for (p, n, L, seed) in specs:
+            X = syn.gaussian_sem(p, n, 8.0, seed=seed).data.astype(np.float32)
+            j = GaussJobState(n, p, L, alpha=0.01)
+            j.set_data(X)
+            jobs.append(j)
+            data.append(X)
+        return jobs, data
+
+    jobs, data = make()
+    b = GaussianBatch(jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    got = b.results()
+    alone, _ = make()
+    run_plain(alone)
+    torch.cuda.synchronize()
+    refs = [oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), n, 0.01, L)
+            for (p, n, L, seed), X in zip(specs, data)]
+    assert refs[0]["executed"][7] > 0
+    for k, ((p, n, L, seed), ref) in enumerate(zip(specs, refs)):
+        assert _same(got[k], alone[k].results(), f"deep job {k} batch vs alone")
+        assert (got[k].skeleton == ref["skeleton"]).all(), f"deep job {k} skeleton vs oracle"
+        assert (got[k].separation_sets == ref["sepsets"]).all(), f"deep job {k} sepsets vs oracle"
+    st = b.batch_stats()
+    oracle_exec = [sum(int(ref["executed"][l]) if l <= L else 0 for (p, n, L, s), ref in zip(specs, refs))
+                   for l in range(9)]
+    say("  deep batch useful per level", [d["useful"] for d in st.levels], "oracle", oracle_exec)
+    assert [d["useful"] for d in st.levels] == oracle_exec
+
+
 def test_batch_membership_does_not_change_results():
     """A job's result is the same bits in two different batches."""
     from gpucsl_amd.batch import GaussianBatch
