@@ -379,6 +379,25 @@ def test_pcalg_mode_matches_oracle(seed):
     assert (np.abs(plain["pmax"][iu] - ref["pmax"][iu]).max() > 0) or len(one_way) > 0
 
 
+def test_pcalg_mode_deep_levels():
+    """pcalg conventions up to level 8 on a dense SEM (degree 8, tests up to
+    level 7): each level's own sqrt(n - l - 3) in the thresholds and in pMax."""
+    from gpucsl_amd.pc.discover_skeleton_gaussian import discover_skeleton_gpu_gaussian
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    p, n = 100, 10000
+    X = syn.gaussian_sem(p, n, 8.0, seed=11).data.astype(np.float32)
+    res, _ = discover_skeleton_gpu_gaussian(init_pc_graph(X), X, None, 0.01, 8, p, n, pcalg_compat=True)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), n, 0.01, 8, pcalg_compat=True)
+    assert ref["executed"][7] > 0
+    assert (res.skeleton == ref["skeleton"]).all()
+    assert (res.seperation_sets == ref["sepsets"]).all()
+    iu = np.triu_indices(p, 1)
+    dp = np.abs(res.pmax[iu].astype(np.float64) - ref["pmax"][iu]).max()
+    say(f"[pcalg deep] edges {int(res.skeleton.sum()) // 2}, max|dpmax| {dp:.3g}")
+    assert dp <= 5e-5  # FP32 z at n = 10 000, as for C2
+
+
 def test_level_by_level_and_batch_of_one_equal_whole_skeleton():
     """The three ways of launching one dataset's levels give the same bits:
     the whole-skeleton entry point (level planning inside the compaction
