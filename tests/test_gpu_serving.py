@@ -78,6 +78,31 @@ def test_served_equals_alone_under_continuous_batching():
         assert _same(c.result, _alone(r), r.kind), f"request {r.rid} ({r.kind} p={r.p} n={r.n}) differs"
 
 
+def test_deep_level_requests_served_equal_alone():
+    """Requests that run to levels 6-8 (dense Gaussian SEMs, a binary network)
+    served together with shallow ones: each result bit-identical to its run
+    alone."""
+    from gpucsl_amd.serving import Request, SkeletonServer
+
+    reqs = [Request("gaussian", 10000, 100, max_level=8, alpha=0.01,
+                    X=syn.gaussian_sem_device(100, 10000, 8.0, seed=11)),
+            Request("gaussian", 5000, 60, max_level=6, alpha=0.01,
+                    X=syn.gaussian_sem_device(60, 5000, 8.0, seed=3))]
+    codes, dims = syn.discrete_bn_device(16, 50000, 5.0, seed=2, card_low=2, card_high=2)
+    reqs.append(Request("discrete", 50000, 16, max_level=8, alpha=0.05, codes=codes, dims=dims))
+    reqs += _requests(9, 3, 2)
+    torch.cuda.synchronize()
+    srv = SkeletonServer()
+    for r in reqs:
+        srv.submit(r)
+    done = srv.drain()
+    assert len(done) == len(reqs) and srv.idle
+    by_id = {c.rid: c for c in done}
+    for r in reqs:
+        assert _same(by_id[r.rid].result, _alone(r), r.kind), f"request {r.rid} ({r.kind} p={r.p} L={r.max_level}) differs"
+    say(f"deep-level mix: {len(reqs)} requests in {srv.ticks} ticks, all equal alone")
+
+
 def test_results_on_device_equal_alone_and_hold_their_range():
     """results_on_device: results are device tensors of the job's own range,
     the same bits as the request alone, and the range stays in use until the
