@@ -101,9 +101,18 @@ def _pcalg_skeleton(data, correlation_matrix, alpha, max_level, p, n, device, ke
 
     group_size, margin = _tuning(kernels)
     with torch.cuda.device(device):
-        job = GaussJobState(n, p, max_level, alpha=alpha, pcalg_compat=True)
+        x_dtype = "float64"
         if correlation_matrix is None:
-            job.set_data(np.asarray(data, np.float64).astype(np.float32))
+            # the correlation is computed from the data as given: float32 data
+            # on the float32 path, anything else in float64 (as the default mode)
+            data = data if isinstance(data, torch.Tensor) else np.asarray(data)
+            if data.dtype in (np.float32, torch.float32):
+                x_dtype = "float32"
+            elif not isinstance(data, torch.Tensor) and data.dtype != np.float64:
+                data = data.astype(np.float64)
+        job = GaussJobState(n, p, max_level, alpha=alpha, pcalg_compat=True, x_dtype=x_dtype)
+        if correlation_matrix is None:
+            job.set_data(data)
         else:
             job.set_correlation(correlation_matrix)
         batch = GaussianBatch([job], margin=margin, group_size=group_size)

@@ -49,7 +49,7 @@ class Request:
     p: int
     max_level: int = 3
     alpha: float = 0.01
-    X: object = None           # gaussian: (n, p) float32 tensor (device or host)
+    X: object = None           # gaussian: (n, p) float32 or float64 tensor or array (device or host)
     codes: object = None       # discrete: (p, >= n) uint8 device tensor, one row per variable
     dims: object = None        # discrete: (p,) category counts
     max_dim: int = 0           # discrete: largest category count (0 = from dims)
@@ -81,11 +81,18 @@ class Completed:
             self.result = None
 
 
+def x_dtype_of(req: Request) -> str:
+    """The dtype a Gaussian request's correlation is computed from: float64 X
+    stays float64 (as in the single-dataset path); anything else is float32."""
+    dt = str(getattr(req.X, "dtype", "float32"))
+    return "float64" if dt in ("float64", "torch.float64") else "float32"
+
+
 def request_footprint(req: Request) -> int:
     """HBM bytes of a request's device state (data, matrices, separation sets,
     workspaces), the unit of admission."""
     if req.kind == "gaussian":
-        return job_footprint(req.n, req.p, req.max_level, 4)
+        return job_footprint(req.n, req.p, req.max_level, 8 if x_dtype_of(req) == "float64" else 4)
     if req.kind == "discrete":
         return disc_layout(req.n, req.p, req.max_level, max(req.max_dim, 2))[1]
     raise GpucslError(f"unknown request kind {req.kind!r}")
@@ -292,9 +299,10 @@ class SkeletonServer:
             arena = self.dev_mem[dr[0]:dr[0] + dr[1]]
             if req.kind == "gaussian":
                 with torch.cuda.stream(self.gstream):
-                    st = GaussJobState(req.n, req.p, req.max_level, alpha=req.alpha, arena=arena)
+                    st = GaussJobState(req.n, req.p, req.max_level, alpha=req.alpha, arena=arena,
+                                       x_dtype=x_dtype_of(req))
                     X = req.X
-                    if X.is_cuda:
+                    if getattr(X, "is_cuda", False):
                         X.record_stream(self.gstream)
                     st.set_data(X)
                 job = _Job(req, st, self.gstream, dr, hr)
@@ -530,7 +538,7 @@ def run_alone(req: Request, margin: float = DEFAULT_MARGIN, group_size: int = DE
 
     torch = nat.require_gpu()
     if req.kind == "gaussian":
-        st = GaussJobState(req.n, req.p, req.max_level, alpha=req.alpha)
+        st = GaussJobState(req.n, req.p, req.max_level, alpha=req.alpha, x_dtype=x_dtype_of(req))
         st.set_data(req.X)
         run_plain([st], margin, group_size)
         return st

@@ -379,6 +379,26 @@ def test_pcalg_mode_matches_oracle(seed):
     assert (np.abs(plain["pmax"][iu] - ref["pmax"][iu]).max() > 0) or len(one_way) > 0
 
 
+@pytest.mark.parametrize("seed", range(10))
+def test_pcalg_mode_fp64_c1(seed):
+    """pcalg mode on C1's float64 data (p=50, n=2000, degree 3, L=3): the
+    correlation is computed from the float64 values, so skeleton and sepsets
+    equal the FP64 oracle's pcalg mode on the same data."""
+    from gpucsl_amd.pc.discover_skeleton_gaussian import discover_skeleton_gpu_gaussian
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    X = syn.gaussian_sem(50, 2000, 3.0, seed=seed).data
+    assert X.dtype == np.float64
+    res, _ = discover_skeleton_gpu_gaussian(init_pc_graph(X), X, None, 0.01, 3, 50, 2000, pcalg_compat=True)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X), 2000, 0.01, 3, pcalg_compat=True)
+    assert (res.skeleton == ref["skeleton"]).all()
+    assert (res.seperation_sets == ref["sepsets"]).all()
+    iu = np.triu_indices(50, 1)
+    dp = np.abs(res.pmax[iu].astype(np.float64) - ref["pmax"][iu]).max()
+    say(f"[pcalg fp64 C1 seed {seed}] edges {int(res.skeleton.sum()) // 2}, max|dpmax| {dp:.3g}")
+    assert dp <= 2e-5
+
+
 def test_pcalg_mode_deep_levels():
     """pcalg conventions up to level 8 on a dense SEM (degree 8, tests up to
     level 7): each level's own sqrt(n - l - 3) in the thresholds and in pMax."""

@@ -179,3 +179,24 @@ def test_admission_by_footprint_serialises_and_rejects():
     big = Request("gaussian", 100, 2000, 3, 0.01, X=torch.zeros((100, 2000), device="cuda"))
     with pytest.raises(GpucslError):
         srv.submit(big)
+
+
+def test_served_float64_request_keeps_float64():
+    """A Gaussian request with float64 data is served on the float64 path:
+    its results equal the same request alone, and the FP64 oracle's skeleton
+    and sepsets on those float64 values."""
+    from gpucsl_amd.serving import Request, SkeletonServer
+
+    X = syn.gaussian_sem(50, 2000, 3.0, seed=3).data
+    assert X.dtype == np.float64
+    req = Request("gaussian", 2000, 50, max_level=3, alpha=0.01, X=torch.from_numpy(X).cuda())
+    srv = SkeletonServer(hbm_budget=1 << 30, host_budget=64 << 20, copy_results=True)
+    srv.submit(req)
+    out = srv.drain()
+    assert len(out) == 1
+    served = out[0].result
+    alone = _alone(Request("gaussian", 2000, 50, max_level=3, alpha=0.01, X=torch.from_numpy(X).cuda()))
+    assert _same(served, alone, "gaussian")
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X), 2000, 0.01, 3)
+    assert (served.skeleton == ref["skeleton"]).all()
+    assert (served.separation_sets == ref["sepsets"]).all()

@@ -189,3 +189,47 @@ def test_pcalg_thresholds_closed_form():
     thr = oracle.pcalg_thresholds(100, 0.01, 10)
     for l in range(10):
         assert thr[l] == pytest.approx(norm.isf(0.005) / np.sqrt(100 - l - 3), rel=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_gauss_pmax_and_counts_vs_bruteforce(seed):
+    """zmin/pmax (min over exactly the tests a sequential pcalg-order loop runs)
+    and the per-level executed and T_l counts of the C oracle against the
+    brute-force loop, which counts every test it runs."""
+    n = 400
+    d = syn.gaussian_sem(10, n, 4.0, seed=30 + seed)
+    C = oracle.corrcoef(d.data)
+    thr = oracle.thresholds(n, 0.05)
+    res = oracle.gauss_skeleton(C, n, 0.05, 4)
+    ref = mini.gauss_run(C, thr, 4, n=n)
+    assert ref["ties"] == 0
+    assert (res["skeleton"] == ref["skeleton"]).all()
+    L = len(ref["executed"])
+    assert (res["executed"][:L] == ref["executed"]).all(), (res["executed"], ref["executed"])
+    assert (res["total"][:L] == ref["total"]).all(), (res["total"], ref["total"])
+    assert ref["executed"][1] > 0
+    iu = np.triu_indices(10, 1)
+    # two FP64 formulas of the same statistic, rounded to float32 zmin
+    assert np.allclose(res["pmax"][iu], ref["pmax"][iu], rtol=1e-5, atol=1e-9)
+    assert (res["pmax"][np.tril_indices(10)] == -1).all()
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_disc_pmax_and_counts_vs_bruteforce(seed):
+    """Discrete pmax is the max, over every subset of both ordered pairs of an
+    edge adjacent at level start, of the float-rounded p-value; executed is
+    that full enumeration.  Checked against the brute-force loop (scipy's
+    chi2.sf; the oracle uses Perlman's pchisq, within 2e-6 of it)."""
+    d = syn.discrete_bn(8, 3000, 2.5, seed=20 + seed)
+    data = np.ascontiguousarray(d.data.T)
+    dims = (data.max(axis=1) + 1).astype(np.uint8)
+    res = oracle.disc_skeleton(data, dims, 0.05, 3)
+    ref = mini.disc_run(data, dims, 0.05, 3, margin=1e-5)
+    assert ref["ties"] == 0
+    assert (res["skeleton"] == ref["skeleton"]).all()
+    L = len(ref["executed"])
+    assert (res["executed"][:L] == ref["executed"]).all(), (res["executed"], ref["executed"])
+    assert ref["executed"][1] > 0
+    iu = np.triu_indices(8, 1)
+    assert np.abs(res["pmax"][iu] - ref["pmax"][iu]).max() < 4e-6
+    assert (res["pmax"][np.tril_indices(8)] == -1).all()
