@@ -1,86 +1,96 @@
-"""Input and output files of the command line, in the formats of
-gpucsl/cli/cli_io.py:11-93: CSV datasets without header, `<stem>.gml`
-(networkx), `<stem>_sepset.txt` (`i k s1 s2 ...` per separated ordered pair),
-`<stem>_pmax.csv` (`%.4f`) and `<stem>_config.txt`."""
-import os
+"""Files the command line reads and writes.
+
+The file names and formats are the interface of gpucsl's command line
+(gpucsl/cli/cli_io.py:39-101), so results can be diffed against it:
+
+* input: a CSV table without a header row; a path without the ``.csv``
+  suffix (or with another suffix) names ``<stem>.csv`` in the same folder;
+* ``<stem>.gml``: the CPDAG, networkx GML;
+* ``<stem>_sepset.txt``: one line ``i k s1 s2 ...`` per ordered pair (i, k)
+  with a recorded separation set, pairs in row-major order (an empty file
+  when the run had no levels above 0);
+* ``<stem>_pmax.csv``: the pmax matrix, comma separated, ``%.4f``;
+* ``<stem>_config.txt``: the command line the files came from.
+"""
+from __future__ import annotations
+
 from pathlib import Path
+from typing import Iterable, Iterator, Optional, Sequence
 
 import networkx as nx
 import numpy as np
 
 from gpucsl_amd.cli.cli_util import error
 
-
-def force_csv_suffix(path: Path) -> Path:
-    if path.suffix != ".csv":
-        path = path.parent / (path.stem + ".csv")
-    return path
+CONFIG_HEADER = "The output for the dataset {stem} was generated using the following given parameters: \n\n"
 
 
-def get_file_name_stem(dataset_path: str) -> str:
-    return Path(dataset_path).stem
+def csv_path(path) -> Path:
+    """The CSV file a dataset argument names: its suffix, if any, becomes .csv."""
+    path = Path(path)
+    return path if path.suffix == ".csv" else path.with_name(path.stem + ".csv")
 
 
-def read_csv(path) -> np.ndarray:
+def dataset_stem(path) -> str:
+    """Output files are named after the dataset file without its suffix."""
+    return Path(path).stem
+
+
+def load_table(path) -> np.ndarray:
+    """A header-less CSV table as an array (numeric or object columns)."""
     import pandas as pd
 
-    try:
-        path = force_csv_suffix(Path(path))
-        return pd.read_csv(path.absolute(), header=None).to_numpy()
-    except FileNotFoundError as err:
-        error(str(err))
+    target = csv_path(path).absolute()
+    if not target.is_file():
+        error(f"no such dataset file: {target}")
+    return pd.read_csv(target, header=None).to_numpy()
 
 
-def read_correlation_matrix(path):
-    """None means: compute the correlation matrix from the data."""
-    if path is None:
-        return None
-    return read_csv(path)
+def load_correlation(path: Optional[str]) -> Optional[np.ndarray]:
+    """A cached correlation matrix, or None when the data's own is wanted."""
+    return None if path is None else load_table(path)
 
 
-def write_seperation_set(output_path: Path, file_name_stem: str, seperation_set) -> None:
-    p = seperation_set.shape[0]
-    with open(output_path / (file_name_stem + "_sepset.txt"), "w", newline="") as f:
-        if seperation_set.size == 0:
-            return  # an empty file still records that there are no separation sets
-        for i in range(p):
-            for k in range(p):
-                s = seperation_set[i][k]
-                if s[0] == -1:
-                    continue
-                items = s[s != -1]
-                f.write(f"{i} {k} ")
-                f.write(" ".join(map(str, items)))
-                f.write("\n")
+def sepset_lines(separation_sets: np.ndarray) -> Iterator[str]:
+    """``i k s1 s2 ...`` for every ordered pair whose first entry is set."""
+    sep = np.asarray(separation_sets)
+    if sep.ndim != 3 or sep.shape[2] == 0:
+        return
+    rows, cols = np.nonzero(sep[:, :, 0] != -1)
+    for i, k in zip(rows.tolist(), cols.tolist()):
+        members = sep[i, k]
+        yield " ".join(str(v) for v in [i, k, *members[members != -1].tolist()])
 
 
-def write_directed_graph(output_path: Path, file_name_stem: str, directed_graph) -> None:
-    nx.write_gml(directed_graph, output_path / (file_name_stem + ".gml"))
+class ResultFiles:
+    """The four output files of one run, under one folder and stem."""
+
+    def __init__(self, folder, stem: str):
+        self.folder = Path(folder)
+        self.stem = stem
+
+    def path(self, suffix: str) -> Path:
+        return self.folder / f"{self.stem}{suffix}"
+
+    def write(self, result, argv: Sequence[str]) -> None:
+        self.folder.mkdir(parents=True, exist_ok=True)
+        nx.write_gml(result.directed_graph, self.path(".gml"))
+        self._write_lines(self.path("_sepset.txt"), sepset_lines(result.separation_sets))
+        np.savetxt(self.path("_pmax.csv"), np.asarray(result.pmax), delimiter=",", fmt="%.4f")
+        with open(self.path("_config.txt"), "w", newline="") as f:
+            f.write(CONFIG_HEADER.format(stem=self.stem))
+            f.write(" ".join(argv) + "\n")
+
+    @staticmethod
+    def _write_lines(path: Path, lines: Iterable[str]) -> None:
+        with open(path, "w", newline="") as f:
+            for line in lines:
+                f.write(line + "\n")
 
 
-def write_pmax(output_path: Path, file_name_stem: str, pmax) -> None:
-    np.savetxt(output_path / (file_name_stem + "_pmax.csv"), pmax, delimiter=",", fmt="%.4f")
-
-
-def write_configuration(output_path: Path, file_name_stem: str, command_line_arguments) -> None:
-    with open(output_path / (file_name_stem + "_config.txt"), "w", newline="") as f:
-        f.write(f"The output for the dataset {file_name_stem} was generated using the following given "
-                f"parameters: \n\n")
-        f.write(" ".join(command_line_arguments))
-        f.write("\n")
-
-
-def write_results(output_path: Path, file_name_stem: str, result) -> None:
-    if not os.path.isdir(output_path):
-        os.makedirs(output_path)
-    write_directed_graph(output_path, file_name_stem, result.directed_graph)
-    write_seperation_set(output_path, file_name_stem, result.separation_sets)
-    write_pmax(output_path, file_name_stem, result.pmax)
-    print(f"successfully wrote results to: {output_path}")
-
-
-def write_results_and_config(output_directory: str, dataset_path: str, command_line_arguments, result) -> None:
-    output_path = Path(output_directory)
-    stem = get_file_name_stem(dataset_path)
-    write_results(output_path, stem, result)
-    write_configuration(output_path, stem, command_line_arguments)
+def save_run(out_dir, dataset_path, argv: Sequence[str], result) -> Path:
+    """Writes the CPDAG, separation sets, pmax and configuration of a run."""
+    files = ResultFiles(out_dir, dataset_stem(dataset_path))
+    files.write(result, argv)
+    print(f"successfully wrote results to: {files.folder}")
+    return files.folder

@@ -1,25 +1,30 @@
-"""Command-line flags of ``python -m gpucsl_amd``, the same as
-gpucsl/cli/command_line_parser.py:4-68 (names, short forms, defaults, types)."""
+"""Flags of ``python -m gpucsl_amd``.
+
+Names, short forms, defaults and value types are gpucsl's
+(gpucsl/cli/command_line_parser.py:4-68), so existing invocations work
+unchanged; the help texts describe this build.
+"""
 import argparse
 
+# (flags, keyword arguments of add_argument)
+_FLAGS = [
+    (("-a", "--alpha"), dict(type=float, default=0.05, help="significance level of every CI test (default 0.05)")),
+    (("-b", "--debug"), dict(action="store_true", help="run the bounds-checked debug build of the kernels")),
+    (("-c", "--correlation-matrix"), dict(help="CSV file with a precomputed correlation matrix (Gaussian only)")),
+    (("-d", "--dataset"), dict(required=True, default="", help="CSV file of observations, one column per variable")),
+    (("--discrete",), dict(action="count", default=0, help="Pearson chi-square test on categorical data")),
+    (("--gaussian",), dict(action="count", default=0, help="Fisher-z partial-correlation test on continuous data")),
+    (("-g", "--gpus"), dict(type=int, nargs="+", help="GPU index to run on (one GPU)")),
+    (("-l", "--max-level"), dict(type=int, help="largest separation-set size to test (default: p - 2)")),
+    (("-o", "--output-directory"), dict(required=True, help="folder for the .gml, _sepset, _pmax and _config files")),
+    (("-s", "--sync-device"), dict(type=int, help="accepted for compatibility; one GPU needs no sync device")),
+    (("-v", "--verbose"), dict(action="count", default=0, help="-v: info log with level timings, -vv: debug log")),
+]
 
-def build_command_line_parser():
-    parser = argparse.ArgumentParser(description="PC-stable causal structure learning on an AMD Instinct GPU.")
-    parser.add_argument("-a", "--alpha", help="Sets the alpha level", default=0.05, type=float)
-    parser.add_argument("-b", "--debug", help="Accepted for compatibility; the kernels are prebuilt",
-                        action="store_true")
-    parser.add_argument("-c", "--correlation-matrix", help="Path to a cached correlation matrix")
-    parser.add_argument("-d", "--dataset", help="The dataset to run", default="", required=True)
-    parser.add_argument("--discrete", help="Uses a discrete conditional independence test", action="count",
-                        default=0)
-    parser.add_argument("--gaussian",
-                        help="Uses a gaussian conditional independence test for data with a multivariate distribution",
-                        action="count", default=0)
-    parser.add_argument("-g", "--gpus", help="Indices of GPUs to use (one GPU in this build)", type=int, nargs="+")
-    parser.add_argument("-l", "--max-level", help="Maximum size of seperation set used in pc algorithm", type=int)
-    parser.add_argument("-o", "--output-directory", help="Output dir to store output files", required=True)
-    parser.add_argument("-s", "--sync-device", help="Index of the GPU used to sync the others", type=int)
-    parser.add_argument("-v", "--verbose",
-                        help="Prints debug messages to stdout: -v logs info, -vv logs debug info",
-                        action="count", default=0)
+
+def build_command_line_parser() -> argparse.ArgumentParser:
+    parser = argparse.ArgumentParser(prog="python -m gpucsl_amd",
+                                     description="PC-stable causal structure learning on an AMD Instinct GPU.")
+    for flags, kwargs in _FLAGS:
+        parser.add_argument(*flags, **kwargs)
     return parser

@@ -1,4 +1,12 @@
-"""Public PC API, drop-in for gpucsl/pc/pc.py (GaussianPC, DiscretePC)."""
+"""Public PC API, drop-in for gpucsl/pc/pc.py: ``GaussianPC`` and ``DiscretePC``.
+
+The class surface is gpucsl's (gpucsl/pc/pc.py:19-123): constructor
+arguments, ``set_distribution_specific_options`` per test,
+``discover_skeleton`` and ``execute`` returning ``TimedReturn(PCResult,
+seconds)``.  A subclass only says which skeleton routine runs and with which
+extra arguments; validation, the fully connected start graph and the
+orientation step are shared.
+"""
 from abc import ABC, abstractmethod
 from typing import List
 
@@ -16,73 +24,72 @@ from gpucsl_amd.pc.pc_validations import (
 
 
 class PC(ABC):
-    """gpucsl/pc/pc.py:19-68."""
-
     def __init__(self, data: np.ndarray, max_level: int, alpha=0.05, kernels=None,
                  is_debug: bool = False, should_log: bool = False):
         validate_alpha(alpha)
         self.data = data
+        self.num_observations, self.num_variables = data.shape[0], data.shape[1]
         self.max_level = determine_max_level(max_level, data)
         self.alpha = alpha
         self.kernels = kernels
         self.is_debug = is_debug
         self.should_log = should_log
-        self.num_variables = data.shape[1]
-        self.num_observations = data.shape[0]
         self.graph = init_pc_graph(data)
+
+    def _common_arguments(self) -> dict:
+        return dict(skeleton=self.graph, data=self.data, alpha=self.alpha, max_level=self.max_level,
+                    num_variables=self.num_variables, num_observations=self.num_observations,
+                    kernels=self.kernels, is_debug=self.is_debug, should_log=self.should_log)
 
     @abstractmethod
     def discover_skeleton(self):
-        pass
+        """TimedReturn(SkeletonResult, seconds) of this test's skeleton routine."""
 
     @abstractmethod
     def set_distribution_specific_options(self):
-        pass
+        """Options of one independence test; returns self."""
 
     @timed
     def execute(self):
-        (skeleton, separation_sets, pmax, kernel_time), skeleton_time = self.discover_skeleton()
-        directed_graph, orientation_time = orient_edges(nx.DiGraph(skeleton), separation_sets)
-        return PCResult(directed_graph, separation_sets, pmax, skeleton_time, orientation_time,
-                        kernel_time)
+        skeleton_run = self.discover_skeleton()
+        sk = skeleton_run.result
+        orientation = orient_edges(nx.DiGraph(sk.skeleton), sk.seperation_sets)
+        return PCResult(orientation.result, sk.seperation_sets, sk.pmax, skeleton_run.runtime,
+                        orientation.runtime, sk.discover_skeleton_kernel_time)
 
 
 class GaussianPC(PC):
-    """gpucsl/pc/pc.py:71-102."""
-
-    def discover_skeleton(self):
-        from gpucsl_amd.pc.discover_skeleton_gaussian import discover_skeleton_gpu_gaussian
-
-        return discover_skeleton_gpu_gaussian(
-            self.graph, self.data, self.correlation_matrix, self.alpha, self.max_level,
-            self.num_variables, self.num_observations, self.kernels, self.is_debug,
-            self.should_log, self.devices, self.sync_device, self.pcalg_compat)
+    """Fisher-z partial-correlation tests (gpucsl/pc/pc.py:71-102)."""
 
     def set_distribution_specific_options(self, devices: List[int] = [0], sync_device: int = None,
                                           correlation_matrix: np.ndarray = None, pcalg_compat: bool = False):
         """pcalg_compat (no counterpart in gpucsl): pcalg's thresholds, pMax and
         one-way separation sets instead of gpucsl's."""
-        self.pcalg_compat = pcalg_compat
-        warn_against_too_high_memory_consumption_gaussian_pc(self.max_level, self.num_variables)
         assert len(devices) > 0, "Invalid device list, must not be empty"
+        warn_against_too_high_memory_consumption_gaussian_pc(self.max_level, self.num_variables)
         validate_given_devices(devices, sync_device)
-        self.devices = devices
-        self.sync_device = sync_device
+        self.devices, self.sync_device = devices, sync_device
         self.correlation_matrix = correlation_matrix
+        self.pcalg_compat = pcalg_compat
         return self
+
+    def discover_skeleton(self):
+        from gpucsl_amd.pc.discover_skeleton_gaussian import discover_skeleton_gpu_gaussian
+
+        return discover_skeleton_gpu_gaussian(correlation_matrix=self.correlation_matrix, devices=self.devices,
+                                              sync_device=self.sync_device, pcalg_compat=self.pcalg_compat,
+                                              **self._common_arguments())
 
 
 class DiscretePC(PC):
-    """gpucsl/pc/pc.py:105-123."""
-
-    def discover_skeleton(self):
-        from gpucsl_amd.pc.discover_skeleton_discrete import discover_skeleton_gpu_discrete
-
-        return discover_skeleton_gpu_discrete(
-            self.graph, self.data, self.alpha, self.max_level, self.num_variables,
-            self.num_observations, self.kernels, self.memory_restriction, self.is_debug,
-            self.should_log)
+    """Pearson chi-square tests on categorical data (gpucsl/pc/pc.py:105-123)."""
 
     def set_distribution_specific_options(self, memory_restriction=None):
         self.memory_restriction = memory_restriction
         return self
+
+    def discover_skeleton(self):
+        from gpucsl_amd.pc.discover_skeleton_discrete import discover_skeleton_gpu_discrete
+
+        return discover_skeleton_gpu_discrete(memory_restriction=self.memory_restriction,
+                                              **self._common_arguments())

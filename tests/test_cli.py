@@ -8,8 +8,7 @@ import numpy as np
 import pytest
 
 import gpucsl_amd.cli.cli as cli
-from gpucsl_amd.cli.cli_io import (force_csv_suffix, get_file_name_stem, read_correlation_matrix,
-                                   write_results, write_results_and_config)
+from gpucsl_amd.cli.cli_io import ResultFiles, csv_path, dataset_stem, load_correlation, save_run, sepset_lines
 from gpucsl_amd.cli.cli_util import GpucslError
 
 
@@ -47,9 +46,9 @@ def stubbed(monkeypatch, tmp_path):
     _StubPC.calls = []
     monkeypatch.setattr(cli, "GaussianPC", _StubPC)
     monkeypatch.setattr(cli, "DiscretePC", _StubPC)
-    monkeypatch.setattr(cli, "read_csv", lambda path: np.zeros((10, 3)))
+    monkeypatch.setattr(cli, "load_table", lambda path: np.zeros((10, 3)))
     written = {}
-    monkeypatch.setattr(cli, "write_results_and_config", lambda *a: written.setdefault("args", a))
+    monkeypatch.setattr(cli, "save_run", lambda *a: written.setdefault("args", a))
     return written
 
 
@@ -90,14 +89,31 @@ def test_log_levels(stubbed, monkeypatch, flags, level):
 
 
 def test_path_helpers():
-    assert force_csv_suffix(Path("foobar/foo.csv")) == Path("foobar/foo.csv")
-    assert force_csv_suffix(Path("foobar/foo")) == Path("foobar/foo.csv")
-    assert get_file_name_stem("foobar/foo.csv") == "foo"
-    assert read_correlation_matrix(None) is None
+    assert csv_path(Path("foobar/foo.csv")) == Path("foobar/foo.csv")
+    assert csv_path(Path("foobar/foo")) == Path("foobar/foo.csv")
+    assert csv_path("foobar/foo.txt") == Path("foobar/foo.csv")
+    assert dataset_stem("foobar/foo.csv") == "foo"
+    assert load_correlation(None) is None
+
+
+def test_missing_dataset_is_an_error(tmp_path):
+    from gpucsl_amd.cli.cli_io import load_table
+
+    with pytest.raises(GpucslError):
+        load_table(tmp_path / "nothing_here")
+
+
+def test_sepset_lines_row_major():
+    sep = np.full((3, 3, 2), -1, np.int32)
+    sep[2, 0, :2] = [1, 2]
+    sep[0, 2, :2] = [1, 2]
+    sep[1, 2, 0] = 0
+    assert list(sepset_lines(sep)) == ["0 2 1 2", "1 2 0", "2 0 1 2"]
+    assert list(sepset_lines(np.zeros((3, 3, 0), np.int32))) == []
 
 
 def test_written_files(tmp_path):
-    write_results_and_config(str(tmp_path / "out"), "some/dir/ds.csv", ["-d", "ds", "--gaussian"], _Inner())
+    save_run(str(tmp_path / "out"), "some/dir/ds.csv", ["-d", "ds", "--gaussian"], _Inner())
     out = tmp_path / "out"
     assert (out / "ds_sepset.txt").read_text().splitlines() == ["0 2 1", "2 0 1"]
     rows = (out / "ds_pmax.csv").read_text().splitlines()
@@ -111,5 +127,5 @@ def test_written_files(tmp_path):
 def test_empty_sepsets_still_write_a_file(tmp_path):
     r = _Inner()
     r.separation_sets = np.zeros((3, 3, 0), np.int32)
-    write_results(tmp_path, "x", r)
+    ResultFiles(tmp_path, "x").write(r, ["-d", "x"])
     assert (tmp_path / "x_sepset.txt").read_text() == ""
