@@ -24,7 +24,12 @@ sys.path.insert(0, str(ROOT))
 METRIC = "CI tests/s + skeleton time, Gaussian p=1000 n=10k; p50/p99 request latency"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_MATRIX_TFLOPS = 157.3
 PEAK_INT8_TOPS = 5000.0
+KERNEL_TIMER = ("hipEvents recorded on the launch stream around every launch of the kernel "
+                "(eager pass of --profile-steps steps)")
+TRAFFIC_SOURCE = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, "
+                  "scaled by the counters' reading of a device copy of known size (not measured in this run)")
 GRAM_SMALL_P = 256      # float32 datasets above this take the integer Gram path
 I8_DIGIT_PAIRS = 26     # digit-plane products of that path (pairs a + b <= 6 of six digits)
 WORKLOADS = {
@@ -60,9 +65,22 @@ def load_traffic(kernel: str):
         return None
 
 
+def cpu_model() -> str:
+    """Host CPU model name (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(X: np.ndarray, n: int, alpha: float, max_level: int):
-    """The FP64 oracle (C restatement of the reference's algorithm, OpenMP) on
-    the host cores: correlation + skeleton of the same dataset."""
+    """The reference's CPU path on the host cores: the correlation matrix with
+    numpy's corrcoef (BLAS, as gpucsl/pc/helpers.py:58-59 computes it) and the
+    skeleton with the FP64 oracle (C restatement of the reference's algorithm,
+    OpenMP over all host cores), on the same dataset."""
     import oracle
 
     oracle.lib()
@@ -70,25 +88,26 @@ def cpu_baseline(X: np.ndarray, n: int, alpha: float, max_level: int):
 
     def once():
         t0 = time.perf_counter()
-        C = oracle.corrcoef(X64)
+        C = np.corrcoef(X64, rowvar=False)
         t1 = time.perf_counter()
         ref = oracle.gauss_skeleton(C, n, alpha, max_level)
         t2 = time.perf_counter()
         return t1 - t0, t2 - t1, int(ref["executed"].sum())
 
+    once()                                   # page in, spin up the thread pools
     tc, ts, tests = once()
     cores = oracle.num_threads()
     oracle.set_threads(1)
     try:
-        tc1, ts1, _ = once()
+        _, ts1, _ = once()
     finally:
         oracle.set_threads(0)
     return dict(value=tests / (tc + ts), unit="CI tests/s", cores=cores, kind="port",
-                sample=(f"full workload once: corrcoef {tc:.3f}s + skeleton {ts:.3f}s, "
-                        f"{tests} tests (FP64 oracle, OpenMP)"),
-                seconds=tc + ts, tests=tests, skeleton_only=tests / ts,
-                single_thread={"value": tests / (tc1 + ts1), "skeleton_only": tests / ts1,
-                               "corrcoef_s": tc1, "skeleton_s": ts1})
+                sample=(f"full workload once: numpy corrcoef {tc:.3f}s + oracle skeleton {ts:.3f}s "
+                        f"({cores} OpenMP threads), {tests} tests"),
+                cpu_model=cpu_model(), os_cpu_count=os.cpu_count(),
+                seconds=tc + ts, corrcoef_s=tc, skeleton_s=ts, tests=tests, skeleton_only=tests / ts,
+                single_thread_skeleton={"value": tests / ts1, "skeleton_s": ts1})
 
 
 def int8_gram(n: int, p: int, f32: bool) -> bool:
@@ -103,18 +122,33 @@ def roofline(timings, profile_steps, stats, shapes, workload):
     avg_s = tot / cnt / 1e3
     launches_per_step = cnt / profile_steps
     if dominant == "corr_gram_i8":
-        # p(p+1)/2 distinct entries x n rows x 26 digit products x 2 int8 ops
+        # algorithmic work of the correlation GEMM: 2 n p^2 flop per dataset,
+        # against the FP32 matrix peak (the precision the GEMM is specified
+        # at; this kernel's result is FP64-accurate)
+        flops = sum(2.0 * n * p * p for n, p, f in shapes if int8_gram(n, p, f)) / launches_per_step
+        achieved = flops / avg_s / 1e12
+        # what the int8 matrix pipe issues: p(p+1)/2 distinct entries x n
+        # rows x 26 digit products x 2 int8 ops
         ops = sum(I8_DIGIT_PAIRS * n * p * (p + 1.0) for n, p, f in shapes if int8_gram(n, p, f)) / launches_per_step
-        achieved = ops / avg_s / 1e12
-        fp64_eq = sum(1.0 * n * p * (p + 1) for n, p, f in shapes if int8_gram(n, p, f)) / launches_per_step
-        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_INT8_TOPS, unit="TOP/s (int8)",
-                    frac=achieved / PEAK_INT8_TOPS, fp64_equivalent_tflops=fp64_eq / avg_s / 1e12)
+        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_MATRIX_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP32_MATRIX_TFLOPS,
+                    work="2 n p^2 flop per dataset (algorithmic), FP32 matrix peak",
+                    int8_pipe={"achieved": ops / avg_s / 1e12, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                               "frac": ops / avg_s / 1e12 / PEAK_INT8_TOPS,
+                               "work": "26 digit products x 2 ops x n p (p+1)/2 entries per dataset (issued)"})
     elif dominant.startswith("corr_gram"):
-        # symmetric rank-n update: n * p * (p + 1) flops for the p(p+1)/2 distinct entries
-        flops = sum(1.0 * n * p * (p + 1) for n, p, f in shapes if not int8_gram(n, p, f)) / launches_per_step
+        # FP64 MFMA path (float64 data, p <= 256): 2 n p^2 algorithmic flop
+        # per dataset against the FP64 matrix peak; the kernel issues the
+        # symmetric half, n p (p + 1)
+        flops = sum(2.0 * n * p * p for n, p, f in shapes if not int8_gram(n, p, f)) / launches_per_step
+        issued = sum(1.0 * n * p * (p + 1) for n, p, f in shapes if not int8_gram(n, p, f)) / launches_per_step
         achieved = flops / avg_s / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS)
+                    frac=achieved / PEAK_FP64_TFLOPS,
+                    work="2 n p^2 flop per dataset (algorithmic), FP64 matrix peak",
+                    fp64_pipe={"achieved": issued / avg_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                               "frac": issued / avg_s / 1e12 / PEAK_FP64_TFLOPS,
+                               "work": "n p (p + 1) flop per dataset (upper triangle, issued)"})
     elif dominant.startswith("gauss_level") and dominant.endswith("_fast"):
         lv = int(dominant[len("gauss_level")])
         # tests of this level per step / launches of this kernel per step = per launch
@@ -126,10 +160,12 @@ def roofline(timings, profile_steps, stats, shapes, workload):
         roof = dict(bound="hbm", achieved=None, peak=PEAK_HBM_GBS, unit="GB/s", frac=None)
     roof["kernel"] = dominant
     roof["avg_launch_ms"] = avg_s * 1e3
+    roof["timer"] = KERNEL_TIMER
     # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this
     # workload (keys "kernel@workload"; bare kernel names are the default workload)
     roof["traffic"] = load_traffic(f"{dominant}@{workload}") or (
         load_traffic(dominant) if workload == "gaussian_p1000_n10000" else None)
+    roof["traffic_source"] = TRAFFIC_SOURCE if roof["traffic"] is not None else None
     return roof
 
 
@@ -195,8 +231,10 @@ def bench_discrete(args):
     avg_s = tot / cnt / 1e3
     achieved = discrete_bytes(dominant, stats, n, cnt / args.profile_steps) / avg_s / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s", frac=achieved / PEAK_HBM_GBS,
-                kernel=dominant, avg_launch_ms=avg_s * 1e3,
+                kernel=dominant, avg_launch_ms=avg_s * 1e3, timer=KERNEL_TIMER,
+                work="(l + 2) n B per level-l test (uint8 columns i, j and S)",
                 traffic=load_traffic(f"{dominant}@{args.workload}"))
+    roof["traffic_source"] = TRAFFIC_SOURCE if roof["traffic"] is not None else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -219,6 +257,7 @@ def bench_discrete(args):
         finally:
             oracle.set_threads(0)
         cpu = dict(value=tests / cs_, unit="CI tests/s", cores=oracle.num_threads(), kind="port",
+                   cpu_model=cpu_model(),
                    sample=f"full workload, {reps} runs of {cs_:.3f}s, {tests} tests each (C oracle, OpenMP)",
                    single_thread={"value": tests / cs1, "seconds": cs1})
 
@@ -268,7 +307,8 @@ def batched_datasets(n_boot: int, n_sets: int):
     return base.astype(np.float32), rows, sets
 
 
-def bench_batched(args):
+def measure_batched(n_boot: int, n_sets: int, steps: int, warmup: int, profile_steps: int, workload: str,
+                    with_cpu: bool) -> dict:
     """Many datasets in shared launches (one captured graph per step) against
     the same datasets one after another through the single-dataset path."""
     import torch
@@ -277,8 +317,8 @@ def bench_batched(args):
     from gpucsl_amd.batch import GaussJobState, GaussianBatch, run_plain
 
     alpha, L = 0.01, 3
-    log(f"[bench] workload {args.workload}: generating {args.bootstraps} resamples + {args.datasets} datasets")
-    base, rows, sets = batched_datasets(args.bootstraps, args.datasets)
+    log(f"[bench] batched: generating {n_boot} resamples + {n_sets} datasets")
+    base, rows, sets = batched_datasets(n_boot, n_sets)
     torch.cuda.set_device(0)
     base_dev = torch.from_numpy(base).cuda()
     jobs, shapes = [], []
@@ -295,27 +335,27 @@ def bench_batched(args):
         shapes.append((n, p, True))
     torch.cuda.synchronize()
     hbm = sum(j.nbytes for j in jobs)
-    log(f"[bench] {len(jobs)} jobs, {hbm / 2**30:.2f} GiB of job state")
+    log(f"[bench] batched: {len(jobs)} jobs, {hbm / 2**30:.2f} GiB of job state")
     batch = GaussianBatch(jobs)
     batch.capture()
 
-    log(f"[bench] batched: warmup {args.warmup}, timing {args.steps} graph replays")
-    for _ in range(args.warmup):
+    log(f"[bench] batched: warmup {warmup}, timing {steps} graph replays")
+    for _ in range(warmup):
         batch.run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.run()
     torch.cuda.synchronize()
-    sec = (time.perf_counter() - t0) / max(args.steps, 1)
+    sec = (time.perf_counter() - t0) / max(steps, 1)
     stats = batch.batch_stats().levels
     useful = sum(s["useful"] for s in stats)
     executed = sum(s["executed"] for s in stats)
     batched_results = [(j.view("skeleton", torch.int16, (j.p, j.p)).clone(),
                         j.view("sepsets", torch.int32, (j.p, j.p, max(j.max_level, 1))).clone()) for j in jobs]
 
-    plain_steps = max(1, min(args.steps, 3))
-    log(f"[bench] plain single-dataset path: {plain_steps} passes over the {len(jobs)} jobs")
+    plain_steps = max(1, min(steps, 3))
+    log(f"[bench] batched: plain single-dataset path, {plain_steps} passes over the {len(jobs)} jobs")
     run_plain(jobs)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -327,40 +367,55 @@ def bench_batched(args):
                torch.equal(b, j.view("sepsets", torch.int32, (j.p, j.p, max(j.max_level, 1))))
                for (a, b), j in zip(batched_results, jobs))
 
-    log(f"[bench] profiled pass {args.profile_steps} eager batched steps")
+    log(f"[bench] batched: profiled pass {profile_steps} eager steps")
     with nat.KernelTimer() as kt:
-        for _ in range(args.profile_steps):
+        for _ in range(profile_steps):
             batch.launch()
         torch.cuda.synchronize()
     timings = kt.timings
-    roof = roofline(timings, args.profile_steps, stats, shapes, args.workload)
+    roof = roofline(timings, profile_steps, stats, shapes, workload)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if with_cpu:
         import oracle
 
         k = 4
-        log(f"[bench] cpu baseline: oracle on {k} resamples")
+        log(f"[bench] batched: cpu baseline, numpy corrcoef + oracle skeleton on {k} resamples")
         oracle.lib()
         t0 = time.perf_counter()
         tests = 0
         for r in rows[:k]:
             X = base[r].astype(np.float64)
-            ref = oracle.gauss_skeleton(oracle.corrcoef(X), 5000, alpha, L)
+            ref = oracle.gauss_skeleton(np.corrcoef(X, rowvar=False), 5000, alpha, L)
             tests += int(ref["executed"].sum())
         cs = time.perf_counter() - t0
         cpu = dict(value=tests / cs, unit="CI tests/s", cores=oracle.num_threads(), kind="port",
-                   sample=f"{k} bootstrap resamples (p=200, n=5000, L=3): corrcoef + skeleton, {tests} tests "
-                          f"in {cs:.2f}s (FP64 oracle, OpenMP)")
+                   cpu_model=cpu_model(),
+                   sample=f"{k} bootstrap resamples (p=200, n=5000, L=3): numpy corrcoef + oracle skeleton, "
+                          f"{tests} tests in {cs:.2f}s")
+    del batch
+    return dict(sec=sec, useful=useful, executed=executed, stats=stats, timings=timings, roof=roof, cpu=cpu,
+                jobs=len(jobs), hbm=hbm, plain_sec=plain_sec, same=bool(same))
 
+
+def batched_summary(m: dict) -> dict:
+    return {"ms_per_step": m["sec"] * 1e3, "ci_tests_per_s": m["useful"] / m["sec"],
+            "plain_ms_per_pass": m["plain_sec"] * 1e3, "plain_ci_tests_per_s": m["useful"] / m["plain_sec"],
+            "batched_over_plain": m["plain_sec"] / m["sec"], "results_identical": m["same"], "jobs": m["jobs"],
+            "dominant_kernel": m["roof"]["kernel"], "dominant_kernel_frac": m["roof"]["frac"]}
+
+
+def bench_batched(args):
+    m = measure_batched(args.bootstraps, args.datasets, args.steps, args.warmup, args.profile_steps,
+                        args.workload, not args.no_cpu_baseline)
     out = {
         "metric": METRIC,
-        "value": useful / sec,
+        "value": m["useful"] / m["sec"],
         "unit": "CI tests/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec * 1e3,
+        "ms_per_step": m["sec"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -368,15 +423,15 @@ def bench_batched(args):
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {args.bootstraps} bootstrap resamples of a p=200 n=5000 "
                                f"dataset + {args.datasets} datasets p~U[50,500] n~U[1000,20000], degree 3, "
-                               f"float32 data, alpha={alpha}, max_level={L}; one captured graph per step",
-                   "jobs": len(jobs), "job_state_gib": hbm / 2**30, "parallelism": "single GPU"},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "plain_path": {"ms_per_pass": plain_sec * 1e3, "ci_tests_per_s": useful / plain_sec,
-                       "batched_over_plain": plain_sec / sec, "results_identical": bool(same)},
-        "tests_per_step": {"useful": useful, "executed": executed},
-        "per_level": stats,
-        "kernel_ms_per_step": {k: v[1] / args.profile_steps for k, v in timings.items()},
+                               f"float32 data, alpha=0.01, max_level=3; one captured graph per step",
+                   "jobs": m["jobs"], "job_state_gib": m["hbm"] / 2**30, "parallelism": "single GPU"},
+        "roofline": m["roof"],
+        "cpu_baseline": m["cpu"],
+        "plain_path": {"ms_per_pass": m["plain_sec"] * 1e3, "ci_tests_per_s": m["useful"] / m["plain_sec"],
+                       "batched_over_plain": m["plain_sec"] / m["sec"], "results_identical": m["same"]},
+        "tests_per_step": {"useful": m["useful"], "executed": m["executed"]},
+        "per_level": m["stats"],
+        "kernel_ms_per_step": {k: v[1] / args.profile_steps for k, v in m["timings"].items()},
     }
     print(json.dumps(out), flush=True)
 
@@ -403,19 +458,19 @@ def serving_requests(count: int, seed: int = 500):
     return reqs
 
 
-def bench_serving(args):
+def measure_serving(n_requests: int, load: float) -> dict:
     """Closed loop (every request at t=0) for capacity, the same requests one
     by one through the single-dataset path, then an open-loop Poisson stream at
-    --load x capacity for p50/p99 latency."""
+    load x capacity for p50/p99 latency."""
     import torch
 
     from gpucsl_amd import _native as nat
     from gpucsl_amd.serving import SkeletonServer, run_alone
 
     torch.cuda.set_device(0)
-    log(f"[bench] workload {args.workload}: generating {args.requests} requests on the device")
+    log(f"[bench] serving: generating {n_requests} requests on the device")
     t0 = time.perf_counter()
-    reqs = serving_requests(args.requests)
+    reqs = serving_requests(n_requests)
     warm = serving_requests(6, seed=900)
     torch.cuda.synchronize()
     log(f"[bench] generated in {time.perf_counter() - t0:.1f}s; "
@@ -497,7 +552,7 @@ def bench_serving(args):
 
     # the open loop delivers results into pinned host memory: its load is a
     # fraction of the capacity measured that way
-    rate = args.load * len(reqs) / makespan_p
+    rate = load * len(reqs) / makespan_p
     rng = np.random.Generator(np.random.PCG64(501))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
     log(f"[bench] open loop: Poisson arrivals at {rate:.2f} requests/s")
@@ -514,51 +569,100 @@ def bench_serving(args):
     timings = kt.timings
     dominant = max(timings, key=lambda k: timings[k][1])
 
+    return dict(reqs=len(reqs), useful=useful, makespan=makespan, capacity=capacity, srv_ticks=srv.ticks,
+                max_batch=srv.max_batch, host_tick_s=srv.host_tick_s, host_time=srv.host_time,
+                makespan_p=makespan_p, makespan_c=makespan_c, plain_gpu_sec=plain_gpu_sec,
+                plain_host_sec=plain_host_sec, identical=identical, rate=rate, load=load, span2=span2,
+                useful2=useful2, lat=lat, lat_g=lat_g, lat_d=lat_d, timings=timings, dominant=dominant,
+                n_gauss=sum(r.kind == "gaussian" for r in reqs), n_disc=sum(r.kind == "discrete" for r in reqs))
+
+
+def _pct(a, q):
+    return float(np.percentile(a, q) * 1e3) if a.size else None
+
+
+def serving_summary(m: dict) -> dict:
+    return {"requests": m["reqs"], "gaussian": m["n_gauss"], "discrete": m["n_disc"],
+            "latency_p50_ms": _pct(m["lat"], 50), "latency_p99_ms": _pct(m["lat"], 99),
+            "open_loop_rate_per_s": m["rate"], "open_loop_load": m["load"],
+            "served_over_plain_with_host_results": m["plain_host_sec"] / m["makespan_c"],
+            "served_over_plain_results_on_device": m["plain_gpu_sec"] / m["makespan"],
+            "served_ci_tests_per_s": m["useful"] / m["makespan"],
+            "results_identical": f"{m['identical']}/{m['reqs']}"}
+
+
+def bench_serving(args):
+    m = measure_serving(args.requests, args.load)
+    lat, lat_g, lat_d = m["lat"], m["lat_g"], m["lat_d"]
     out = {
         "metric": METRIC,
-        "value": useful / makespan,
+        "value": m["useful"] / m["makespan"],
         "unit": "CI tests/s",
         "n_gpus": 1,
         "steps": 1,
         "warmup": 1,
-        "ms_per_step": makespan * 1e3,
+        "ms_per_step": m["makespan"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {len(reqs)} requests, 70% Gaussian p~U[20,2000] "
+        "config": {"workload": f"{args.workload}: {m['reqs']} requests, 70% Gaussian p~U[20,2000] "
                                f"n~U[500,50000] degree 3, 30% discrete p~U[20,300] card 2-5 n~U[1000,100000] "
                                f"degree 2, max_level 3, alpha 0.01, seed 500; HBM-footprint admission, "
                                f"continuous batching by level",
                    "parallelism": "single GPU"},
-        "closed_loop": {"makespan_s": makespan, "requests_per_s": capacity, "ci_tests_per_s": useful / makespan,
-                        "ticks": srv.ticks, "largest_gaussian_batch": srv.max_batch,
-                        "host_seconds_in_tick": srv.host_tick_s, "host_seconds_by_phase": srv.host_time,
+        "closed_loop": {"makespan_s": m["makespan"], "requests_per_s": m["capacity"],
+                        "ci_tests_per_s": m["useful"] / m["makespan"],
+                        "ticks": m["srv_ticks"], "largest_gaussian_batch": m["max_batch"],
+                        "host_seconds_in_tick": m["host_tick_s"], "host_seconds_by_phase": m["host_time"],
                         "results": "left on the device, released on arrival",
-                        "makespan_results_in_pinned_host_memory_s": makespan_p,
-                        "makespan_with_owned_host_copies_s": makespan_c},
-        "plain_path": {"seconds_results_on_device": plain_gpu_sec,
-                       "seconds_results_on_host": plain_host_sec,
-                       "ci_tests_per_s": useful / plain_gpu_sec,
-                       "served_over_plain": plain_gpu_sec / makespan,
-                       "served_over_plain_with_host_results": plain_host_sec / makespan_c,
-                       "results_identical": f"{identical}/{len(reqs)}"},
-        "open_loop": {"arrival_rate_per_s": rate, "load": args.load, "span_s": span2,
-                      "throughput_requests_per_s": len(reqs) / span2, "ci_tests_per_s": useful2 / span2,
-                      "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
-                      "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
-                      "gaussian_p50_ms": float(np.percentile(lat_g, 50) * 1e3) if lat_g.size else None,
-                      "gaussian_p99_ms": float(np.percentile(lat_g, 99) * 1e3) if lat_g.size else None,
-                      "discrete_p50_ms": float(np.percentile(lat_d, 50) * 1e3) if lat_d.size else None,
-                      "discrete_p99_ms": float(np.percentile(lat_d, 99) * 1e3) if lat_d.size else None},
-        "roofline": {"kernel": dominant, "bound": None, "achieved": None, "peak": None, "unit": None,
+                        "makespan_results_in_pinned_host_memory_s": m["makespan_p"],
+                        "makespan_with_owned_host_copies_s": m["makespan_c"]},
+        "plain_path": {"seconds_results_on_device": m["plain_gpu_sec"],
+                       "seconds_results_on_host": m["plain_host_sec"],
+                       "ci_tests_per_s": m["useful"] / m["plain_gpu_sec"],
+                       "served_over_plain": m["plain_gpu_sec"] / m["makespan"],
+                       "served_over_plain_with_host_results": m["plain_host_sec"] / m["makespan_c"],
+                       "results_identical": f"{m['identical']}/{m['reqs']}"},
+        "open_loop": {"arrival_rate_per_s": m["rate"], "load": m["load"], "span_s": m["span2"],
+                      "throughput_requests_per_s": m["reqs"] / m["span2"],
+                      "ci_tests_per_s": m["useful2"] / m["span2"],
+                      "latency_p50_ms": _pct(lat, 50), "latency_p99_ms": _pct(lat, 99),
+                      "gaussian_p50_ms": _pct(lat_g, 50), "gaussian_p99_ms": _pct(lat_g, 99),
+                      "discrete_p50_ms": _pct(lat_d, 50), "discrete_p99_ms": _pct(lat_d, 99)},
+        "roofline": {"kernel": m["dominant"], "bound": None, "achieved": None, "peak": None, "unit": None,
                      "frac": None, "traffic": None,
                      "note": "mixed workload; per-kernel rooflines are in the single-dataset lines"},
         "cpu_baseline": None,
-        "plain_path_kernel_ms": {k: v[1] for k, v in sorted(timings.items(), key=lambda x: -x[1][1])[:14]},
+        "plain_path_kernel_ms": {k: v[1] for k, v in sorted(m["timings"].items(), key=lambda x: -x[1][1])[:14]},
     }
     print(json.dumps(out), flush=True)
+
+
+def dropin_execute(data: np.ndarray, L: int, alpha: float, reps: int = 3) -> dict:
+    """The drop-in API as the reference's benchmark times it
+    (benchmarks/benchmark_gpucsl.py:104-127): the correlation matrix computed
+    beforehand on the host, then GaussianPC(...).set_distribution_specific_options(
+    devices, None, C).execute() wall time, which includes the C upload, the
+    kernels, the result copies to the host and the edge orientation."""
+    from gpucsl_amd.pc.pc import GaussianPC
+
+    C = np.corrcoef(data.astype(np.float64), rowvar=False)
+    runs = []
+    for k in range(reps + 1):
+        pc = GaussianPC(data, L, alpha).set_distribution_specific_options([0], None, C)
+        res, wall = pc.execute()
+        if k:
+            runs.append((wall, res.discover_skeleton_runtime, res.edge_orientation_runtime,
+                         res.discover_skeleton_kernel_runtime))
+    runs.sort()
+    w, sk, orient, kern = runs[len(runs) // 2]
+    log(f"[bench] drop-in GaussianPC.execute(): {w * 1e3:.1f} ms (median of {reps})")
+    return {"wall_ms": w * 1e3, "discover_skeleton_ms": sk * 1e3, "edge_orientation_ms": orient * 1e3,
+            "skeleton_computation_ms": kern * 1e3, "runs": reps,
+            "note": "median wall time of execute() with a host correlation matrix; includes H2D of C, "
+                    "the kernels, D2H of skeleton/sepsets/pmax and the orientation"}
 
 
 def main():
@@ -575,6 +679,10 @@ def main():
     ap.add_argument("--datasets", type=int, default=64, help="batched workload: uneven datasets")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload only: skip the drop-in, batched and serving sub-measurements")
+    ap.add_argument("--serving-requests", type=int, default=48,
+                    help="requests of the serving sub-measurement in the default line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -647,10 +755,38 @@ def main():
     per_step_kernel_ms = {k: v[1] / args.profile_steps for k, v in timings.items()}
     roof = roofline(timings, args.profile_steps, stats, [(n, p, dtype == np.float32)], args.workload)
 
+    # the skeleton alone (correlation matrix resident in HBM, as the
+    # reference's benchmark excludes it, benchmarks/benchmark_gpucsl.py:104-117)
+    eng_sk = GaussianEngine(p, n, L, thr)
+    eng_sk.C32.copy_(eng.C32)
+    eng_sk.C64.copy_(eng.C64)
+    eng_sk.capture()
+    for _ in range(args.warmup):
+        eng_sk.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng_sk.replay()
+    torch.cuda.synchronize()
+    sk_sec = (time.perf_counter() - t0) / max(args.steps, 1)
+    del eng_sk
+
+    dropin = None
+    if args.workload == "gaussian_p1000_n10000" and not args.no_extras:
+        dropin = dropin_execute(data, L, alpha)
+
     cpu = None
     if not args.no_cpu_baseline:
-        log("[bench] cpu baseline (oracle)")
+        log("[bench] cpu baseline (numpy corrcoef + oracle skeleton)")
         cpu = cpu_baseline(data, n, alpha, L)
+
+    batched = serving = None
+    if args.workload == "gaussian_p1000_n10000" and not args.no_extras:
+        batched = batched_summary(measure_batched(256, 64, 5, 2, 2, "batched_bootstrap256_p200_n5000_plus64",
+                                                  False))
+        torch.cuda.empty_cache()
+        serving = serving_summary(measure_serving(args.serving_requests, args.load))
+        torch.cuda.empty_cache()
 
     out = {
         "metric": METRIC,
@@ -674,7 +810,11 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "skeleton_ms": sec * 1e3,
+        "skeleton_ms": sk_sec * 1e3,
+        "skeleton_ms_note": "levels 0..L + pmax with C resident (correlation excluded); ms_per_step includes it",
+        "dropin_execute": dropin,
+        "batched": batched,
+        "serving": serving,
         "tests_per_step": {"useful": useful, "executed": executed},
         "gpu_vs_cpu": (useful / sec) / cpu["value"] if cpu else None,
         "per_level": stats,

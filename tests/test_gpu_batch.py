@@ -207,3 +207,50 @@ def test_mixed_inputs_rejected():
     c.set_correlation(np.eye(10))
     with pytest.raises(GpucslError):
         GaussianBatch([a, c]).launch()
+
+
+def test_full_c4_batch_equals_alone_and_oracle():
+    """The whole batched workload: 256 bootstrap resamples of the p=200,
+    n=5000 dataset plus the 64 uneven datasets (p ~ U[50, 500], n ~ U[1000,
+    20000]), max level 3, in one batch (the batched benchmark's jobs).  Every
+    one of the 320 jobs equals the same job through the single-dataset path,
+    bit for bit; five jobs, two of them above 256 columns (integer Gram path),
+    equal the FP64 oracle's skeleton and separation sets."""
+    import bench
+    from gpucsl_amd.batch import GaussJobState, GaussianBatch, run_plain
+
+    base, rows, sets = bench.batched_datasets(256, 64)
+    base_dev = torch.from_numpy(base).cuda()
+    jobs, plain, xs = [], [], []
+    for r in rows:
+        X = base_dev[torch.from_numpy(r).cuda()]
+        xs.append((base, r))
+        for lst in (jobs, plain):
+            j = GaussJobState(5000, 200, 3, alpha=0.01)
+            j.set_data(X)
+            lst.append(j)
+    for X in sets:
+        n, p = X.shape
+        xs.append((X, None))
+        for lst in (jobs, plain):
+            j = GaussJobState(n, p, 3, alpha=0.01)
+            j.set_data(X)
+            lst.append(j)
+    torch.cuda.synchronize()
+    say(f"c4: {len(jobs)} jobs, {sum(j.nbytes for j in jobs) / 2**30:.2f} GiB of job state per copy")
+    bt = GaussianBatch(jobs)
+    bt.launch()
+    run_plain(plain)
+    torch.cuda.synchronize()
+    got = bt.results()
+    for k, (a, j) in enumerate(zip(got, plain)):
+        assert _same(a, j.results(), f"c4 job {k}"), f"job {k} differs from its run alone"
+    big = [256 + k for k, X in enumerate(sets) if X.shape[1] > 256]
+    assert len(big) >= 2
+    for k in [0, 255, 256, big[0], big[1]]:
+        X, r = xs[k]
+        X64 = (X[r] if r is not None else X).astype(np.float64)
+        n = X64.shape[0]
+        ref = oracle.gauss_skeleton(oracle.corrcoef(X64), n, 0.01, 3)
+        assert (got[k].skeleton == ref["skeleton"]).all() and (got[k].separation_sets == ref["sepsets"]).all(), k
+        say(f"  c4 job {k} p={X64.shape[1]} n={n} equals the oracle; edges {int(got[k].skeleton.sum()) // 2}")

@@ -200,3 +200,45 @@ def test_served_float64_request_keeps_float64():
     ref = oracle.gauss_skeleton(oracle.corrcoef(X), 2000, 0.01, 3)
     assert (served.skeleton == ref["skeleton"]).all()
     assert (served.separation_sets == ref["sepsets"]).all()
+
+
+def test_serving_stream_at_full_sizes():
+    """The serving workload's request mix (32 requests of the seeded stream:
+    70 % Gaussian p ~ U[20, 2000], n ~ U[500, 50000]; 30 % discrete p ~ U[20,
+    300], n ~ U[1000, 100000]) plus a Gaussian p=2000, n=50000 and a discrete
+    p=300, n=100000 request, all in flight together: every result is the same
+    bits as the request alone; three Gaussian requests with p <= 310 and the
+    discrete ones with p <= 110 equal the FP64 oracle."""
+    import bench
+    from gpucsl_amd.serving import Request, SkeletonServer
+
+    reqs = bench.serving_requests(32, seed=500)
+    reqs.append(Request("gaussian", 50000, 2000, 3, 0.01, X=syn.gaussian_sem_device(2000, 50000, 3.0, seed=777)))
+    codes, dims = syn.discrete_bn_device(300, 100000, 2.0, seed=778)
+    reqs.append(Request("discrete", 100000, 300, 3, 0.01, codes=codes, dims=dims))
+    torch.cuda.synchronize()
+    say(f"stream: {sum(r.kind == 'gaussian' for r in reqs)} Gaussian, {sum(r.kind == 'discrete' for r in reqs)} "
+        f"discrete; largest Gaussian p={max(r.p for r in reqs if r.kind == 'gaussian')}")
+    srv = SkeletonServer(copy_results=True)
+    for r in reqs:
+        srv.submit(r)
+    done = srv.drain()
+    assert len(done) == len(reqs) and srv.idle
+    by_id = {c.rid: c for c in done}
+    checked = 0
+    for r in reqs:
+        served = by_id[r.rid].result
+        assert _same(served, _alone(r), r.kind), f"request {r.rid} ({r.kind} p={r.p} n={r.n}) differs"
+        if r.kind == "gaussian" and r.p <= 310 and checked < 3:
+            X = r.X.double().cpu().numpy()
+            ref = oracle.gauss_skeleton(oracle.corrcoef(X), r.n, 0.01, r.max_level)
+            assert (served.skeleton == ref["skeleton"]).all() and (served.separation_sets == ref["sepsets"]).all()
+            checked += 1
+        elif r.kind == "discrete" and r.p <= 110 and checked < 5:
+            dims_h = r.dims.cpu().numpy().astype(np.uint8)
+            col = np.ascontiguousarray(r.codes.cpu().numpy()[:, : r.n])
+            ref = oracle.disc_skeleton(col, dims_h, 0.01, r.max_level)
+            assert (served.skeleton == ref["skeleton"]).all() and (served.separation_sets == ref["sepsets"]).all()
+            checked += 1
+    say(f"stream: {len(reqs)} served equal alone; {checked} checked against the oracle; "
+        f"{srv.ticks} ticks, largest batch {srv.max_batch}")
