@@ -25,7 +25,7 @@ from gpucsl_amd import _native as nat
 from gpucsl_amd.cli.cli_util import GpucslError
 from gpucsl_amd.gaussian import DEFAULT_GROUP_SIZE, DEFAULT_MARGIN, GaussianResult, thresholds_for_levels
 
-MAX_LEVELS = 9          # GPUCSL_MAX_LEVELS: thresholds for levels 0..8
+MAX_LEVELS = 33         # GPUCSL_MAX_LEVELS: thresholds for levels 0..32 (GPUCSL_MAX_LEVEL)
 ALIGN = 256
 
 

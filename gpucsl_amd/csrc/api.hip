@@ -133,6 +133,15 @@ __global__ void unrank_kernel(int N, const uint64_t *__restrict__ ranks, int cou
     for (int k = 0; k < L; ++k) out[(size_t)t * L + k] = pos[k];
 }
 
+__global__ void unrank_rt_kernel(int N, int l, const uint64_t *__restrict__ ranks, int count,
+                                 int32_t *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int pos[GPUCSL_MAX_LEVEL];
+    unrank_lex_rt(N, l, ranks[t], pos);
+    for (int k = 0; k < l; ++k) out[(size_t)t * l + k] = pos[k];
+}
+
 // nbr/deg (library layout) -> reference Gc layout [deg, nbrs..., 0...]
 __global__ void gc_layout_kernel(const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg, int p,
                                  int32_t *__restrict__ Gc) {
@@ -158,8 +167,8 @@ static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); 
 // neighbours for them: compact once more, read the max degree (one host sync).
 template <typename CompactFn>
 static int check_unbuilt_levels(int max_level, uint64_t *stats, CompactFn compact, hipStream_t s) {
-    if (max_level <= GPUCSL_MAX_LEVEL_TEMPLATE) return OK;
-    const int next = GPUCSL_MAX_LEVEL_TEMPLATE + 1;
+    if (max_level <= GPUCSL_MAX_LEVEL) return OK;
+    const int next = GPUCSL_MAX_LEVEL + 1;
     unsigned long long *st = reinterpret_cast<unsigned long long *>(stats) + (size_t)next * GPUCSL_STAT_SLOTS;
     int rc = compact(next, st);
     if (rc) return rc;
@@ -170,7 +179,7 @@ static int check_unbuilt_levels(int max_level, uint64_t *stats, CompactFn compac
     if (rc) return rc;
     if ((long long)maxdeg - 1 >= next)
         return set_error(ERR_UNSUPPORTED, "level %d has work (max degree %llu) but only levels <= %d are built", next,
-                         maxdeg, GPUCSL_MAX_LEVEL_TEMPLATE);
+                         maxdeg, GPUCSL_MAX_LEVEL);
     return OK;
 }
 
@@ -180,7 +189,7 @@ const char *gpucsl_last_error(void) { return g_err; }
 
 int gpucsl_abi_version(void) { return GPUCSL_ABI_VERSION; }
 
-int gpucsl_max_built_level(void) { return GPUCSL_MAX_LEVEL_TEMPLATE; }
+int gpucsl_max_built_level(void) { return GPUCSL_MAX_LEVEL; }
 
 int gpucsl_device_info(int *compute_units, size_t *free_bytes, size_t *total_bytes) {
     int dev = 0;
@@ -327,7 +336,7 @@ int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss
     if ((rc = gauss_batch_level0(jobs, jobs_dev, nj, margin, reinterpret_cast<unsigned long long *>(stats),
                                  S(stream))))
         return rc;
-    const int built = max_level < GPUCSL_MAX_LEVEL_TEMPLATE ? max_level : GPUCSL_MAX_LEVEL_TEMPLATE;
+    const int built = max_level < GPUCSL_MAX_LEVEL ? max_level : GPUCSL_MAX_LEVEL;
     for (int l = 1; l <= built; ++l) {
         rc = gauss_batch_level(l, jobs, jobs_dev, nj, margin, group_size,
                                reinterpret_cast<unsigned long long *>(stats) + (size_t)l * GPUCSL_STAT_SLOTS, ws,
@@ -401,7 +410,7 @@ int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const u
     rc = gpucsl_disc_level0(data, n, stride, dims, p, max_level, max_dim, alpha, G, pmax, stats, ws, ws_bytes,
                             stream);
     if (rc) return rc;
-    const int built = max_level < GPUCSL_MAX_LEVEL_TEMPLATE ? max_level : GPUCSL_MAX_LEVEL_TEMPLATE;
+    const int built = max_level < GPUCSL_MAX_LEVEL ? max_level : GPUCSL_MAX_LEVEL;
     for (int l = 1; l <= built; ++l) {
         rc = gpucsl_disc_level(l, data, n, stride, dims, p, max_level, max_dim, alpha, G, sepsets, pmax,
                                stats + (size_t)l * GPUCSL_STAT_SLOTS, ws, ws_bytes, stream);
@@ -512,7 +521,10 @@ int gpucsl_unrank(int N, int l, const uint64_t *ranks, int count, int32_t *out, 
         break;
         GPUCSL_U(1) GPUCSL_U(2) GPUCSL_U(3) GPUCSL_U(4) GPUCSL_U(5) GPUCSL_U(6) GPUCSL_U(7) GPUCSL_U(8)
 #undef GPUCSL_U
-    default: return set_error(ERR_ARG, "unrank: l must be in 1..8");
+    default:
+        if (l < 1 || l > GPUCSL_MAX_LEVEL) return set_error(ERR_ARG, "unrank: l must be in 1..%d", GPUCSL_MAX_LEVEL);
+        hipLaunchKernelGGL(unrank_rt_kernel, grid, dim3(256), 0, S(stream), N, l, ranks, count, out);
+        break;
     }
     return check_hip(hipGetLastError(), "unrank launch");
 }

@@ -5,6 +5,10 @@
 
 #include <cstddef>
 
+#include "../../include/gpucsl_hip.h"
+
+// levels 1..GPUCSL_MAX_LEVEL_TEMPLATE have kernels specialised per level;
+// levels above run kernels that take the level at run time
 #define GPUCSL_MAX_LEVEL_TEMPLATE 8
 
 namespace gpucsl {
@@ -149,6 +153,25 @@ __device__ __forceinline__ void unrank_step_tab(int N, uint64_t r, int prev, int
 template <int L>
 __device__ __forceinline__ void unrank_lex_tab(int N, uint64_t r, int (&pos)[L], const uint64_t *tab, int stride) {
     unrank_step_tab<L, 0>(N, r, 0, pos, tab, stride);
+}
+
+// The same unranking for a run-time l (levels above the templated range):
+// position c takes the smallest value v > pos[c-1] for which the
+// combinations that start with values <= v at position c cover the rank; the
+// block of combinations with value v there is C(N - v, l - c - 1).
+__host__ __device__ inline void unrank_lex_rt(int N, int l, uint64_t r, int *pos) {
+    int prev = 0;
+    for (int c = 0; c < l; ++c) {
+        int v = prev + 1;
+        for (;;) {
+            const uint64_t block = binom(N - v, l - c - 1);
+            if (r < block || v >= N) break;
+            r -= block;
+            ++v;
+        }
+        pos[c] = v;
+        prev = v;
+    }
 }
 
 // ----------------------------------------------------------------------------

@@ -561,6 +561,158 @@ __global__ void disc_writeback_kernel(int p, const int32_t *__restrict__ nbr, co
     Gm[(size_t)b * p + a] = 0;
 }
 
+// ----------------------------------------------------------------------------
+// levels above the templated range (9..GPUCSL_MAX_LEVEL), the level taken at
+// run time: the same test per workgroup, with the conditioning set and its
+// radices in LDS and the stratum index of 16 observations accumulated
+// conditioning variable by conditioning variable (one 16-byte load each).
+// ----------------------------------------------------------------------------
+__device__ void count_table_rt(const uint8_t *__restrict__ data, int64_t n, int64_t stride, int vi, int vj,
+                               const int *S, const int *radix, int L, int di, int dj, uint32_t *tab, int cells,
+                               int replicas) {
+    const int tid = threadIdx.x;
+    for (int c = tid; c < cells * replicas; c += DT_THREADS) tab[c] = 0;
+    __syncthreads();
+    uint32_t *mine = tab + (size_t)((tid >> 6) % replicas) * cells;
+    const uint8_t *ci = data + (size_t)vi * stride;
+    const uint8_t *cj = data + (size_t)vj * stride;
+    const int64_t chunks = (n + 15) / 16;
+    for (int64_t ch = tid; ch < chunks; ch += DT_THREADS) {
+        unsigned g[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) g[q] = 0;
+        for (int c = 0; c < L; ++c) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(data + (size_t)S[c] * stride + ch * 16);
+            const unsigned rad = (unsigned)radix[c];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) g[q] += (((&w.x)[q >> 2] >> ((q & 3) * 8)) & 0xffu) * rad;
+        }
+        const uint4 wa = *reinterpret_cast<const uint4 *>(ci + ch * 16);
+        const uint4 wb = *reinterpret_cast<const uint4 *>(cj + ch * 16);
+        const int lim = (ch * 16 + 16 <= n) ? 16 : (int)(n - ch * 16);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if (q < lim) {
+                const int word = q >> 2, sh = (q & 3) * 8;
+                const unsigned a = ((&wa.x)[word] >> sh) & 0xffu;
+                const unsigned b = ((&wb.x)[word] >> sh) & 0xffu;
+                atomicAdd(&mine[(g[q] * di + a) * dj + b], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    if (replicas > 1) {
+        for (int c = tid; c < cells; c += DT_THREADS) {
+            uint32_t v = tab[c];
+            for (int r = 1; r < replicas; ++r) v += tab[(size_t)r * cells + c];
+            tab[c] = v;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(DT_THREADS) void disc_level_rt_kernel(
+    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int p, int L,
+    double alpha, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+    const unsigned long long *__restrict__ testoff, float *__restrict__ pmax,
+    unsigned long long *__restrict__ keys, uint32_t *__restrict__ gtab, size_t gtab_words) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int sS[GPUCSL_MAX_LEVEL], sRadix[GPUCSL_MAX_LEVEL];
+    __shared__ int sDimS;
+    const unsigned long long T = testoff[p];
+    for (unsigned long long t = blockIdx.x; t < T; t += gridDim.x) {
+        int lo = 0, hi = p;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (testoff[mid] <= t) lo = mid; else hi = mid;
+        }
+        const int i = lo;
+        const int d = deg[i];
+        const unsigned long long per_pair = binom(d - 1, L);
+        const unsigned long long o = t - testoff[i];
+        const int jpos = (int)(o / per_pair);
+        const unsigned long long rank = o - (unsigned long long)jpos * per_pair;
+        const int32_t *row = nbr + (size_t)i * p;
+        const int j = row[jpos];
+        if (threadIdx.x == 0) {
+            int pos[GPUCSL_MAX_LEVEL];
+            unrank_lex_rt(d - 1, L, rank, pos);
+            int dim_s = 1;
+            for (int c = 0; c < L; ++c) {
+                const int idx = pos[c] - 1;
+                sS[c] = row[idx + (idx >= jpos)];
+                sRadix[c] = dim_s;
+                dim_s *= dims[sS[c]];
+            }
+            sDimS = dim_s;
+        }
+        __syncthreads();
+        const int dim_s = sDimS;
+        const int di = dims[i], dj = dims[j];
+        const int cells = dim_s * di * dj;
+        const int marg = dim_s * (di + dj + 1);
+        uint32_t *tab;
+        int replicas;
+        if (4 * cells + marg <= DT_LDS_WORDS) {
+            tab = lds;
+            replicas = 4;
+        } else if (cells + marg <= DT_LDS_WORDS) {
+            tab = lds;
+            replicas = 1;
+        } else {
+            tab = gtab + (size_t)blockIdx.x * gtab_words;
+            replicas = 1;
+        }
+        count_table_rt(data, n, stride, i, j, sS, sRadix, L, di, dj, tab, cells, replicas);
+        uint32_t *Nis = tab + (size_t)replicas * cells, *Njs = Nis + dim_s * di, *Ns = Njs + dim_s * dj;
+        marginals(tab, dim_s, di, dj, Nis, Njs, Ns);
+        double chi2 = chi2_sum(tab, Nis, Njs, Ns, dim_s, di, dj, false);
+        if (threadIdx.x < 64) {
+            chi2 = __shfl(chi2, 0);
+            const float pf = (float)pchisq_wave(chi2, (di - 1) * (dj - 1) * dim_s);
+            const int a = i < j ? i : j, b = i < j ? j : i;
+            if (threadIdx.x == 0) atomic_max_nonneg(&pmax[(size_t)a * p + b], pf);
+            if (threadIdx.x == 0 && (double)pf >= alpha) {
+                const unsigned long long key = ((unsigned long long)(i < j ? 0 : 1) << 62) | rank;
+                atomicMin(&keys[(size_t)a * p + b], key);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void disc_writeback_rt_kernel(int p, int L, const int32_t *__restrict__ nbr,
+                                         const int32_t *__restrict__ deg, unsigned long long *__restrict__ keys,
+                                         uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, int maxL) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (b >= p || b <= a) return;
+    const unsigned long long key = keys[(size_t)a * p + b];
+    if (key == KEY_NONE) return;
+    keys[(size_t)a * p + b] = KEY_NONE;
+    const int side = (int)(key >> 62);
+    const unsigned long long rank = key & ((1ull << 62) - 1);
+    const int i = side ? b : a, j = side ? a : b;
+    const int32_t *row = nbr + (size_t)i * p;
+    const int d = deg[i];
+    int lo = 0, hi = d;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (row[mid] < j) lo = mid + 1; else hi = mid;
+    }
+    const int jpos = lo;
+    int pos[GPUCSL_MAX_LEVEL];
+    unrank_lex_rt(d - 1, L, rank, pos);
+    for (int c = 0; c < L; ++c) {
+        const int idx = pos[c] - 1;
+        const int v = row[idx + (idx >= jpos)];
+        sep[((size_t)i * p + j) * maxL + c] = v;
+        sep[((size_t)j * p + i) * maxL + c] = v;
+    }
+    Gm[(size_t)a * p + b] = 0;
+    Gm[(size_t)b * p + a] = 0;
+}
+
 // Single test for known-answer checks: table (cells words), chi^2 and p.
 template <int L>
 __global__ __launch_bounds__(DT_THREADS) void disc_single_test_kernel(
@@ -764,8 +916,8 @@ int disc_level_launch(int level, const uint8_t *data, int64_t n, int64_t stride,
                       int max_level, int max_dim, double alpha, uint8_t *Gm, int32_t *sep, float *pmax,
                       unsigned long long *st, void *ws, size_t ws_bytes, hipStream_t s) {
     if (level < 1 || level > max_level) return set_error(ERR_ARG, "disc_level: level %d outside [1, %d]", level, max_level);
-    if (level > GPUCSL_MAX_LEVEL_TEMPLATE)
-        return set_error(ERR_UNSUPPORTED, "disc_level: levels above %d are not built", GPUCSL_MAX_LEVEL_TEMPLATE);
+    if (level > GPUCSL_MAX_LEVEL)
+        return set_error(ERR_UNSUPPORTED, "disc_level: levels above %d are not built", GPUCSL_MAX_LEVEL);
     if (!data || !dims || !Gm || !sep || !pmax || !st || !ws) return set_error(ERR_ARG, "disc_level: null pointer");
     if (stride < (n + 15) / 16 * 16 || stride % 16 != 0)
         return set_error(ERR_ARG, "disc_level: stride must be a multiple of 16 and >= n rounded up to 16");
@@ -784,6 +936,12 @@ int disc_level_launch(int level, const uint8_t *data, int64_t n, int64_t stride,
         break;
         GPUCSL_D(1) GPUCSL_D(2) GPUCSL_D(3) GPUCSL_D(4) GPUCSL_D(5) GPUCSL_D(6) GPUCSL_D(7) GPUCSL_D(8)
 #undef GPUCSL_D
+    default:
+        GPUCSL_LAUNCH("disc_level_rt", disc_level_rt_kernel, dim3(w.grid), dim3(DT_THREADS), DT_LDS_BYTES, s, data,
+                      n, stride, dims, p, level, alpha, nbr, deg, w.testoff, pmax, w.keys, w.gtab, w.gtab_words);
+        GPUCSL_LAUNCH("disc_writeback", disc_writeback_rt_kernel, dim3((p + 255) / 256, p), dim3(256), 0, s, p,
+                      level, nbr, deg, w.keys, Gm, sep, max_level);
+        break;
     }
     return check_hip(hipGetLastError(), "disc_level launch");
 }

@@ -1521,6 +1521,229 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
 
 
 // ----------------------------------------------------------------------------
+// levels above the templated range (9..GPUCSL_MAX_LEVEL): one wave per
+// ordered pair, the 64 lanes evaluating one CI test together in FP64 in the
+// reference's operation order (gaussian_helpers.cu:16-262, the same steps as
+// ref64::gauss_z), so every test takes the FP64 decision directly and no
+// recheck queue is needed.  Each output element of a matrix product is one
+// lane's sequential dot product in the reference's k order; the Cholesky
+// factor is formed column by column and the Gauss-Jordan inverse pivot by
+// pivot, every element with the reference's formula and operands, so the
+// bits equal the sequential routine's.  A wave walks the pair's subsets in
+// rank order (the lexicographic successor of the previous one, one lane per
+// position) and stops at the first passing test.  Side A, then side B, as in
+// the templated levels.
+// ----------------------------------------------------------------------------
+constexpr int GEN_THREADS = 64;
+
+__host__ __device__ inline size_t gen_lds_bytes(int L) {
+    // doubles: M2, T, Lo, R, M2i (L*L each), M1 and W (2L each), H (4); then S (L ints)
+    return sizeof(double) * ((size_t)5 * L * L + 4 * (size_t)L + 4) + sizeof(int) * (size_t)L;
+}
+
+struct GenMats {
+    double *M2, *T, *Lo, *R, *M2i, *M1, *W, *H;
+    int *S;
+};
+
+__device__ __forceinline__ GenMats gen_carve(void *lds, int L) {
+    GenMats m;
+    double *d = (double *)lds;
+    const int LL = L * L;
+    m.M2 = d;
+    m.T = d + LL;
+    m.Lo = d + 2 * LL;
+    m.R = d + 3 * LL;
+    m.M2i = d + 4 * LL;
+    m.M1 = d + 5 * LL;
+    m.W = m.M1 + 2 * L;
+    m.H = m.W + 2 * L;
+    m.S = (int *)(m.H + 4);
+    return m;
+}
+
+// R (D1 x D3) = op(A) op(B) over D2, one output element per lane step.
+// MODE 0: A (D1 x D2) B (D2 x D3); 1: A^T with A (D2 x D1); 2: B^T with B (D3 x D2)
+template <int MODE>
+__device__ __forceinline__ void gen_mm(const double *A, const double *B, double *R, int D1, int D2, int D3) {
+#pragma clang fp contract(off)
+    for (int o = threadIdx.x; o < D1 * D3; o += GEN_THREADS) {
+        const int r = o / D3, c = o - (o / D3) * D3;
+        double acc = 0.0;
+        for (int k = 0; k < D2; ++k) {
+            const double a = MODE == 1 ? A[k * D1 + r] : A[r * D2 + k];
+            const double b = MODE == 2 ? B[c * D2 + k] : B[k * D3 + c];
+            acc += a * b;
+        }
+        R[o] = acc;
+    }
+    __syncthreads();
+}
+
+// Lo = cholesky(A) as ref64::cholesky: Lo[r][c] = sqrt(A[r][r] - acc) on the
+// diagonal, (1 / Lo[c][c]) (A[r][c] - acc) below, acc = sum_{k<c} Lo[r][k] Lo[c][k]
+__device__ void gen_cholesky(const double *A, double *Lo, int L) {
+#pragma clang fp contract(off)
+    for (int o = threadIdx.x; o < L * L; o += GEN_THREADS) Lo[o] = 0.0;
+    __syncthreads();
+    for (int c = 0; c < L; ++c) {
+        if (threadIdx.x == 0) {
+            double acc = 0.0;
+            for (int k = 0; k < c; ++k) acc += Lo[c * L + k] * Lo[c * L + k];
+            Lo[c * L + c] = sqrt(A[c * L + c] - acc);
+        }
+        __syncthreads();
+        for (int r = c + 1 + threadIdx.x; r < L; r += GEN_THREADS) {
+            double acc = 0.0;
+            for (int k = 0; k < c; ++k) acc += Lo[r * L + k] * Lo[c * L + k];
+            Lo[r * L + c] = (1.0 / Lo[c * L + c] * (A[r * L + c] - acc));
+        }
+        __syncthreads();
+    }
+}
+
+// Inv = M^-1 by Gauss-Jordan without pivoting, as ref64::gj_inverse (M is
+// overwritten).  Each pivot: the pivot row divided by f, then every other row
+// r minus the pivot row times its old M[r][piv].
+__device__ void gen_gj_inverse(double *M, double *Inv, int L) {
+#pragma clang fp contract(off)
+    constexpr int PER = (GPUCSL_MAX_LEVEL * GPUCSL_MAX_LEVEL + GEN_THREADS - 1) / GEN_THREADS;
+    for (int o = threadIdx.x; o < L * L; o += GEN_THREADS) Inv[o] = (o / L == o % L) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int piv = 0; piv < L; ++piv) {
+        const double f = M[piv * L + piv];
+        __syncthreads();
+        for (int c = threadIdx.x; c < L; c += GEN_THREADS) {
+            Inv[piv * L + c] = Inv[piv * L + c] / f;
+            M[piv * L + c] = M[piv * L + c] / f;
+        }
+        __syncthreads();
+        double g[PER];
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int o = threadIdx.x + t * GEN_THREADS;
+            g[t] = o < L * L ? M[(o / L) * L + piv] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int o = threadIdx.x + t * GEN_THREADS;
+            if (o < L * L) {
+                const int r = o / L, c = o - r * L;
+                if (r != piv) {
+                    Inv[o] -= Inv[piv * L + c] * g[t];
+                    M[o] -= M[piv * L + c] * g[t];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// z of (vi, vj | S) from C64, S in m.S (L ids); every lane returns it.
+__device__ double gen_gauss_z(const double *__restrict__ C, int p, int vi, int vj, const GenMats &m, int L) {
+#pragma clang fp contract(off)
+    for (int o = threadIdx.x; o < L * L; o += GEN_THREADS) {
+        const int r = o / L, c = o - r * L;
+        m.M2[o] = C[(size_t)m.S[r] * p + m.S[c]];
+    }
+    for (int k = threadIdx.x; k < L; k += GEN_THREADS) {
+        m.M1[k] = C[(size_t)vi * p + m.S[k]];
+        m.M1[L + k] = C[(size_t)vj * p + m.S[k]];
+    }
+    __syncthreads();
+    // compute_m_2_inv (gaussian_helpers.cu:201-218)
+    gen_mm<1>(m.M2, m.M2, m.T, L, L, L);
+    gen_cholesky(m.T, m.Lo, L);
+    gen_mm<1>(m.Lo, m.Lo, m.T, L, L, L);
+    gen_gj_inverse(m.T, m.R, L);
+    gen_mm<0>(m.Lo, m.R, m.T, L, L, L);
+    gen_mm<0>(m.T, m.R, m.M2i, L, L, L);
+    gen_mm<2>(m.M2i, m.Lo, m.T, L, L, L);
+    gen_mm<2>(m.T, m.M2, m.M2i, L, L, L);
+    // compute_h (gaussian_helpers.cu:78-89)
+    gen_mm<2>(m.M2i, m.M1, m.W, L, L, 2);
+    gen_mm<0>(m.M1, m.W, m.H, 2, L, 2);
+    const double m0[4] = {1.0, C[(size_t)vi * p + vj], C[(size_t)vj * p + vi], 1.0};
+    double H[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) H[t] = m0[t] - m.H[t];
+    __syncthreads();  // m.H and the matrices are rewritten by the next test
+    // compute_fishers_z_transform (gaussian_helpers.cu:220-227)
+    const double rho = H[1] / (sqrt(fabs(H[0])) * sqrt(fabs(H[3])));
+    return fabs(0.5 * (log(fabs(1 + rho)) - log(fabs(1 - rho))));
+}
+
+__global__ __launch_bounds__(GEN_THREADS) void gauss_level_generic_kernel(const Job *__restrict__ jobs, int level,
+                                                                        int side, unsigned long long *__restrict__ st) {
+    extern __shared__ double gen_lds[];
+    const Job &J = jobs[blockIdx.y];
+    if (level > J.max_level) return;
+    const int p = J.p, L = level, lane = threadIdx.x;
+    JobWs jw = carve_job(J.ws, p);
+    const int32_t *rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
+    const int total = rowoff[p];
+    const double thr = J.thr[L];
+    const float zscale = level_zscale(J, L);
+    const bool both = !sepset_one_way(J);
+    GenMats m = gen_carve(gen_lds, L);
+    unsigned long long n_exec = 0;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int lo = 0, hi = p;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rowoff[mid] <= w) lo = mid; else hi = mid;
+        }
+        const int i = lo;
+        const int d = jw.deg[i];
+        const int cnt = rowoff[i + 1] - rowoff[i];
+        const int jpos = (side == 0 ? d - cnt : 0) + (w - rowoff[i]);
+        const int32_t *nb = jw.nbr + (size_t)i * p;
+        const int j = nb[jpos];
+        if (side == 1 && J.G[(size_t)i * p + j] != 1) continue;  // side A removed the edge
+        const int N = d - 1;
+        const uint64_t count = binom(N, L);
+        // lane k < L holds the k-th 1-based position of the current subset
+        int pos = lane + 1;
+        float zl = __builtin_inff();
+        for (uint64_t r = 0; r < count; ++r) {
+            if (lane < L) {
+                const int idx = pos - 1;
+                m.S[lane] = nb[idx + (idx >= jpos)];
+            }
+            __syncthreads();
+            const double z = gen_gauss_z(J.C64, p, i, j, m, L);
+            ++n_exec;
+            if (!(z != z)) zl = fminf(zl, (float)z);
+            if (z < thr) {
+                stg(J.G + (size_t)i * p + j, (uint8_t)0);
+                stg(J.G + (size_t)j * p + i, (uint8_t)0);
+                if (lane < L) {
+                    const int idx = pos - 1;
+                    const int v = nb[idx + (idx >= jpos)];
+                    J.sepsets[((size_t)i * p + j) * J.max_level + lane] = v;
+                    if (both) J.sepsets[((size_t)j * p + i) * J.max_level + lane] = v;
+                }
+                break;
+            }
+            // lexicographic successor: the last position below its maximum
+            // N - L + 1 + k goes up by one, the positions after it follow
+            const uint64_t up = __ballot(lane < L && pos < N - L + 1 + lane);
+            if (up) {
+                const int K = 63 - __clzll((long long)up);
+                const int base = __shfl(pos, K) + 1;
+                if (lane >= K && lane < L) pos = base + (lane - K);
+            }
+        }
+        if (lane == 0 && zl < __builtin_inff()) update_zmin(J, p, i, j, zl * zscale);
+    }
+    if (lane == 0 && n_exec) {
+        atomicAdd(&st[ST_EXEC], n_exec);
+        atomicAdd(&st[ST_USEFUL], n_exec);
+    }
+}
+
+// ----------------------------------------------------------------------------
 // pmax and output conversion
 // ----------------------------------------------------------------------------
 // pmax = 2 (1 - Phi(sqrt(n-3) |zmin|)) = erfc(sqrt(n-3) |zmin| / sqrt 2); -1 on
@@ -1665,8 +1888,8 @@ static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int 
 int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float margin_k, int group_size,
                       unsigned long long *st, void *bws, size_t bws_bytes, hipStream_t s) {
     if (level < 1) return set_error(ERR_ARG, "gauss_level: level %d < 1", level);
-    if (level > GPUCSL_MAX_LEVEL_TEMPLATE)
-        return set_error(ERR_UNSUPPORTED, "gauss_level: levels above %d are not built", GPUCSL_MAX_LEVEL_TEMPLATE);
+    if (level > GPUCSL_MAX_LEVEL)
+        return set_error(ERR_UNSUPPORTED, "gauss_level: levels above %d are not built", GPUCSL_MAX_LEVEL);
     if (group_size != 8 && group_size != 16 && group_size != 32 && group_size != 64)
         return set_error(ERR_ARG, "group size must be 8, 16, 32 or 64 (got %d)", group_size);
     if (!st || !bws) return set_error(ERR_ARG, "gauss_level: null stats or workspace");
@@ -1680,6 +1903,14 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
                         w.itemsB};
     int rc = compact_and_plan(jh, jd, nj, level, group_size, st, plan, s);
     if (rc) return rc;
+    if (level > GPUCSL_MAX_LEVEL_TEMPLATE) {
+        // a few waves per compute unit: each holds its test's matrices in LDS
+        const dim3 grid((unsigned)(device_cus() * 8), (unsigned)nj);
+        for (int side = 0; side < 2; ++side)
+            GPUCSL_LAUNCH(side == 0 ? "gauss_level_generic_a" : "gauss_level_generic_b", gauss_level_generic_kernel,
+                          grid, dim3(GEN_THREADS), gen_lds_bytes(level), s, jd, level, side, st);
+        return check_hip(hipGetLastError(), "gauss_level launch");
+    }
     for (int side = 0; side < 2; ++side) {
         switch (level) {
 #define GPUCSL_CASE(LV)                                                     \
@@ -1800,8 +2031,8 @@ int gauss_level_launch(int level, const float *C32, const double *C64, int p, in
     if (level < 1 || level > max_level) return set_error(ERR_ARG, "gauss_level: level %d outside [1, %d]", level, max_level);
     if (!ws) return set_error(ERR_ARG, "gauss_level: null workspace");
     if (ws_bytes < gauss_workspace_bytes(p)) return set_error(ERR_WORKSPACE, "gauss_level: workspace too small");
-    if (level > GPUCSL_MAX_LEVEL_TEMPLATE)
-        return set_error(ERR_UNSUPPORTED, "gauss_level: levels above %d are not built", GPUCSL_MAX_LEVEL_TEMPLATE);
+    if (level > GPUCSL_MAX_LEVEL)
+        return set_error(ERR_UNSUPPORTED, "gauss_level: levels above %d are not built", GPUCSL_MAX_LEVEL);
     double thr_all[GPUCSL_MAX_LEVELS] = {0};
     thr_all[level] = thr;
     Job j = single_job(p, max_level, C32, C64, Gm, sep, zmin, thr_all, GPUCSL_MAX_LEVELS, ws);
@@ -1823,7 +2054,7 @@ int gauss_skeleton_launch(const float *C32, const double *C64, int p, int max_le
     if (rc) return rc;
     rc = gauss_batch_level0(&j, sw.desc, 1, margin_k, stats, s);
     if (rc) return rc;
-    const int built = max_level < GPUCSL_MAX_LEVEL_TEMPLATE ? max_level : GPUCSL_MAX_LEVEL_TEMPLATE;
+    const int built = max_level < GPUCSL_MAX_LEVEL ? max_level : GPUCSL_MAX_LEVEL;
     for (int l = 1; l <= built; ++l) {
         rc = gauss_batch_level(l, &j, sw.desc, 1, margin_k, group_size, stats + (size_t)l * ST_SLOTS, sw.bws,
                                sw.bws_bytes, s);

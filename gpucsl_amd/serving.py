@@ -39,7 +39,7 @@ from gpucsl_amd.cli.cli_util import GpucslError
 from gpucsl_amd.discrete import DiscreteEngine, DiscreteResult, disc_layout
 from gpucsl_amd.gaussian import DEFAULT_GROUP_SIZE, DEFAULT_MARGIN, GaussianResult
 
-MAX_SERVED_LEVEL = 8
+MAX_SERVED_LEVEL = 32    # GPUCSL_MAX_LEVEL: the deepest level any kernel runs
 
 
 @dataclass

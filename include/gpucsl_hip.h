@@ -36,13 +36,17 @@
 extern "C" {
 #endif
 
-#define GPUCSL_ABI_VERSION 1
+#define GPUCSL_ABI_VERSION 2
 #define GPUCSL_STAT_SLOTS 8
 
 /* Thread-local description of the last failed call. */
 const char *gpucsl_last_error(void);
 
-/* ABI version (GPUCSL_ABI_VERSION) and the highest level with a built kernel. */
+/* ABI version (GPUCSL_ABI_VERSION) and the highest level the kernels run
+ * (GPUCSL_MAX_LEVEL).  Levels 1..8 have kernels specialised per level; levels
+ * 9..GPUCSL_MAX_LEVEL run a kernel that takes the level at run time (the
+ * reference instantiates every level at run time,
+ * gpucsl/pc/kernel_management.py:211-212). */
 int gpucsl_abi_version(void);
 int gpucsl_max_built_level(void);
 
@@ -67,7 +71,8 @@ int gpucsl_corr_c64_to_c32(const double *C64, int p, float *C32, void *stream);
 /* ---------------------------------------------------------------------------
  * Gaussian skeleton
  * ------------------------------------------------------------------------- */
-#define GPUCSL_MAX_LEVELS 9   /* thresholds for levels 0..8 */
+#define GPUCSL_MAX_LEVEL 32   /* highest level any kernel runs */
+#define GPUCSL_MAX_LEVELS 33  /* thresholds for levels 0..GPUCSL_MAX_LEVEL */
 
 /* One Gaussian skeleton job (one dataset).  Every pointer is device memory
  * owned by the caller: ws is a per-job level workspace of
@@ -311,7 +316,8 @@ int gpucsl_gauss_outputs(const float *zmin, const uint8_t *G, int p, int64_t n, 
 
 /* Lexicographic unranking used by the CI kernels, for tests: out[c*l + k] is
  * the k-th 1-based element of combination ranks[c] of l out of N
- * (gpucsl/pc/cuda/helpers/set_helpers.cu:25-49).  1 <= l <= 8. */
+ * (gpucsl/pc/cuda/helpers/set_helpers.cu:25-49).  1 <= l <= GPUCSL_MAX_LEVEL;
+ * levels 1..8 use the kernels' specialised unranking, higher ones the run-time one. */
 int gpucsl_unrank(int N, int l, const uint64_t *ranks, int count, int32_t *out, void *stream);
 
 /* ---------------------------------------------------------------------------

@@ -44,7 +44,7 @@
 #pragma GCC diagnostic ignored "-Wmaybe-uninitialized"
 
 #define OR_EXPORT __attribute__((visibility("default")))
-#define OR_MAXL 16
+#define OR_MAXL 32
 
 /* ------------------------------------------------------------------------- */
 /* combinatorics                                                              */

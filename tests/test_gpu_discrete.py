@@ -156,6 +156,23 @@ def test_high_levels_parity():
     _compare(res, ref, "discrete, max level 8")
 
 
+def test_levels_above_8_parity():
+    """Levels 9-10 (the kernel that takes the level at run time): a binary
+    variable, ten noisy copies of it and a threshold of their sum, so the edge
+    between the first and the last goes at level 10; skeleton, sepsets,
+    per-level test counts and pmax against the oracle."""
+    rng = np.random.default_rng(5)
+    n, k = 50000, 10
+    i = rng.integers(0, 2, n)
+    m = (i[:, None] ^ (rng.random((n, k)) < 0.2)).astype(np.int64)
+    j = ((m.sum(1) + rng.integers(0, 2, n)) > 5).astype(np.int64)
+    codes = np.column_stack([i, m, j]).astype(np.uint8)
+    res, ref = _run(codes, 0.01, 10)
+    assert ref["executed"][9] > 0 and ref["executed"][10] > 0
+    assert ref["skeleton"][0, 11] == 0
+    _compare(res, ref, "discrete, max level 10")
+
+
 @pytest.mark.parametrize("p,card,n,label", [
     (14, (14, 16), 2500, "one-hot Gram matrix, 16 categories"),
     (12, (17, 20), 2500, "pair-by-pair tables, more than 16 categories"),

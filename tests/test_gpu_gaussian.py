@@ -60,7 +60,7 @@ def _compare(res, ref, label, pmax_atol=2e-5):
 
 
 def test_unrank_matches_oracle():
-    for l in range(1, 6):
+    for l in list(range(1, 6)) + [9, 10, 12]:
         for N in sorted({l, l + 1, 7, 13, 40, 150}):
             total = oracle.binomial(N, l)
             if total == 0:
@@ -245,6 +245,71 @@ def test_high_levels_match_oracle():
     _compare(res, ref, "equicorrelated, max level 8")
     for l in range(9):
         assert res.stats[l]["useful"] == ref["executed"][l]
+
+
+def _mediators(n, k, seed):
+    """i -> m_1..m_k -> j: the edge i - j survives every level below k and
+    goes at level k with separation set {m_1..m_k}."""
+    rng = np.random.default_rng(seed)
+    i = rng.standard_normal(n)
+    m = 0.6 * i[:, None] + rng.standard_normal((n, k))
+    j = 0.4 * m.sum(1) + rng.standard_normal(n)
+    return np.column_stack([i, m, j])
+
+
+def test_levels_above_8_match_oracle():
+    """Levels 9-12 (the kernel that takes the level at run time): ten
+    mediators between two variables, so the edge goes at level 10 with all ten
+    as its separation set; and an equicorrelated block of 14 variables, whose
+    every subset of every pair runs up to level 12.  Skeleton, sepsets, pmax
+    and the per-level test counts against the FP64 oracle."""
+    X = _mediators(100000, 10, 5)
+    _, res = _engine_run(X, 100000, 0.01, 10)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X), 100000, 0.01, 10)
+    assert ref["executed"][9] > 0 and ref["executed"][10] > 0
+    assert ref["skeleton"][0, 11] == 0 and list(ref["sepsets"][0, 11]) == list(range(1, 11))
+    _compare(res, ref, "ten mediators, max level 10", pmax_atol=5e-5)
+    for l in range(11):
+        assert res.stats[l]["useful"] == ref["executed"][l], l
+    p, n = 14, 100000
+    C = np.full((p, p), 0.5)
+    np.fill_diagonal(C, 1.0)
+    _, res = _engine_run(None, n, 0.01, 12, C=C)
+    ref = oracle.gauss_skeleton(C, n, 0.01, 12)
+    assert ref["executed"][12] == 91 * 2     # both sides of every pair, C(12, 12) subsets
+    _compare(res, ref, "equicorrelated, max level 12")
+    for l in range(13):
+        assert res.stats[l]["useful"] == ref["executed"][l], l
+    iu = np.triu_indices(p, 1)
+    # levels 9-12 decide in FP64 with the reference's operation order
+    assert np.abs(res.zmin[iu].astype(np.float64) - ref["zmin"][iu]).max() <= 1e-6
+
+
+def test_levels_above_8_batched_and_captured():
+    """A job that runs to level 10 in a batch with shallow jobs, launched
+    eagerly and replayed from a captured graph: the same bits as alone."""
+    from gpucsl_amd.batch import GaussJobState, GaussianBatch, run_plain
+
+    X = _mediators(50000, 10, 6).astype(np.float32)
+    Y = syn.gaussian_sem(80, 3000, 3.0, seed=8).data.astype(np.float32)
+    jobs, plain = [], []
+    for lst in (jobs, plain):
+        a = GaussJobState(50000, 12, 10, alpha=0.01)
+        a.set_data(X)
+        b = GaussJobState(3000, 80, 3, alpha=0.01)
+        b.set_data(Y)
+        lst += [a, b]
+    run_plain(plain)
+    bt = GaussianBatch(jobs)
+    bt.capture()
+    for rep in range(3):
+        bt.run()
+        torch.cuda.synchronize()
+        for a, c in zip(bt.results(), [j.results() for j in plain]):
+            assert (a.skeleton == c.skeleton).all() and (a.separation_sets == c.separation_sets).all()
+            assert np.array_equal(a.zmin.view(np.uint32), c.zmin.view(np.uint32))
+    r = bt.results()[0]
+    assert r.skeleton[0, 11] == 0 and list(r.separation_sets[0, 11]) == list(range(1, 11))
 
 
 @pytest.mark.parametrize("seed", range(5))

@@ -41,7 +41,8 @@ def test_abi_version_without_gpu():
     from gpucsl_amd import _native
 
     lib = _native.load()
-    assert lib.gpucsl_abi_version() == 1
+    assert lib.gpucsl_abi_version() == 2
+    assert lib.gpucsl_max_built_level() == 32
     assert lib.gpucsl_max_built_level() >= 4
 
 
@@ -53,14 +54,15 @@ def test_product_does_not_import_oracle():
 
 def test_job_struct_layout_matches_header():
     """The ctypes mirror of gpucsl_gauss_job has the C layout: 4+4+8+4+4+4+4
-    bytes of scalars, ten pointers, nine doubles (x86-64 / gfx950 alignment)."""
+    bytes of scalars, ten pointers, 33 doubles (x86-64 / gfx950 alignment)."""
     import ctypes as ct
 
     from gpucsl_amd.batch import GaussJob, MAX_LEVELS
 
-    assert MAX_LEVELS == 9
+    assert MAX_LEVELS == 33
     assert GaussJob.n.offset == 8 and GaussJob.flags.offset == 24 and GaussJob.X.offset == 32
-    assert GaussJob.corr_ws.offset == 104 and GaussJob.thr.offset == 112 and ct.sizeof(GaussJob) == 184
+    assert GaussJob.corr_ws.offset == 104 and GaussJob.thr.offset == 112 and ct.sizeof(GaussJob) == 112 + 33 * 8
+    assert "#define GPUCSL_MAX_LEVELS 33" in (ROOT / "include" / "gpucsl_hip.h").read_text()
     src = (ROOT / "include" / "gpucsl_hip.h").read_text()
     body = src[src.index("typedef struct gpucsl_gauss_job"):src.index("} gpucsl_gauss_job;")]
     names = [ln.split(";")[0].replace("*", " ").split()[-1].split("[")[0]
