@@ -44,6 +44,7 @@ SIGNATURES = {
     "gpucsl_gauss_batch_skeleton": (_I, [_P, _P, _I, _F, _I, _P, _P, _SZ, _P]),
     "gpucsl_gauss_level": (_I, [_I, _P, _P, _I, _I, _D, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_gauss_skeleton": (_I, [_P, _P, _I, _I, _P, _F, _I, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gpucsl_gauss_skeleton_noalloc": (_I, [_P, _P, _I, _I, _P, _F, _I, _P, _P, _P, _P, _P]),
     "gpucsl_set_level_tuning": (_I, [_I, _P]),
     "gpucsl_get_level_tuning": (_I, [_I, _P]),
     "gpucsl_pmax_from_zmin": (_I, [_P, _I, _I64, _P, _P]),
@@ -66,6 +67,7 @@ SIGNATURES = {
 }
 
 STAT_SLOTS = 8
+NOALLOC_SCRATCH_BYTES = 64 << 20   # GPUCSL_NOALLOC_SCRATCH_BYTES
 STAT_NAMES = ("executed", "total", "rechecked", "flipped", "max_err_ratio", "max_degree", "useful")
 
 _lib = None

@@ -53,7 +53,11 @@ static hipEvent_t take() {
     return e;
 }
 
-bool enabled() { return g_on; }
+static thread_local int t_suppress = 0;
+Suppress::Suppress() { ++t_suppress; }
+Suppress::~Suppress() { --t_suppress; }
+
+bool enabled() { return g_on && t_suppress == 0; }
 
 void record_start(hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -89,6 +93,9 @@ int gauss_level_launch(int level, const float *C32, const double *C64, int p, in
 int gauss_skeleton_launch(const float *C32, const double *C64, int p, int max_level, const double *thr,
                           float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
                           unsigned long long *stats, int nst, void *ws, size_t ws_bytes, hipStream_t s);
+int gauss_skeleton_noalloc_launch(const float *C32, const double *C64, int p, int max_level, const double *thr,
+                                  float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
+                                  unsigned long long *stats, int nst, hipStream_t s);
 int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats, hipStream_t s);
 void level_list_views(void *ws, int p, int32_t **nbr, int32_t **deg);
 int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s);
@@ -267,6 +274,17 @@ int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_le
     return check_unbuilt_levels(
         max_level, stats,
         [&](int lv, unsigned long long *st) { return compact_launch(G, p, lv, ws, st, S(stream)); }, S(stream));
+}
+
+int gpucsl_gauss_skeleton_noalloc(const float *C32, const double *C64, int p, int max_level, const double *thr,
+                                  float margin, int group_size, uint8_t *G, int32_t *sepsets, float *zmin,
+                                  uint64_t *stats, void *stream) {
+    if (!thr || !stats) return set_error(ERR_ARG, "gauss_skeleton_noalloc: null thr or stats");
+    if (!C32 || !C64 || !G || !zmin || (max_level > 0 && !sepsets))
+        return set_error(ERR_ARG, "gauss_skeleton_noalloc: null matrix");
+    return gauss_skeleton_noalloc_launch(C32, C64, p, max_level, thr, margin, group_size, G, sepsets, zmin,
+                                         reinterpret_cast<unsigned long long *>(stats),
+                                         (max_level + 1) * GPUCSL_STAT_SLOTS, S(stream));
 }
 
 // ---- batches of Gaussian jobs ----------------------------------------------

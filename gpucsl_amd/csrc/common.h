@@ -507,6 +507,12 @@ namespace prof {
 bool enabled();
 void record_start(hipStream_t s);
 void record_stop(const char *name, hipStream_t s);
+// Launches of this thread are not timed while one exists (the entry point
+// that allocates nothing creates no events).
+struct Suppress {
+    Suppress();
+    ~Suppress();
+};
 struct Scope {
     const char *name;
     hipStream_t s;

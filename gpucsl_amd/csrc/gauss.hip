@@ -2063,6 +2063,61 @@ int gauss_skeleton_launch(const float *C32, const double *C64, int p, int max_le
     return OK;
 }
 
+// ---- single dataset without a caller workspace ----------------------------
+// For callers that cannot allocate device memory (code under graph capture,
+// serving loops, harnesses that forbid hipMalloc): the workspace of
+// gauss_skeleton_launch lives in this file-scope array, one copy per device,
+// held until the process ends.  Host code never passes the array's name to a
+// kernel: its device address, per device, comes from hipGetSymbolAddress on
+// the first call with that device current.  Every call on a device shares its
+// array, so calls that overlap on one device race; the kernels set every word
+// they read (descriptor, counters, statistics, lists), so nothing needs
+// clearing between calls.
+static __device__ __align__(256) unsigned char scratch[GPUCSL_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+static int scratch_of_current_device(void **out) {
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return set_error(ERR_UNSUPPORTED, "gauss_skeleton_noalloc: device %d beyond %d", dev, SCRATCH_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        if (!ptr) return set_error(ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return OK;
+}
+
+int gauss_skeleton_noalloc_launch(const float *C32, const double *C64, int p, int max_level, const double *thr,
+                                  float margin_k, int group_size, uint8_t *Gm, int32_t *sep, float *zmin,
+                                  unsigned long long *stats, int nst, hipStream_t s) {
+    if (p < 1 || max_level < 0) return set_error(ERR_ARG, "gauss_skeleton_noalloc: bad p or max_level");
+    if (max_level > GPUCSL_MAX_LEVEL)
+        return set_error(ERR_UNSUPPORTED, "gauss_skeleton_noalloc: max_level %d above %d", max_level,
+                         GPUCSL_MAX_LEVEL);
+    const size_t need = gauss_workspace_bytes(p);
+    if (need > (size_t)GPUCSL_NOALLOC_SCRATCH_BYTES)
+        return set_error(ERR_WORKSPACE,
+                         "gauss_skeleton_noalloc: p = %d needs %zu bytes of scratch, more than its %zu-byte array; "
+                         "use gpucsl_gauss_skeleton with a workspace",
+                         p, need, (size_t)GPUCSL_NOALLOC_SCRATCH_BYTES);
+    void *ws = nullptr;
+    int rc = scratch_of_current_device(&ws);
+    if (rc) return rc;
+    prof::Suppress no_events;  // event timing would create events
+    return gauss_skeleton_launch(C32, C64, p, max_level, thr, margin_k, group_size, Gm, sep, zmin, stats, nst, ws,
+                                 need, s);
+}
+
 int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long long *stats, hipStream_t s) {
     Job j = single_job(p, level, nullptr, nullptr, const_cast<uint8_t *>(Gm), nullptr, nullptr, nullptr, 0, ws);
     if (!stats) {

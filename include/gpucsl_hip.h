@@ -156,6 +156,22 @@ int gpucsl_gauss_skeleton(const float *C32, const double *C64, int p, int max_le
                           int32_t *sepsets, float *zmin, uint64_t *stats, void *ws, size_t ws_bytes,
                           void *stream);
 
+/* gpucsl_gauss_skeleton without a workspace argument: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a library array of GPUCSL_NOALLOC_SCRATCH_BYTES per device,
+ * held until the process ends; p is limited to what it holds
+ * (gpucsl_gauss_workspace_bytes(p, max_level) <= GPUCSL_NOALLOC_SCRATCH_BYTES,
+ * p <= 1234), else status 4 (workspace) and nothing runs; max_level <=
+ * GPUCSL_MAX_LEVEL, else status 3.  Calls no allocating routine (and records
+ * no timing events).  All calls on one device share its array: calls that
+ * overlap on one device race and must be ordered by the caller (one stream,
+ * or events).  Results are the same bits as gpucsl_gauss_skeleton's. */
+#define GPUCSL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gpucsl_gauss_skeleton_noalloc(const float *C32, const double *C64, int p, int max_level, const double *thr,
+                                  float margin, int group_size, uint8_t *G, int32_t *sepsets, float *zmin,
+                                  uint64_t *stats, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Batches of Gaussian jobs.  Every call takes the job table twice: `jobs` in
  * host memory (sizes and validation) and `jobs_dev`, the same bytes in device
