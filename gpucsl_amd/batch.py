@@ -187,11 +187,12 @@ class GaussJobState:
     def results(self, stats=None) -> GaussianResult:
         torch = self.torch
         p, L = self.p, max(self.max_level, 1)
-        skel = self.view("skeleton", torch.int16, (p, p)).cpu().numpy().view(np.uint16).copy()
-        sep = self.view("sepsets", torch.int32, (p, p, L)).cpu().numpy()[:, :, : self.max_level].copy()
-        pmax = self.view("pmax", torch.float32, (p, p)).cpu().numpy().copy()
-        zmin = self.view("zmin", torch.float32, (p, p)).cpu().numpy().copy()
-        return GaussianResult(skel, sep, pmax, zmin, stats if stats is not None else [])
+        skel, sep, pmax, zmin = nat.to_host(self.view("skeleton", torch.int16, (p, p)),
+                                            self.view("sepsets", torch.int32, (p, p, L)),
+                                            self.view("pmax", torch.float32, (p, p)),
+                                            self.view("zmin", torch.float32, (p, p)))
+        return GaussianResult(skel.view(np.uint16), sep[:, :, : self.max_level], pmax, zmin,
+                              stats if stats is not None else [])
 
 
 @dataclass

@@ -39,14 +39,25 @@ bool in_sepset(const int32_t *sep, int p, int L, int v, int a, int b) {
     return false;
 }
 
-void orient_v_structures(const uint8_t *skel, int p, const int32_t *sep, int L, Graph &dag) {
-    std::vector<std::vector<int>> nbr(p);
-    for (int i = 0; i < p; ++i)
+// Ascending skeleton neighbours of every node.  Orienting only removes one
+// arc of an edge, so every arc the rules look at is a skeleton edge: walking
+// neighbour lists visits the same (v1, v2, v3) triples, in the same order, as
+// walking every node index and skipping non-arcs.
+using Nbrs = std::vector<std::vector<int>>;
+
+Nbrs skeleton_neighbours(const uint8_t *skel, int p) {
+    Nbrs nbr(p);
+    for (int i = 0; i < p; ++i) {
+        const uint8_t *row = skel + (size_t)i * p;
         for (int j = 0; j < p; ++j)
-            if (skel[(size_t)i * p + j]) nbr[i].push_back(j);
+            if (row[j]) nbr[i].push_back(j);
+    }
+    return nbr;
+}
+
+void orient_v_structures(const uint8_t *skel, int p, const int32_t *sep, int L, const Nbrs &nbr, Graph &dag) {
     for (int v2 = 0; v2 < p; ++v2)
-        for (int v1 = 0; v1 < p; ++v1) {
-            if (!skel[(size_t)v1 * p + v2]) continue;
+        for (int v1 : nbr[v2]) {  // skel is symmetric: v1 -> v2 is a skeleton arc
             for (int v3 : nbr[v2]) {
                 if (v3 == v1 || skel[(size_t)v1 * p + v3]) continue;
                 if (L > 0 && (in_sepset(sep, p, L, v2, v1, v3) || in_sepset(sep, p, L, v2, v3, v1))) continue;
@@ -59,7 +70,7 @@ void orient_v_structures(const uint8_t *skel, int p, const int32_t *sep, int L, 
 }
 
 // one pass of R1, R2, R3; returns whether any rule fired
-bool apply_rules_once(Graph &dag, std::vector<uint8_t> &snap) {
+bool apply_rules_once(Graph &dag, std::vector<uint8_t> &snap, const Nbrs &nbr) {
     const int p = dag.p;
     const size_t pp = (size_t)p * p;
     bool changed = false;
@@ -69,9 +80,9 @@ bool apply_rules_once(Graph &dag, std::vector<uint8_t> &snap) {
     // R1: v1 -> v2 - v3, v1 and v3 non-adjacent  =>  v2 -> v3
     std::memcpy(snap.data(), dag.a, pp);
     for (int v2 = 0; v2 < p; ++v2)
-        for (int v1 = 0; v1 < p; ++v1) {
+        for (int v1 : nbr[v2]) {
             if (!g.has(v1, v2) || g.has(v2, v1)) continue;
-            for (int v3 = 0; v3 < p; ++v3) {
+            for (int v3 : nbr[v2]) {
                 if (v3 == v1 || !g.has(v2, v3)) continue;
                 if (g.has(v3, v2) && non_adjacent(v1, v3) && dag.has(v2, v3) && dag.has(v3, v2)) {
                     dag.set(v3, v2, 0);
@@ -83,9 +94,9 @@ bool apply_rules_once(Graph &dag, std::vector<uint8_t> &snap) {
     // R2: v1 -> v3 -> v2 and v1 - v2  =>  v1 -> v2
     std::memcpy(snap.data(), dag.a, pp);
     for (int v2 = 0; v2 < p; ++v2)
-        for (int v1 = 0; v1 < p; ++v1) {
+        for (int v1 : nbr[v2]) {
             if (!g.has(v1, v2) || !g.has(v2, v1)) continue;
-            for (int v3 = 0; v3 < p; ++v3) {
+            for (int v3 : nbr[v1]) {
                 if (!g.has(v1, v3) || !g.has(v3, v2)) continue;
                 if (!g.has(v3, v1) && !g.has(v2, v3)) {
                     dag.set(v1, v2, 1);
@@ -99,10 +110,10 @@ bool apply_rules_once(Graph &dag, std::vector<uint8_t> &snap) {
     std::memcpy(snap.data(), dag.a, pp);
     std::vector<int> C;
     for (int v2 = 0; v2 < p; ++v2)
-        for (int v1 = 0; v1 < p; ++v1) {
+        for (int v1 : nbr[v2]) {
             if (!g.has(v1, v2) || !g.has(v2, v1)) continue;
             C.clear();
-            for (int v = 0; v < p; ++v)
+            for (int v : nbr[v1])
                 if (g.has(v1, v) && g.has(v, v1) && g.has(v, v2) && !g.has(v2, v)) C.push_back(v);
             for (size_t x = 0; x < C.size(); ++x)
                 for (size_t y = x + 1; y < C.size(); ++y)
@@ -123,12 +134,13 @@ extern "C" int gpucsl_orient_edges(const uint8_t *skeleton, int p, const int32_t
     const size_t pp = (size_t)p * p;
     std::memcpy(dag, skeleton, pp);
     Graph g{p, dag};
-    orient_v_structures(skeleton, p, sepsets, max_level, g);
+    const Nbrs nbr = skeleton_neighbours(skeleton, p);
+    orient_v_structures(skeleton, p, sepsets, max_level, nbr, g);
     std::vector<uint8_t> snap(pp);
     int n = 0;
     for (;;) {
         ++n;
-        if (!apply_rules_once(g, snap)) break;
+        if (!apply_rules_once(g, snap, nbr)) break;
     }
     if (passes) *passes = n;
     return 0;

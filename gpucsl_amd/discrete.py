@@ -214,9 +214,9 @@ class DiscreteEngine:
         return nat.decode_stats(self.stats.cpu().numpy().view(np.uint64), self.max_level)
 
     def results(self) -> DiscreteResult:
-        skel = self.skel16.cpu().numpy().view(np.uint16).copy()
-        sep = self.sep.cpu().numpy()[:, :, : self.max_level].copy()
-        return DiscreteResult(skel, sep, self.pmax.cpu().numpy().copy(), self.level_stats())
+        skel, sep, pmax, st = nat.to_host(self.skel16, self.sep, self.pmax, self.stats)
+        return DiscreteResult(skel.view(np.uint16), sep[:, :, : self.max_level], pmax,
+                              nat.decode_stats(st.view(np.uint64), self.max_level))
 
     def ci_test(self, vi: int, vj: int, S) -> tuple:
         """One CI test on the loaded data: (table, chi2, p, df)."""

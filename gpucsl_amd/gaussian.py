@@ -169,7 +169,6 @@ class GaussianEngine:
         return nat.decode_stats(self.stats.cpu().numpy().view(np.uint64), self.max_level)
 
     def results(self) -> GaussianResult:
-        skel = self.skel16.cpu().numpy().view(np.uint16).copy()
-        sep = self.sep.cpu().numpy()[:, :, : self.max_level].copy()
-        return GaussianResult(skel, sep, self.pmax.cpu().numpy().copy(),
-                              self.zmin.cpu().numpy().copy(), self.level_stats())
+        skel, sep, pmax, zmin, st = nat.to_host(self.skel16, self.sep, self.pmax, self.zmin, self.stats)
+        return GaussianResult(skel.view(np.uint16), sep[:, :, : self.max_level], pmax, zmin,
+                              nat.decode_stats(st.view(np.uint64), self.max_level))

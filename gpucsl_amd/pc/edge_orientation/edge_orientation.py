@@ -39,6 +39,23 @@ def orient_dense(skeleton: np.ndarray, separation_sets: np.ndarray) -> np.ndarra
     return dag
 
 
+def _digraph_of(dag: np.ndarray) -> nx.DiGraph:
+    p = dag.shape[0]
+    out = nx.DiGraph()
+    out.add_nodes_from(range(p))
+    ii, jj = np.nonzero(dag)
+    out.add_edges_from(zip(ii.tolist(), jj.tolist()))
+    return out
+
+
+@timed
+def orient_skeleton(skeleton: np.ndarray, separation_sets: np.ndarray) -> nx.DiGraph:
+    """orient_edges for a skeleton given as its (p, p) adjacency array (what
+    the skeleton routines return): the same CPDAG without building a networkx
+    graph of the skeleton first."""
+    return _digraph_of(orient_dense(skeleton, separation_sets))
+
+
 @timed
 def orient_edges(skeleton: nx.Graph, separation_sets: np.ndarray) -> nx.DiGraph:
     """skeleton: undirected graph (or a DiGraph with both arcs per edge);
@@ -47,12 +64,7 @@ def orient_edges(skeleton: nx.Graph, separation_sets: np.ndarray) -> nx.DiGraph:
         p = skeleton.number_of_nodes()
         adj = nx.to_numpy_array(skeleton, nodelist=range(p), weight=None, dtype=np.uint8)
         adj = np.maximum(adj, adj.T)
-        dag = orient_dense(adj, separation_sets)
-        out = nx.DiGraph()
-        out.add_nodes_from(range(p))
-        ii, jj = np.nonzero(dag)
-        out.add_edges_from(zip(ii.tolist(), jj.tolist()))
-        return out
+        return _digraph_of(orient_dense(adj, separation_sets))
     dag = skeleton.to_directed()
     orient_v_structure(dag, separation_sets, skeleton)
     apply_rules(dag)

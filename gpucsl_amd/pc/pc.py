@@ -10,10 +10,9 @@ orientation step are shared.
 from abc import ABC, abstractmethod
 from typing import List
 
-import networkx as nx
 import numpy as np
 
-from gpucsl_amd.pc.edge_orientation.edge_orientation import orient_edges
+from gpucsl_amd.pc.edge_orientation.edge_orientation import orient_skeleton
 from gpucsl_amd.pc.helpers import PCResult, init_pc_graph, timed
 from gpucsl_amd.pc.pc_validations import (
     determine_max_level,
@@ -53,7 +52,9 @@ class PC(ABC):
     def execute(self):
         skeleton_run = self.discover_skeleton()
         sk = skeleton_run.result
-        orientation = orient_edges(nx.DiGraph(sk.skeleton), sk.seperation_sets)
+        # the CPDAG of orient_edges(nx.DiGraph(skeleton), sepsets), from the
+        # adjacency array directly
+        orientation = orient_skeleton(sk.skeleton, sk.seperation_sets)
         return PCResult(orientation.result, sk.seperation_sets, sk.pmax, skeleton_run.runtime,
                         orientation.runtime, sk.discover_skeleton_kernel_time)
 

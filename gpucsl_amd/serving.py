@@ -28,7 +28,6 @@ from __future__ import annotations
 import ctypes as ct
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -180,7 +179,6 @@ class SkeletonServer:
         self.dev_mem = torch.empty((self.budget,), dtype=torch.uint8, device="cuda")
         self.host_mem = torch.empty((self.host_budget,), dtype=torch.uint8, pin_memory=True)
         self._ws = None
-        self._copying: list = []
         # copy_results=False hands out views of the pinned pool (no host copy);
         # the caller gives them back with Completed.release().  With
         # results_on_device the skeleton, sepsets, pmax and zmin stay in the
@@ -188,9 +186,11 @@ class SkeletonServer:
         # the discrete statistics) until Completed.release().
         self.copy_results = bool(copy_results)
         self.results_on_device = bool(results_on_device)
-        self._workers = ThreadPoolExecutor(max_workers=4, thread_name_prefix="gpucsl-results")
         self.gstream = torch.cuda.Stream()
         self.dstreams = [torch.cuda.Stream() for _ in range(max(1, discrete_streams))]
+        # results cross to the pinned pool on their own stream, so the copies
+        # of finished jobs overlap the next rounds' kernels
+        self.cstream = torch.cuda.Stream()
         self._rr = 0
         self.pending: deque = deque()
         self.active: list = []
@@ -224,7 +224,7 @@ class SkeletonServer:
         fp = request_footprint(req)
         if fp > self.budget:
             raise GpucslError(f"request needs {fp} bytes of HBM, more than the budget {self.budget}")
-        if result_bytes(req) > self.host_budget:
+        if not (self.copy_results and not self.results_on_device) and result_bytes(req) > self.host_budget:
             raise GpucslError(f"request results need {result_bytes(req)} bytes, more than the host budget")
         req.rid = self._next_id
         self._next_id += 1
@@ -234,7 +234,7 @@ class SkeletonServer:
 
     @property
     def idle(self) -> bool:
-        return not self.pending and not self.active and not self._copying
+        return not self.pending and not self.active
 
     # ------------------------------------------------------------ scheduling
     def tick(self) -> list:
@@ -261,12 +261,10 @@ class SkeletonServer:
         timeout passes)."""
         events = [q[0][0] for q in self.outstanding.values() if q]
         events += [j.done_event for j in self.active if j.done_event is not None]
-        futs = [f for _, f in self._copying]
-        if not events and not futs:
+        if not events:
             return
         end = time.perf_counter() + timeout
-        while (not any(e.query() for e in events) and not any(f.done() for f in futs)
-               and time.perf_counter() < end):
+        while not any(e.query() for e in events) and time.perf_counter() < end:
             time.sleep(5e-5)
 
     def drain(self) -> list:
@@ -291,8 +289,11 @@ class SkeletonServer:
             dr = self.dev_pool.alloc(request_footprint(req))
             if dr is None:
                 break
-            hr = self.host_pool.alloc(result_bytes(req))
-            if hr is None:
+            # owned results get pinned tensors of their own at the finish
+            # stage; zero-copy and device results use the pinned pool
+            owned = self.copy_results and not self.results_on_device
+            hr = None if owned else self.host_pool.alloc(result_bytes(req))
+            if hr is None and not owned:
                 self.dev_pool.release(*dr)
                 break
             self.pending.popleft()
@@ -324,8 +325,19 @@ class SkeletonServer:
             self.active.append(job)
 
     def _host_views(self, j, torch):
-        """Pinned result buffers of a job inside the host pool."""
+        """Pinned result buffers of a job: inside the host pool, or (owned
+        results) pinned tensors of its own that the caller keeps."""
         p, L = j.req.p, max(j.req.max_level, 1)
+        if j.host_range is None:
+            out = {"skeleton": torch.empty((p, p), dtype=torch.int16, pin_memory=True),
+                   "sepsets": torch.empty((p, p, L), dtype=torch.int32, pin_memory=True),
+                   "pmax": torch.empty((p, p), dtype=torch.float32, pin_memory=True)}
+            if j.req.kind == "gaussian":
+                out["zmin"] = torch.empty((p, p), dtype=torch.float32, pin_memory=True)
+            else:
+                out["stats"] = torch.empty(((j.req.max_level + 1) * nat.STAT_SLOTS,), dtype=torch.int64,
+                                           pin_memory=True)
+            return out
         base, pp = j.host_range[0], p * p
         sizes = [("skeleton", torch.int16, (p, p), 2 * pp), ("sepsets", torch.int32, (p, p, L), 4 * pp * L),
                  ("pmax", torch.float32, (p, p), 4 * pp)]
@@ -424,13 +436,17 @@ class SkeletonServer:
             j.done_event.record(self.gstream)
             return
         h = self._host_views(j, torch)
-        h["skeleton"].copy_(st.view("skeleton", torch.int16, (p, p)), non_blocking=True)
-        h["sepsets"].copy_(st.view("sepsets", torch.int32, (p, p, L)), non_blocking=True)
-        h["pmax"].copy_(st.view("pmax", torch.float32, (p, p)), non_blocking=True)
-        h["zmin"].copy_(st.view("zmin", torch.float32, (p, p)), non_blocking=True)
-        j.host = h
-        j.done_event = torch.cuda.Event()
-        j.done_event.record(self.gstream)
+        ready = torch.cuda.Event()
+        ready.record(self.gstream)
+        self.cstream.wait_event(ready)
+        with torch.cuda.stream(self.cstream):
+            h["skeleton"].copy_(st.view("skeleton", torch.int16, (p, p)), non_blocking=True)
+            h["sepsets"].copy_(st.view("sepsets", torch.int32, (p, p, L)), non_blocking=True)
+            h["pmax"].copy_(st.view("pmax", torch.float32, (p, p)), non_blocking=True)
+            h["zmin"].copy_(st.view("zmin", torch.float32, (p, p)), non_blocking=True)
+            j.host = h
+            j.done_event = torch.cuda.Event()
+            j.done_event.record(self.cstream)
 
     def _step_discrete(self, j) -> None:
         torch = self.torch
@@ -443,26 +459,30 @@ class SkeletonServer:
             else:
                 eng.finish()
                 h = self._host_views(j, torch)
-                h["stats"].copy_(eng.stats, non_blocking=True)
-                if self.results_on_device:
-                    h["skeleton"], h["sepsets"], h["pmax"] = eng.skel16, eng.sep, eng.pmax
-                else:
-                    h["skeleton"].copy_(eng.skel16, non_blocking=True)
-                    h["sepsets"].copy_(eng.sep, non_blocking=True)
-                    h["pmax"].copy_(eng.pmax, non_blocking=True)
-                j.host = h
-                j.done_event = torch.cuda.Event()
-                j.done_event.record(j.stream)
+                ready = torch.cuda.Event()
+                ready.record(j.stream)
+                self.cstream.wait_event(ready)
+                with torch.cuda.stream(self.cstream):
+                    h["stats"].copy_(eng.stats, non_blocking=True)
+                    if self.results_on_device:
+                        h["skeleton"], h["sepsets"], h["pmax"] = eng.skel16, eng.sep, eng.pmax
+                    else:
+                        h["skeleton"].copy_(eng.skel16, non_blocking=True)
+                        h["sepsets"].copy_(eng.sep, non_blocking=True)
+                        h["pmax"].copy_(eng.pmax, non_blocking=True)
+                    j.host = h
+                    j.done_event = torch.cuda.Event()
+                    j.done_event.record(self.cstream)
                 return
         j.stage += 1
 
     def _reap(self) -> list:
         """Jobs whose device work and result copy are complete give back their
-        device range at once; their results are copied out of the pinned pool
-        by worker threads (large fresh host arrays cost page faults, which must
-        not stall the launches), and the request completes when its copy has.
-        With results on the device a finished job completes at once and keeps
-        its device range until the caller releases the result."""
+        device range at once and complete: owned results are the pinned
+        tensors their copy went into, zero-copy results are views of the
+        pinned pool until released.  With results on the device a finished job
+        completes at once and keeps its device range until the caller
+        releases the result."""
         still, zero_copy = [], []
         for j in self.active:
             if j.done_event is not None and j.done_event.query():
@@ -475,8 +495,9 @@ class SkeletonServer:
                     zero_copy.append(c)
                     continue
                 self.dev_pool.release(*j.dev_range)
-                if self.copy_results:
-                    self._copying.append((j, self._workers.submit(self._result, j)))
+                if j.host_range is None:
+                    # owned results: the pinned tensors are the caller's
+                    zero_copy.append(self._result(j, copy=False))
                 else:
                     c = self._result(j, copy=False)
                     c._release = (lambda hr=j.host_range: self.host_pool.release(*hr))
@@ -484,14 +505,7 @@ class SkeletonServer:
             else:
                 still.append(j)
         self.active = still
-        out, busy = zero_copy, []
-        for j, fut in self._copying:
-            if fut.done():
-                out.append(fut.result())
-                self.host_pool.release(*j.host_range)
-            else:
-                busy.append((j, fut))
-        self._copying = busy
+        out = zero_copy
         # release launch tables and workspaces of completed rounds
         for q in self.outstanding.values():
             while q and q[0][0].query():

@@ -231,3 +231,23 @@ def test_orientation_recovers_the_cpdag_of_random_dags():
         assert _python_orientation(skel, sep) == want, f"trial {trial}"
         checked_directed += sum(1 for (a, b) in want if (b, a) not in want)
     assert checked_directed > 20
+
+
+def test_orient_skeleton_array_equals_orient_edges():
+    """The array form PC.execute uses gives the graph orient_edges gives for
+    the networkx skeleton."""
+    from gpucsl_amd.pc.edge_orientation.edge_orientation import orient_skeleton
+
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        p = int(rng.integers(3, 30))
+        skel = np.triu((rng.random((p, p)) < 0.3).astype(np.uint16), 1)
+        skel = skel + skel.T
+        sep = np.full((p, p, 2), -1, np.int32)
+        for a in range(p):
+            for b in range(a + 1, p):
+                if not skel[a, b] and rng.random() < 0.5:
+                    sep[a, b, 0] = sep[b, a, 0] = int(rng.integers(0, p))
+        a, _ = orient_skeleton(skel, sep)
+        b, _ = orient_edges(nx.DiGraph(skel), sep)
+        assert set(a.edges) == set(b.edges) and sorted(a.nodes) == sorted(b.nodes)
