@@ -12,6 +12,7 @@ from pathlib import Path
 from gpucsl_amd.cli.cli_util import GpucslError
 
 LIB_PATH = Path(__file__).resolve().parent / "libgpucsl_hip.so"
+DEBUG_LIB_PATH = Path(__file__).resolve().parent / "libgpucsl_hip_debug.so"
 
 _P = ct.c_void_p
 _I = ct.c_int
@@ -70,25 +71,50 @@ STAT_SLOTS = 8
 NOALLOC_SCRATCH_BYTES = 64 << 20   # GPUCSL_NOALLOC_SCRATCH_BYTES
 STAT_NAMES = ("executed", "total", "rechecked", "flipped", "max_err_ratio", "max_degree", "useful")
 
-_lib = None
+_libs = {}
+_debug = False
 
 
 def load():
-    """Load the library (once).  Raises GpucslError if it is not built."""
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
+    """The library calls go to (each loaded once): the release build, or its
+    index-checked debug build while a ``debug_library(True)`` block is active
+    (process-wide).  Raises GpucslError if it is not built."""
+    key = "debug" if _debug else "release"
+    lib = _libs.get(key)
+    if lib is None:
+        path = DEBUG_LIB_PATH if _debug else LIB_PATH
+        if not path.exists():
             raise GpucslError(
-                f"{LIB_PATH.name} is not built; run `python -m gpucsl_amd.build` "
+                f"{path.name} is not built; run `python -m gpucsl_amd.build` "
                 "(hipcc --offload-arch=gfx950)")
-        lib = ct.CDLL(str(LIB_PATH))
+        lib = ct.CDLL(str(path))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = lib
+        _libs[key] = lib
         _apply_tuning_table()
-    return _lib
+    return lib
+
+
+class debug_library:
+    """Context manager: calls inside go to the debug build when ``on``
+    (gpucsl's is_debug, which compiles its kernels with device asserts,
+    gpucsl/pc/kernel_management.py:32)."""
+
+    def __init__(self, on: bool):
+        self.on = bool(on)
+
+    def __enter__(self):
+        global _debug
+        self.prev = _debug
+        _debug = self.on or _debug
+        return self
+
+    def __exit__(self, *exc):
+        global _debug
+        _debug = self.prev
+        return False
 
 
 TUNING_TABLE = Path(__file__).resolve().parent / "tuning.json"

@@ -11,6 +11,21 @@
 // levels above run kernels that take the level at run time
 #define GPUCSL_MAX_LEVEL_TEMPLATE 8
 
+// Index checks of the debug build (-DGPUCSL_DEBUG, libgpucsl_hip_debug.so):
+// a failed check prints the condition and its place and stops the kernel; the
+// release build compiles them away.
+#ifdef GPUCSL_DEBUG
+#define GPUCSL_DCHECK(cond)                                                                   \
+    do {                                                                                      \
+        if (!(cond)) {                                                                        \
+            printf("gpucsl debug check failed: %s (%s:%d)\n", #cond, __FILE__, __LINE__);     \
+            __builtin_trap();                                                                 \
+        }                                                                                     \
+    } while (0)
+#else
+#define GPUCSL_DCHECK(cond) ((void)0)
+#endif
+
 namespace gpucsl {
 
 // ----------------------------------------------------------------------------
@@ -190,7 +205,8 @@ __device__ __forceinline__ void atomic_max_nonneg(float *addr, float v) {
 }
 
 // ----------------------------------------------------------------------------
-// Fisher-z in FP64, in the reference's operation order
+// Fisher-z in FP64, in the reference's operation order (host and device: the
+// CPU tests also run these bodies in a host build under ASan/UBSan)
 // (gpucsl/pc/cuda/helpers/gaussian_helpers.cu:16-262).  Used only to recheck
 // FP32 tests that land within the error margin of the threshold, so the
 // removal decision is the one the FP64 reference takes.
@@ -198,7 +214,7 @@ __device__ __forceinline__ void atomic_max_nonneg(float *addr, float v) {
 namespace ref64 {
 
 template <int D1, int D2, int D3>
-__device__ __forceinline__ void mm_nn(const double *A, const double *B, double *R) {
+__host__ __device__ __forceinline__ void mm_nn(const double *A, const double *B, double *R) {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int r = 0; r < D1; ++r)
@@ -211,7 +227,7 @@ __device__ __forceinline__ void mm_nn(const double *A, const double *B, double *
         }
 }
 template <int D1, int D2, int D3>
-__device__ __forceinline__ void mm_tn(const double *A, const double *B, double *R) {
+__host__ __device__ __forceinline__ void mm_tn(const double *A, const double *B, double *R) {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int r = 0; r < D1; ++r)
@@ -224,7 +240,7 @@ __device__ __forceinline__ void mm_tn(const double *A, const double *B, double *
         }
 }
 template <int D1, int D2, int D3>
-__device__ __forceinline__ void mm_nt(const double *A, const double *B, double *R) {
+__host__ __device__ __forceinline__ void mm_nt(const double *A, const double *B, double *R) {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int r = 0; r < D1; ++r)
@@ -238,7 +254,7 @@ __device__ __forceinline__ void mm_nt(const double *A, const double *B, double *
 }
 
 template <int D>
-__device__ __forceinline__ void gj_inverse(double *M, double *Inv) {
+__host__ __device__ __forceinline__ void gj_inverse(double *M, double *Inv) {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int r = 0; r < D; ++r)
@@ -267,7 +283,7 @@ __device__ __forceinline__ void gj_inverse(double *M, double *Inv) {
 }
 
 template <int D>
-__device__ __forceinline__ void cholesky(const double *A, double *Lo) {
+__host__ __device__ __forceinline__ void cholesky(const double *A, double *Lo) {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int t = 0; t < D * D; ++t) Lo[t] = 0.0;
@@ -287,7 +303,7 @@ __device__ __forceinline__ void cholesky(const double *A, double *Lo) {
 
 // z for (vi, vj | S) from the FP64 correlation matrix.
 template <int L>
-__device__ __noinline__ double gauss_z(const double *__restrict__ C, int p, int vi, int vj,
+__host__ __device__ __noinline__ double gauss_z(const double *__restrict__ C, int p, int vi, int vj,
                                        const int *S) {
 #pragma clang fp contract(off)
     double M0[4], M1[2 * L], M2[L * L], T[L * L], Lo[L * L], R[L * L], M2i[L * L], W[2 * L], H[4];
@@ -324,7 +340,7 @@ __device__ __noinline__ double gauss_z(const double *__restrict__ C, int p, int 
 }
 
 // Level-0 Fisher-z, gpucsl/pc/cuda/gaussian_ci.cu:36-37.
-__device__ __forceinline__ double gauss_z0(double r) {
+__host__ __device__ __forceinline__ double gauss_z0(double r) {
 #pragma clang fp contract(off)
     return fabs(0.5 * log(fabs((1 + r) / (1 - r))));
 }
@@ -335,7 +351,7 @@ __device__ __forceinline__ double gauss_z0(double r) {
 // Perlman's poz / pchisq (Hill & Pike Algorithm 299), FP64, published method
 // (gpucsl/pc/cuda/helpers/math_helpers.cu:54-175).
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ double poz(double z) {
+__host__ __device__ __forceinline__ double poz(double z) {
 #pragma clang fp contract(off)
     double x;
     if (z == 0.0) {
@@ -378,9 +394,9 @@ __device__ __forceinline__ double poz(double z) {
     return z > 0.0 ? ((x + 1.0) * 0.5) : ((1.0 - x) * 0.5);
 }
 
-__device__ __forceinline__ double ex20(double v) { return v < -20.0 ? 0.0 : exp(v); }
+__host__ __device__ __forceinline__ double ex20(double v) { return v < -20.0 ? 0.0 : exp(v); }
 
-__device__ __forceinline__ double pchisq(double x, int df) {
+__host__ __device__ __forceinline__ double pchisq(double x, int df) {
 #pragma clang fp contract(off)
     if (x <= 0.0 || df < 1) return 1.0;
     const double a = 0.5 * x;

@@ -87,6 +87,7 @@ __device__ void count_table(const uint8_t *__restrict__ data, int64_t n, int64_t
                 unsigned g = 0;
 #pragma unroll
                 for (int c = 0; c < L; ++c) g += (((&ws[c].x)[word] >> sh) & 0xffu) * (unsigned)radix[c];
+                GPUCSL_DCHECK(a < (unsigned)di && b < (unsigned)dj && (g * di + a) * dj + b < (unsigned)cells);
                 atomicAdd(&mine[(g * di + a) * dj + b], 1u);
             }
         }
@@ -480,6 +481,7 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level_kernel(
         const int jpos = (int)(o / per_pair);
         const unsigned long long rank = o - (unsigned long long)jpos * per_pair;
         const int32_t *row = nbr + (size_t)i * p;
+        GPUCSL_DCHECK(i >= 0 && i < p && jpos >= 0 && jpos < d);
         const int j = row[jpos];
         int pos[L];
         IArr<L> S, radix;
@@ -596,6 +598,7 @@ __device__ void count_table_rt(const uint8_t *__restrict__ data, int64_t n, int6
                 const int word = q >> 2, sh = (q & 3) * 8;
                 const unsigned a = ((&wa.x)[word] >> sh) & 0xffu;
                 const unsigned b = ((&wb.x)[word] >> sh) & 0xffu;
+                GPUCSL_DCHECK(a < (unsigned)di && b < (unsigned)dj && (g[q] * di + a) * dj + b < (unsigned)cells);
                 atomicAdd(&mine[(g[q] * di + a) * dj + b], 1u);
             }
         }

@@ -855,6 +855,7 @@ __device__ __forceinline__ void subset_of(const ItemCtx &c, uint64_t rank, int j
     for (int k = 0; k < L; ++k) {
         const int idx = pos[k] - 1;
         sp[k] = idx + (idx >= jpos);
+        GPUCSL_DCHECK(sp[k] >= 0 && sp[k] < c.d && sp[k] != jpos);
     }
 }
 
@@ -1057,6 +1058,7 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         c.count = binom_k<L>(c.Nn);
         const int team = team_lanes(c.count, tm);
         c.q0 = it.q0;
+        GPUCSL_DCHECK(c.i >= 0 && c.i < c.p && c.d < c.p && c.cnt <= c.d && c.q0 < c.cnt);
         c.q1 = min(c.cnt, c.q0 + items_pairs(c.count, tm));
         c.jbase = side == 0 ? c.d - c.cnt : 0;
         c.D = c.d + 1;
@@ -1361,6 +1363,7 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 const int d = jw.deg[i];
                 jpos = side == 0 ? d - (rowoff[i + 1] - rowoff[i]) + k : k;
                 gnb = jw.nbr + (size_t)i * p;
+                GPUCSL_DCHECK(i >= 0 && i < p && jpos >= 0 && jpos < d);
                 j = gnb[jpos];
                 Nn = d - 1;
                 count = binom_k<L>(Nn);
@@ -1699,6 +1702,7 @@ __global__ __launch_bounds__(GEN_THREADS) void gauss_level_generic_kernel(const 
         const int cnt = rowoff[i + 1] - rowoff[i];
         const int jpos = (side == 0 ? d - cnt : 0) + (w - rowoff[i]);
         const int32_t *nb = jw.nbr + (size_t)i * p;
+        GPUCSL_DCHECK(i >= 0 && i < p && jpos >= 0 && jpos < d);
         const int j = nb[jpos];
         if (side == 1 && J.G[(size_t)i * p + j] != 1) continue;  // side A removed the edge
         const int N = d - 1;
@@ -1709,6 +1713,7 @@ __global__ __launch_bounds__(GEN_THREADS) void gauss_level_generic_kernel(const 
         for (uint64_t r = 0; r < count; ++r) {
             if (lane < L) {
                 const int idx = pos - 1;
+                GPUCSL_DCHECK(idx >= 0 && idx < N);
                 m.S[lane] = nb[idx + (idx >= jpos)];
             }
             __syncthreads();

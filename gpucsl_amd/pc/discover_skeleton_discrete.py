@@ -22,6 +22,30 @@ def discover_skeleton_gpu_discrete(
     is_debug: bool = False,
     should_log: bool = False,
 ) -> SkeletonResult:
+    """The skeleton routine above the C ABI (see the body below); with
+    is_debug its calls go to the index-checked debug build of the library, as
+    the reference compiles its kernels with device asserts
+    (gpucsl/pc/kernel_management.py:32)."""
+    from gpucsl_amd import _native as nat
+
+    with nat.debug_library(is_debug):
+        return _discrete_skeleton(skeleton=skeleton, data=data, alpha=alpha, max_level=max_level, num_variables=num_variables,
+                                  num_observations=num_observations, kernels=kernels, memory_restriction=memory_restriction,
+                                  is_debug=is_debug, should_log=should_log)
+
+
+def _discrete_skeleton(
+    skeleton: np.ndarray,
+    data: np.ndarray,
+    alpha: float,
+    max_level: int,
+    num_variables: int,
+    num_observations: int,
+    kernels=None,
+    memory_restriction: int = None,
+    is_debug: bool = False,
+    should_log: bool = False,
+) -> SkeletonResult:
     """Signature and result of gpucsl/pc/discover_skeleton_discrete.py:24-128.
 
     Numeric data are uploaded once and ordinal-encoded on the GPU into a padded

@@ -41,6 +41,34 @@ def discover_skeleton_gpu_gaussian(
     sync_device: int = None,
     pcalg_compat: bool = False,
 ) -> SkeletonResult:
+    """The skeleton routine above the C ABI (see the body below); with
+    is_debug its calls go to the index-checked debug build of the library, as
+    the reference compiles its kernels with device asserts
+    (gpucsl/pc/kernel_management.py:32)."""
+    from gpucsl_amd import _native as nat
+
+    with nat.debug_library(is_debug):
+        return _gaussian_skeleton(skeleton=skeleton, data=data, correlation_matrix=correlation_matrix, alpha=alpha,
+                                  max_level=max_level, num_variables=num_variables, num_observations=num_observations,
+                                  kernels=kernels, is_debug=is_debug, should_log=should_log, devices=devices,
+                                  sync_device=sync_device, pcalg_compat=pcalg_compat)
+
+
+def _gaussian_skeleton(
+    skeleton: np.ndarray,
+    data: np.ndarray,
+    correlation_matrix: np.ndarray,
+    alpha: float,
+    max_level: int,
+    num_variables: int,
+    num_observations: int,
+    kernels=None,
+    is_debug: bool = False,
+    should_log: bool = False,
+    devices: List[int] = [0],
+    sync_device: int = None,
+    pcalg_compat: bool = False,
+) -> SkeletonResult:
     """Signature and result of gpucsl/pc/discover_skeleton_gaussian.py:116-172.
 
     ``skeleton`` contributes its shape only (level 0 sets every pair, as in

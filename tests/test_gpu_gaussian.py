@@ -528,3 +528,23 @@ def test_fused_outputs_equal_separate_entry_points():
     torch.cuda.synchronize()
     assert np.array_equal(pm.cpu().numpy().view(np.uint32), res.pmax.view(np.uint32))
     assert np.array_equal(sk.cpu().numpy().view(np.uint16), res.skeleton)
+
+
+def test_debug_build_runs_and_matches_release():
+    """is_debug=True runs the index-checked debug build of the library (every
+    check holds on valid input) and gives the release build's results."""
+    from gpucsl_amd.pc.pc import DiscretePC, GaussianPC
+
+    X = syn.gaussian_sem(120, 3000, 3.0, seed=41).data
+    rel = GaussianPC(X, 3, 0.01).set_distribution_specific_options().execute().result
+    dbg = GaussianPC(X, 3, 0.01, is_debug=True).set_distribution_specific_options().execute().result
+    assert "debug" in nat._libs and not nat._debug
+    assert set(rel.directed_graph.edges) == set(dbg.directed_graph.edges)
+    assert (rel.separation_sets == dbg.separation_sets).all()
+    assert np.array_equal(rel.pmax.view(np.uint32), dbg.pmax.view(np.uint32))
+    d = syn.discrete_bn(25, 4000, 2.0, seed=42).data
+    rel = DiscretePC(d, 3, 0.05).set_distribution_specific_options().execute().result
+    dbg = DiscretePC(d, 3, 0.05, is_debug=True).set_distribution_specific_options().execute().result
+    assert set(rel.directed_graph.edges) == set(dbg.directed_graph.edges)
+    assert (rel.separation_sets == dbg.separation_sets).all()
+    assert np.array_equal(rel.pmax.view(np.uint32), dbg.pmax.view(np.uint32))

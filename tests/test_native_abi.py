@@ -21,14 +21,29 @@ def test_header_parses():
     assert len(names) >= 15
 
 
-def test_library_exports_header_symbols():
+@pytest.mark.parametrize("which", ["LIB_PATH", "DEBUG_LIB_PATH"])
+def test_library_exports_header_symbols(which):
+    """The release library and its index-checked debug build export every
+    entry point the header declares."""
     from gpucsl_amd import _native
 
-    if not _native.LIB_PATH.exists():
-        pytest.fail("libgpucsl_hip.so is not built (run __graft_entry__.build())")
-    lib = ct.CDLL(str(_native.LIB_PATH))
+    path = getattr(_native, which)
+    if not path.exists():
+        pytest.fail(f"{path.name} is not built (run __graft_entry__.build())")
+    lib = ct.CDLL(str(path))
     for name in declared():
         assert hasattr(lib, name), name
+
+
+def test_debug_library_selection_is_scoped():
+    from gpucsl_amd import _native
+
+    assert not _native._debug
+    with _native.debug_library(True):
+        assert _native._debug
+        with _native.debug_library(False):
+            assert _native._debug          # an inner non-debug call keeps the outer choice
+    assert not _native._debug
 
 
 def test_bindings_cover_header():
