@@ -48,6 +48,9 @@ SIGNATURES = {
     "gpucsl_gauss_skeleton_noalloc": (_I, [_P, _P, _I, _I, _P, _F, _I, _P, _P, _P, _P, _P]),
     "gpucsl_set_level_tuning": (_I, [_I, _P]),
     "gpucsl_get_level_tuning": (_I, [_I, _P]),
+    "gpucsl_set_bucket_tuning": (_I, [_I, _I, _P]),
+    "gpucsl_get_bucket_tuning": (_I, [_I, _I, _P]),
+    "gpucsl_tuning_bucket": (_I, [_I, _I]),
     "gpucsl_pmax_from_zmin": (_I, [_P, _I, _I64, _P, _P]),
     "gpucsl_compact": (_I, [_P, _I, _P, _P, _SZ, _P]),
     "gpucsl_skeleton_to_u16": (_I, [_P, _I, _P, _P]),

@@ -101,8 +101,9 @@ void level_list_views(void *ws, int p, int32_t **nbr, int32_t **deg);
 int pmax_launch(const float *zmin, int p, int64_t n, float *pmax, hipStream_t s);
 int outputs_launch(const float *zmin, const uint8_t *Gm, int p, int64_t n, float *pmax, uint16_t *out, hipStream_t s);
 size_t gauss_batch_ws_bytes(const Job *jobs_host, int nj);
-int set_level_tuning(int level, const gpucsl_level_tuning *t);
-int get_level_tuning(int level, gpucsl_level_tuning *t);
+int set_level_tuning(int bucket, int level, const gpucsl_level_tuning *t);
+int get_level_tuning(int bucket, int level, gpucsl_level_tuning *t);
+int tuning_bucket(int nj, int max_p);
 int gauss_batch_init(const Job *jh, const Job *jd, int nj, hipStream_t s);
 int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, unsigned long long *st, hipStream_t s);
 int gauss_batch_compact(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
@@ -369,9 +370,24 @@ int gpucsl_gauss_batch_skeleton(const gpucsl_gauss_job *jobs, const gpucsl_gauss
     return gauss_batch_finish(jobs, jobs_dev, nj, S(stream));
 }
 
-int gpucsl_set_level_tuning(int level, const gpucsl_level_tuning *tuning) { return set_level_tuning(level, tuning); }
+int gpucsl_set_level_tuning(int level, const gpucsl_level_tuning *tuning) { return set_level_tuning(-1, level, tuning); }
 
-int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning) { return get_level_tuning(level, tuning); }
+int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning) {
+    return get_level_tuning(GPUCSL_TUNING_LARGE, level, tuning);
+}
+
+int gpucsl_set_bucket_tuning(int bucket, int level, const gpucsl_level_tuning *tuning) {
+    return set_level_tuning(bucket, level, tuning);
+}
+
+int gpucsl_get_bucket_tuning(int bucket, int level, gpucsl_level_tuning *tuning) {
+    return get_level_tuning(bucket, level, tuning);
+}
+
+int gpucsl_tuning_bucket(int jobs, int max_p) {
+    if (jobs < 1 || max_p < 1) return set_error(ERR_ARG, "tuning_bucket: jobs and max_p must be >= 1"), -1;
+    return tuning_bucket(jobs, max_p);
+}
 
 int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void *stream) {
     return pmax_launch(zmin, p, n, pmax, S(stream));

@@ -75,52 +75,68 @@ __host__ __device__ __forceinline__ uint64_t binom_level(int m, int level) {
     }
 }
 
-// per-level launch shapes of the FAST kernel (settable at run time, e.g. from
-// an autotuning table); index 0 is unused
-// (process-wide, defaults on first use; reads and writes under one mutex)
+// Launch shapes of the FAST kernel per (launch bucket, level), settable at run
+// time from an autotuning table.  The bucket is chosen on the host from the
+// launch: one dataset with p <= 256, one dataset with p > 256, or a batch of
+// several; the degree of a row is handled on the device, where each row's
+// subset count C(deg - 1, l) picks its team size against the shape's
+// thresholds.  Level index 0 is unused.  Process-wide, defaults on first
+// use; reads and writes under one mutex.
 struct TuningTable {
-    gpucsl_level_tuning lv[GPUCSL_MAX_LEVEL_TEMPLATE + 1];
+    gpucsl_level_tuning lv[GPUCSL_TUNING_BUCKETS][GPUCSL_MAX_LEVEL_TEMPLATE + 1];
 };
 static TuningTable &tuning_table() {
     static TuningTable t = [] {
         TuningTable x;
-        for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) x.lv[l] = gpucsl_level_tuning{8, 8, 4, 32768, 4096, 0};
+        for (int b = 0; b < GPUCSL_TUNING_BUCKETS; ++b)
+            for (int l = 0; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l)
+                x.lv[b][l] = gpucsl_level_tuning{8, 8, 4, 32768, 4096, 0};
         return x;
     }();
     return t;
 }
 static std::mutex g_tuning_mu;
-static gpucsl_level_tuning tuning_of(int level) {
-    std::lock_guard<std::mutex> lk(g_tuning_mu);
-    return tuning_table().lv[level];
+
+int tuning_bucket(int nj, int max_p) {
+    if (nj > 1) return GPUCSL_TUNING_BATCH;
+    return max_p <= 256 ? GPUCSL_TUNING_SMALL : GPUCSL_TUNING_LARGE;
 }
 
-static Team team_for(int level, int gs_min) {
-    const gpucsl_level_tuning t = tuning_of(level);
+static gpucsl_level_tuning tuning_of(int bucket, int level) {
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    return tuning_table().lv[bucket][level];
+}
+
+static Team team_for(int bucket, int level, int gs_min) {
+    const gpucsl_level_tuning t = tuning_of(bucket, level);
     if (t.wave_items) return Team{gs_min, gs_min * t.small_team_iters, 0x7fffffff, t.item_tests, 1};
     return Team{gs_min, gs_min * t.small_team_iters, 64 * t.wave_team_iters, t.item_tests, 0};
 }
 
-int set_level_tuning(int level, const gpucsl_level_tuning *t) {
+// bucket -1: every bucket; level 0: every level
+int set_level_tuning(int bucket, int level, const gpucsl_level_tuning *t) {
     if (!t) return set_error(ERR_ARG, "tuning: null");
     if (t->small_team_iters < 1 || t->wave_team_iters < 1 || t->fast_blocks_per_cu < 1 || t->fast_blocks_per_cu > 32 ||
         t->fast_lds_bytes < 4096 || t->fast_lds_bytes > FB_MAX_LDS_BYTES || t->fast_lds_bytes % 1024 ||
         t->item_tests < 64 || t->item_tests > (1 << 24) || t->wave_items < 0 || t->wave_items > 1)
         return set_error(ERR_ARG, "tuning: a field is out of range");
-    std::lock_guard<std::mutex> lk(g_tuning_mu);
-    if (level == 0) {
-        for (int l = 1; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l) tuning_table().lv[l] = *t;
-    } else if (level >= 1 && level <= GPUCSL_MAX_LEVEL_TEMPLATE) {
-        tuning_table().lv[level] = *t;
-    } else {
+    if (bucket < -1 || bucket >= GPUCSL_TUNING_BUCKETS)
+        return set_error(ERR_ARG, "tuning: bucket %d outside -1..%d", bucket, GPUCSL_TUNING_BUCKETS - 1);
+    if (level < 0 || level > GPUCSL_MAX_LEVEL_TEMPLATE)
         return set_error(ERR_ARG, "tuning: level %d outside 0..%d", level, GPUCSL_MAX_LEVEL_TEMPLATE);
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    for (int b = 0; b < GPUCSL_TUNING_BUCKETS; ++b) {
+        if (bucket >= 0 && b != bucket) continue;
+        for (int l = 1; l <= GPUCSL_MAX_LEVEL_TEMPLATE; ++l)
+            if (level == 0 || l == level) tuning_table().lv[b][l] = *t;
     }
     return OK;
 }
 
-int get_level_tuning(int level, gpucsl_level_tuning *t) {
-    if (!t || level < 1 || level > GPUCSL_MAX_LEVEL_TEMPLATE) return set_error(ERR_ARG, "tuning: bad level or null");
-    *t = tuning_of(level);
+int get_level_tuning(int bucket, int level, gpucsl_level_tuning *t) {
+    if (!t || level < 1 || level > GPUCSL_MAX_LEVEL_TEMPLATE || bucket < 0 || bucket >= GPUCSL_TUNING_BUCKETS)
+        return set_error(ERR_ARG, "tuning: bad bucket or level, or null");
+    *t = tuning_of(bucket, level);
     return OK;
 }
 
@@ -1800,8 +1816,8 @@ __global__ void to_u16_kernel(const uint8_t *__restrict__ g, size_t count, uint1
 // host side: launches over a job table
 // ----------------------------------------------------------------------------
 static int slow_grid() { return device_cus() / 2; }
-static int fast_grid(int level) {
-    return device_cus() * tuning_of(level).fast_blocks_per_cu;
+static int fast_grid(int bucket, int level) {
+    return device_cus() * tuning_of(bucket, level).fast_blocks_per_cu;
 }
 
 size_t gauss_batch_ws_bytes(const Job *jobs_host, int nj) {
@@ -1843,7 +1859,8 @@ int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, uns
 static int compact_and_plan(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
                             const PlanArgs &plan, hipStream_t s) {
     const int mp = max_p(jh, nj);
-    const Team team = team_for(level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
+    const Team team = team_for(tuning_bucket(nj, mp),
+                               level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
                                gs_min);
     GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd, level, team, st, plan);
     // the scan only when a plan or the level statistics are wanted (the
@@ -1870,20 +1887,22 @@ static const char *const kPhaseSlowName[] = {
 // Side 0: the FAST kernel.  Side 1: the FAST kernel, then the SLOW kernel for
 // the pairs both sides queued (and the level's statistics).
 template <int L>
-static void launch_phase(const Job *jd, int nj, const BatchWs &w, int side, int gs_min, float margin_k,
+static void launch_phase(const Job *jd, int nj, int bucket, const BatchWs &w, int side, int gs_min, float margin_k,
                          unsigned long long *st, hipStream_t s) {
     const int32_t *base = side == 0 ? w.itembaseA : w.itembaseB;
     const WorkItem *items = side == 0 ? w.itemsA : w.itemsB;
     SlowItem *queue = side == 0 ? w.queueA : w.queueB;
     unsigned int *qcount = w.counters + CNT_QUEUE + side;
-    const gpucsl_level_tuning tl = tuning_of(L);
+    const gpucsl_level_tuning tl = tuning_of(bucket, L);
     const int lds = tl.fast_lds_bytes;
     if (tl.wave_items)
-        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_wave_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj,
-                      base, items, side, team_for(L, gs_min), lds / 16, margin_k, queue, qcount, w.bstats);
+        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_wave_kernel<L>, dim3(fast_grid(bucket, L)), dim3(256), lds, s,
+                      jd, nj, base, items, side, team_for(bucket, L, gs_min), lds / 16, margin_k, queue, qcount,
+                      w.bstats);
     else
-        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(L)), dim3(256), lds, s, jd, nj,
-                      base, items, side, team_for(L, gs_min), lds / 4, margin_k, queue, qcount, w.bstats);
+        GPUCSL_LAUNCH(kPhaseFastName[L], gauss_level_fast_kernel<L>, dim3(fast_grid(bucket, L)), dim3(256), lds, s,
+                      jd, nj, base, items, side, team_for(bucket, L, gs_min), lds / 4, margin_k, queue, qcount,
+                      w.bstats);
     if (side == 0) return;
     GPUCSL_LAUNCH(kPhaseSlowName[L], (gauss_slow_kernel<L, 64>), dim3(slow_grid()), dim3(256), 0, s, jd, margin_k,
                   w.counters + STRIPES, w.queueA, w.counters + CNT_QUEUE, w.queueB, w.counters + CNT_QUEUE + 1,
@@ -1904,6 +1923,7 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
         if (!jh[b].C32 || !jh[b].C64 || !jh[b].G || !jh[b].zmin || !jh[b].ws || (jh[b].max_level > 0 && !jh[b].sepsets))
             return set_error(ERR_ARG, "gauss_level: job %d has a null buffer", b);
     BatchWs w = carve_batch(bws, nj, qcap, icap);
+    const int bucket = tuning_bucket(nj, max_p(jh, nj));
     const PlanArgs plan{w.counters + CNT_PLAN_DONE, nj, w.itembaseA, w.itembaseB, w.counters, w.bstats, w.itemsA,
                         w.itemsB};
     int rc = compact_and_plan(jh, jd, nj, level, group_size, st, plan, s);
@@ -1920,7 +1940,7 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
         switch (level) {
 #define GPUCSL_CASE(LV)                                                     \
     case LV:                                                                \
-        launch_phase<LV>(jd, nj, w, side, group_size, margin_k, st, s);     \
+        launch_phase<LV>(jd, nj, bucket, w, side, group_size, margin_k, st, s); \
         break;
             GPUCSL_CASE(1) GPUCSL_CASE(2) GPUCSL_CASE(3) GPUCSL_CASE(4)
             GPUCSL_CASE(5) GPUCSL_CASE(6) GPUCSL_CASE(7) GPUCSL_CASE(8)
@@ -2127,7 +2147,8 @@ int compact_launch(const uint8_t *Gm, int p, int level, void *ws, unsigned long 
     Job j = single_job(p, level, nullptr, nullptr, const_cast<uint8_t *>(Gm), nullptr, nullptr, nullptr, 0, ws);
     if (!stats) {
         GPUCSL_LAUNCH("compact", compact_one_kernel, dim3((p + 3) / 4, 1), dim3(256), 0, s, j, level,
-                      team_for(level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
+                      team_for(tuning_bucket(1, p),
+                               level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
                                16));
         return check_hip(hipGetLastError(), "compact launch");
     }

@@ -145,3 +145,19 @@ def discrete_bn_device(p: int, n: int, degree: float = 2.0, seed: int = 0, card_
     codes[:, :n] = torch.gather(remap, 1, body).to(torch.uint8)
     dims = present.sum(dim=1).to(torch.uint8)
     return codes, dims
+
+
+def batched_workload(n_boot: int, n_sets: int):
+    """Bootstrap resamples of one p=200, n=5000 dataset (degree 3, seed 300;
+    resample b draws its rows with seed 300+b) and uneven datasets k with
+    p ~ U[50, 500], n ~ U[1000, 20000] drawn with seed 400+k (degree 3, seed
+    400+k).  Returns the base data, the row indices of every resample and the
+    uneven datasets (float32)."""
+    base = gaussian_sem(200, 5000, 3.0, seed=300).data
+    rows = [np.random.Generator(np.random.PCG64(300 + b)).integers(0, 5000, size=5000) for b in range(n_boot)]
+    sets = []
+    for k in range(n_sets):
+        rng = np.random.Generator(np.random.PCG64(400 + k))
+        p, n = int(rng.integers(50, 501)), int(rng.integers(1000, 20001))
+        sets.append(gaussian_sem(p, n, 3.0, seed=400 + k).data.astype(np.float32))
+    return base.astype(np.float32), rows, sets

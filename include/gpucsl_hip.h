@@ -223,6 +223,23 @@ typedef struct gpucsl_level_tuning {
 int gpucsl_set_level_tuning(int level, const gpucsl_level_tuning *tuning);
 int gpucsl_get_level_tuning(int level, gpucsl_level_tuning *tuning);
 
+/* The table is kept per launch bucket: GPUCSL_TUNING_SMALL (one dataset, p <=
+ * 256), GPUCSL_TUNING_LARGE (one dataset, p > 256) and GPUCSL_TUNING_BATCH
+ * (several datasets in one launch).  gpucsl_set_level_tuning sets every
+ * bucket, gpucsl_get_level_tuning reads GPUCSL_TUNING_LARGE; the calls below
+ * address one bucket (level 0 in a set: every level).  A row's degree is
+ * handled on the device: its subset count C(deg - 1, l) picks the team size
+ * against the shape's thresholds.  gpucsl_tuning_bucket gives the bucket of a
+ * launch of `jobs` datasets whose largest has max_p variables (-1 on bad
+ * arguments). */
+#define GPUCSL_TUNING_SMALL 0
+#define GPUCSL_TUNING_LARGE 1
+#define GPUCSL_TUNING_BATCH 2
+#define GPUCSL_TUNING_BUCKETS 3
+int gpucsl_set_bucket_tuning(int bucket, int level, const gpucsl_level_tuning *tuning);
+int gpucsl_get_bucket_tuning(int bucket, int level, gpucsl_level_tuning *tuning);
+int gpucsl_tuning_bucket(int jobs, int max_p);
+
 /* pmax = 2 (1 - Phi(sqrt(n-3) |zmin|)), -1 on and below the diagonal.
  * Replaces transform_to_pmax_cupy + postprocess_pmax_cupy
  * (gpucsl/pc/helpers.py:62-82). */

@@ -86,30 +86,56 @@ def test_job_struct_layout_matches_header():
 
 
 def test_level_tuning_roundtrip_and_validation():
-    """Launch shapes are host state of the library: settable per level or for
-    every level, range-checked, readable back (no GPU needed)."""
+    """Launch shapes are host state of the library, per (bucket, level):
+    settable for one bucket and level, every level or every bucket,
+    range-checked, readable back (no GPU needed)."""
     from gpucsl_amd import _native as nat
     from gpucsl_amd.cli.cli_util import GpucslError
-    from gpucsl_amd.tune import get_level, set_level
+    from gpucsl_amd.tune import BUCKETS, get_shape, set_shape
 
-    nat.load()
-    before = {lv: get_level(lv) for lv in range(1, 9)}
+    lib = nat.load()
+    assert lib.gpucsl_tuning_bucket(1, 200) == BUCKETS["small"]
+    assert lib.gpucsl_tuning_bucket(1, 1000) == BUCKETS["large"]
+    assert lib.gpucsl_tuning_bucket(320, 500) == BUCKETS["batch"]
+    assert lib.gpucsl_tuning_bucket(0, 10) == -1
+    before = {(b, lv): get_shape(b, lv) for b in range(3) for lv in range(1, 9)}
     try:
         shape = dict(small_team_iters=2, wave_team_iters=4, fast_blocks_per_cu=8, fast_lds_bytes=16384,
                      item_tests=2048, wave_items=1)
-        set_level(3, shape)
-        assert get_level(3) == shape
-        set_level(0, dict(shape, fast_blocks_per_cu=2))
-        assert all(get_level(lv)["fast_blocks_per_cu"] == 2 for lv in range(1, 9))
+        set_shape(1, 3, shape)
+        assert get_shape(1, 3) == shape
+        assert get_shape(0, 3) == before[(0, 3)] and get_shape(2, 3) == before[(2, 3)]
+        set_shape(2, 0, dict(shape, fast_blocks_per_cu=2))
+        assert all(get_shape(2, lv)["fast_blocks_per_cu"] == 2 for lv in range(1, 9))
+        set_shape(-1, 4, dict(shape, item_tests=4096))
+        assert all(get_shape(b, 4)["item_tests"] == 4096 for b in range(3))
         for bad in (dict(shape, fast_lds_bytes=70000), dict(shape, small_team_iters=0),
                     dict(shape, fast_lds_bytes=5000), dict(shape, item_tests=10), dict(shape, wave_items=2)):
             with pytest.raises(GpucslError):
-                set_level(1, bad)
+                set_shape(1, 1, bad)
         with pytest.raises(GpucslError):
-            set_level(9, shape)
+            set_shape(1, 9, shape)
+        with pytest.raises(GpucslError):
+            set_shape(3, 1, shape)
     finally:
-        for lv, sh in before.items():
-            set_level(lv, sh)
+        for (b, lv), sh in before.items():
+            set_shape(b, lv, sh)
+
+
+def test_tuning_table_buckets_apply():
+    from gpucsl_amd import _native as nat
+    from gpucsl_amd.tune import apply_table, get_shape, set_shape
+
+    nat.load()
+    before = {(b, lv): get_shape(b, lv) for b in range(3) for lv in range(1, 9)}
+    try:
+        apply_table({"buckets": {"batch": {"levels": {"2": {"fast_blocks_per_cu": 24}}},
+                                 "small": {"levels": {"2": {"item_tests": 256}}}}})
+        assert get_shape(2, 2)["fast_blocks_per_cu"] == 24 and get_shape(0, 2)["item_tests"] == 256
+        assert get_shape(1, 2) == before[(1, 2)]
+    finally:
+        for (b, lv), sh in before.items():
+            set_shape(b, lv, sh)
 
 
 def test_corr_job_workspace_holds_digit_planes():
