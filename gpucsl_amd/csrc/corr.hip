@@ -449,33 +449,85 @@ constexpr int I8_MAXU = 6;        // digit pairs (a, b) with a + b <= I8_MAXU
 // (16 consecutive columns, one chunk) hit 16 different groups of four banks.
 constexpr int I8_COLB = 64;
 
-// Split-K partial Gram tile (ta <= tb) of 64 x 64 from the digit planes: the
-// 26 digit pairs (a, b) with a + b <= 6, D_a^T D_b accumulated exactly in int32
-// per weight u = a + b (at most 6 pairs x 64 rows x 64^2 per step, so a split
-// of at most I8_MAX_STEPS steps stays below 2^31), then combined as
-// sum_u T_u 128^-u in FP64 from the smallest weight up.  Block: 4 waves in 2 x 2,
-// each wave a 32 x 32 quarter (2 x 2 MFMA blocks of 16 x 16).  A 64-row step of
-// both column panels (6 digits x 64 columns x 64 bytes each) is staged in LDS;
-// the next step's panels are loaded into registers while the current one is
-// multiplied.  On a diagonal tile (ta == tb) the two panels are the same and
-// the wave below the diagonal, whose entries are never read, only loads.
-__global__ __launch_bounds__(256, 2) void gram_i8_kernel(const Job *__restrict__ jobs) {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][I8_PLANES][64][I8_COLB];
-    const Job &J = jobs[blockIdx.z];
+// Blocks of the integer Gram launch for one dataset: the upper 64 x 64 tiles
+// (ta <= tb) taken two at a time along each tile row, (ta, ta) with
+// (ta, ta + 1), then (ta, ta + 2) with (ta, ta + 3), ...; with t tiles a side
+// that is sum over k = 1..t of ceil(k / 2) = (t + 1)^2 / 4 blocks.
+__host__ __device__ inline int i8_pair_blocks(int p) {
+    const int t = (p + CT - 1) / CT;
+    return (t + 1) * (t + 1) / 4;
+}
+
+// Split-K partial Gram tiles (ta, tb0) and (ta, tb0 + 1) from the digit planes:
+// the 26 digit pairs (a, b) with a + b <= 6, D_a^T D_b accumulated exactly in
+// int32 per weight u = a + b (at most 6 pairs x 64 rows x 64^2 per step, so a
+// split of at most I8_MAX_STEPS steps stays below 2^31), then combined as
+// sum_u T_u 128^-u in FP64 from the smallest weight up.  Block: 4 waves, wave
+// w computes rows 32 (w / 2) .. + 31 of tile (ta, tb0 + w % 2), i.e. 2 x 4 MFMA
+// blocks of 16 x 16 per digit pair; a wave whose tile lies past the last
+// column tile only loads.  A 64-row step of the panels (6 digits x the A tile
+// and the two B tiles, 4 KB each) is staged in LDS by direct global-to-LDS
+// loads, double-buffered: the next step's panels stream into one buffer while
+// the current step is multiplied out of the other, with one barrier per step.
+constexpr int I8_PANEL_SLOTS = 3;  // A tile, B tiles tb0 and tb0 + 1
+typedef int8_t I8Panels[I8_PLANES][I8_PANEL_SLOTS][64][I8_COLB];
+typedef __attribute__((address_space(3))) void *LdsPtr;
+
+// The integer Gram launch covers only the blocks that have work: the jobs of
+// a launch that take the integer path, up to I8_LAUNCH_JOBS of them, job[s]
+// owning blocks first[s] .. first[s + 1] - 1 (pair blocks x splits, split
+// major).  Passed by value, so a captured launch keeps it.
+constexpr int I8_LAUNCH_JOBS = 64;
+struct I8Launch {
+    int count;
+    int job[I8_LAUNCH_JOBS];
+    int first[I8_LAUNCH_JOBS + 1];
+};
+
+// Workgroups go to the 8 XCDs in turn (workgroup w to XCD w % 8), and each
+// XCD has its own L2.  A launch of one job is renumbered so that XCD k runs a
+// contiguous range of (split, pair) blocks: blocks of one split that share
+// panel tiles then meet in the same L2 (a bijection on the launch).  A launch
+// of several jobs keeps the dispatch order, which spreads each job's blocks
+// over all XCDs; its jobs come longest blocks first.
+__device__ __forceinline__ void gram_block_order(const I8Launch &L, int &job, int &local) {
+    const unsigned N = gridDim.x, hw = blockIdx.x;
+    const unsigned q = N / 8, r = N % 8, k = hw % 8;
+    const int id = L.count == 1 ? (int)(k * q + (k < r ? k : r) + hw / 8) : (int)hw;
+    int lo = 0, hi = L.count - 1;  // the slot s with first[s] <= id < first[s + 1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (L.first[mid] <= id)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    job = L.job[lo];
+    local = id - L.first[lo];
+}
+
+__global__ __launch_bounds__(256, 1) void gram_i8_kernel(const Job *__restrict__ jobs, I8Launch launch) {
+    // two separate arrays (not one indexed by step parity), so the compiler
+    // sees that reads of one buffer do not wait for the loads into the other
+    __shared__ __attribute__((aligned(16))) I8Panels lds0;
+    __shared__ __attribute__((aligned(16))) I8Panels lds1;
+    int job_id, local;
+    gram_block_order(launch, job_id, local);
+    const Job &J = jobs[job_id];
     const int p = J.p;
     const int64_t n = J.n;
-    if (!gram_int8(J.x_dtype, p) || (int)blockIdx.x >= upper_tiles(p) || (int)blockIdx.y >= J.k_splits) return;
+    const int split = local / i8_pair_blocks(p);
     const CorrWs cw = carve_corr(J);
     const int tiles = (p + CT - 1) / CT;
-    int t = blockIdx.x, ta = 0;
-    while (t >= tiles - ta) {
-        t -= tiles - ta;
+    int x = local % i8_pair_blocks(p), ta = 0;
+    for (;;) {
+        const int nb = (tiles - ta + 1) >> 1;
+        if (x < nb) break;
+        x -= nb;
         ++ta;
     }
-    const int tb = ta + t;
-    const int a0 = ta * CT, b0 = tb * CT;
-    const bool diag = ta == tb;
-    const int split = blockIdx.y;
+    const int tb0 = ta + 2 * x;
+    const bool two = tb0 + 1 < tiles;  // tile (ta, tb0 + 1) exists
     const int64_t npad = i8_rows(n);
     const int ppad = i8_cols(p);
     const int64_t steps = npad / 64;
@@ -485,109 +537,122 @@ __global__ __launch_bounds__(256, 2) void gram_i8_kernel(const Job *__restrict__
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int half = wave & 1, wr = (wave >> 1) * 32;
     const int ci = lane & 15, kq = lane >> 4;
-    const bool idle = diag && wr > wc;
+    const bool idle = half == 1 && !two;
 
-    // loader: thread -> bytes 16 tid .. 16 tid + 15 of each 4 KB panel tile,
-    // i.e. column tid / 4, 16-byte chunk tid % 4
+    // LDS panel tile: 64 bytes per column, the four 16-byte chunks of column c
+    // in the order q ^ ((c / 4) % 4).  A wave's direct load writes 1 KB of a
+    // tile, lane l at byte 16 l, so thread tid fetches the chunk that belongs at
+    // byte 16 tid: column tid / 4, chunk (tid % 4) ^ ((tid / 16) % 4).  Slot 1
+    // holds B tile tb0 and slot 2 B tile tb0 + 1, where a tile that does not
+    // exist is replaced by the A tile, so that the loads take no branches.
+    // Tile offsets and LDS addresses are wave-uniform (M0 comes from scalars).
     const int CB = ppad / 64;
     const size_t plane_stride = (size_t)CB * steps * I8_TILE;
-    const int8_t *srcA = cw.planes + i8_tile(0, ta, 0, CB, steps) + 16 * tid;
-    const int8_t *srcB = cw.planes + i8_tile(0, tb, 0, CB, steps) + 16 * tid;
-    i32x4 ra[I8_PLANES], rb[I8_PLANES];
-    const int wchunk = (tid & 3) ^ ((tid >> 4) & 3);  // column tid / 4
-    const int rchunk = kq ^ ((ci >> 2) & 3);          // column 16 m + ci
-    auto load = [&](int64_t st) {
-        const size_t off = (size_t)st * I8_TILE;
-#pragma unroll
-        for (int a = 0; a < I8_PLANES; ++a) ra[a] = *(const i32x4 *)(srcA + a * plane_stride + off);
-        if (!diag)
-#pragma unroll
-            for (int a = 0; a < I8_PLANES; ++a) rb[a] = *(const i32x4 *)(srcB + a * plane_stride + off);
+    const unsigned gsrc = 16u * (tid ^ ((tid >> 4) & 3));
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+    const size_t offA = i8_tile(0, ta, 0, CB, steps);
+    const size_t offB0 = i8_tile(0, tb0, 0, CB, steps);
+    const size_t offB1 = two ? offB0 + (size_t)steps * I8_TILE : offA;  // column tile tb0 + 1
+    const int rchunk = kq ^ ((ci >> 2) & 3);                           // column 16 m + ci
+    // direct load of panel tile t (digit t / 3, slot t % 3) of step st into buf
+    auto issue = [&](I8Panels &buf, int64_t st, int t) {
+        const int a = t / I8_PANEL_SLOTS, slot = t % I8_PANEL_SLOTS;
+        const size_t off = (size_t)st * I8_TILE + a * plane_stride + (slot == 0 ? offA : slot == 1 ? offB0 : offB1);
+        __builtin_amdgcn_global_load_lds(cw.planes + off + gsrc, (LdsPtr)&buf[a][slot][16 * wave_u][0], 16, 0, 0);
     };
-    auto store = [&]() {
-#pragma unroll
-        for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[0][a][tid >> 2][16 * wchunk] = ra[a];
-        if (!diag)
-#pragma unroll
-            for (int a = 0; a < I8_PLANES; ++a) *(i32x4 *)&lds[1][a][tid >> 2][16 * wchunk] = rb[a];
-    };
+    constexpr int I8_STEP_TILES = I8_PLANES * I8_PANEL_SLOTS;
 
-    i32x4 acc[I8_MAXU + 1][2][2];
+    i32x4 acc[I8_MAXU + 1][2][4];
 #pragma unroll
     for (int u = 0; u <= I8_MAXU; ++u)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[u][i][j] = i32x4{0, 0, 0, 0};
+            for (int j = 0; j < 4; ++j) acc[u][i][j] = i32x4{0, 0, 0, 0};
 
-    const int bp = diag ? 0 : 1;
-    if (s0 < s1) {
-        load(s0);
-        store();
-    }
-    __syncthreads();
-    for (int64_t st = s0; st < s1; ++st) {
-        if (st + 1 < s1) load(st + 1);
-        if (!idle) {
-            // B fragments of digits 0..3 are held for the whole step; those of
-            // digits 4 and 5 (paired only with a <= 2 and a <= 1) are read where
-            // used, which leaves the registers to read the next digit's A
-            // fragments while the current digit's products run.
-            auto frag = [&](int buf, int digit, int col) {
-                return *(const i32x4 *)&lds[buf][digit][col][16 * rchunk];
-            };
-            i32x4 bf[4][2];
+    // One step: the products of the panels in buf, and (with nxt) the direct
+    // loads of step nst into the other buffer.  The digit pairs are taken in
+    // shells of max(a, b) = 0, 1, ..., 5, so that the first products need only
+    // the fragments of digit 0: the fragments of digit d + 1 are read while the
+    // products of shell d run, and all of them stay in registers for the step.
+    // The next step's 18 tile loads go out two after each of the first nine
+    // digit pairs, between their products, so that they have the rest of the
+    // step to land and never hold up the matrix pipe for long.
+    auto step = [&](const I8Panels &buf, I8Panels *nxt, int64_t nst) {
+        if (idle) {
+            if (nxt)
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+                for (int t = 0; t < I8_STEP_TILES; ++t) issue(*nxt, nst, t);
+            return;
+        }
+        const int bslot = 1 + half;
+        i32x4 af[I8_PLANES][2], bf[I8_PLANES][4];
+        auto read = [&](int d) {
 #pragma unroll
-                for (int j = 0; j < 2; ++j) bf[b][j] = frag(bp, b, wc + 16 * j + ci);
-            i32x4 af[2], an[2];
+            for (int i = 0; i < 2; ++i) af[d][i] = *(const i32x4 *)&buf[d][0][wr + 16 * i + ci][16 * rchunk];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) af[i] = frag(0, 0, wr + 16 * i + ci);
+            for (int j = 0; j < 4; ++j) bf[d][j] = *(const i32x4 *)&buf[d][bslot][16 * j + ci][16 * rchunk];
+        };
+        int k = 0;  // digit pairs done this step
+        auto pair = [&](int a, int b) {
 #pragma unroll
-            for (int a = 0; a < I8_PLANES; ++a) {
-                if (a + 1 < I8_PLANES)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) an[i] = frag(0, a + 1, wr + 16 * i + ci);
-                __builtin_amdgcn_sched_barrier(0);  // keep those reads ahead of the products
-#pragma unroll
-                for (int b = 0; b < I8_PLANES && a + b <= I8_MAXU; ++b) {
-                    i32x4 bb[2];
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) bb[j] = b < 4 ? bf[b < 4 ? b : 0][j] : frag(bp, b, wc + 16 * j + ci);
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-                            acc[a + b][i][j] =
-                                __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bb[j], acc[a + b][i][j], 0, 0, 0);
-                }
-                if (a + 1 < I8_PLANES)
-#pragma unroll
-                    for (int i = 0; i < 2; ++i) af[i] = an[i];
+                for (int j = 0; j < 4; ++j)
+                    acc[a + b][i][j] =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a][i], bf[b][j], acc[a + b][i][j], 0, 0, 0);
+            if (nxt && 2 * k < I8_STEP_TILES) {
+                issue(*nxt, nst, 2 * k);
+                issue(*nxt, nst, 2 * k + 1);
             }
+            ++k;
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        read(0);
+#pragma unroll
+        for (int d = 0; d < I8_PLANES; ++d) {
+            // the reads of digit d + 1 go out after the products of shell d - 1
+            // and before those of shell d (at most 12 reads in flight)
+            if (d + 1 < I8_PLANES) read(d + 1);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int e = 0; e < d; ++e)
+                if (e + d <= I8_MAXU) {
+                    pair(e, d);
+                    pair(d, e);
+                }
+            if (2 * d <= I8_MAXU) pair(d, d);
         }
+    };
+
+    // the barrier after each step also waits for this wave's direct loads
+    if (s0 < s1)
+#pragma unroll
+        for (int t = 0; t < I8_STEP_TILES; ++t) issue(lds0, s0, t);
+    __syncthreads();
+    for (int64_t st = s0; st < s1; st += 2) {
+        step(lds0, st + 1 < s1 ? &lds1 : nullptr, st + 1);
         __syncthreads();
-        if (st + 1 < s1) {
-            store();
-            __syncthreads();
-        }
+        if (st + 1 >= s1) break;
+        step(lds1, st + 2 < s1 ? &lds0 : nullptr, st + 2);
+        __syncthreads();
     }
     if (idle) return;
 
     // D layout of v_mfma_i32_16x16x64_i8 (checked on gfx950): accumulator item v
     // of a lane holds column (lane & 15) and row 4 * (lane >> 4) + v.
     double *out = cw.gram + (size_t)split * p * p;
+    const int a0 = ta * CT + wr, b0 = (tb0 + half) * CT;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int col = b0 + wc + j * 16 + ci;
+        for (int j = 0; j < 4; ++j) {
+            const int col = b0 + j * 16 + ci;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = a0 + wr + i * 16 + 4 * (lane >> 4) + v;
+                const int row = a0 + i * 16 + 4 * kq + v;
                 double h = (double)acc[I8_MAXU][i][j][v];
 #pragma unroll
                 for (int u = I8_MAXU - 1; u >= 0; --u) h = h * (1.0 / 128.0) + (double)acc[u][i][j][v];
@@ -715,20 +780,21 @@ __global__ void c64_to_c32_kernel(const Job *__restrict__ jobs) {
 // host side
 // ----------------------------------------------------------------------------
 // Split-K factor of one dataset's Gram matrix: ks in 1..16 (at least 256 rows
-// per split; more when float32 data has more than 16 x 65536 rows) that minimises the busiest compute unit's work when the
-// tiles x ks blocks spread evenly over the device, ceil(tiles ks / CUs) blocks
-// of ceil(steps / ks) row steps each, plus the split-K partial tiles (written
-// once, read once; about `over` steps' time per block).  The integer path
-// (64 x 64 tiles, 64-row steps) and the FP64 path (32 x 32 tiles for
-// p <= GRAM_SMALL_P, 16-row slabs) have their own models.  Depends on
-// (n, p, dtype) and the device only, so a dataset gets the same ks, and the
-// same bits, alone and in any batch.
+// per split; more when float32 data has more than 16 x 65536 rows) that
+// minimises the busiest compute unit's work when the blocks x ks spread evenly
+// over the device, ceil(blocks ks / CUs) blocks of ceil(steps / ks) row steps
+// each, plus the split-K partial tiles (written once, read once; about `over`
+// steps' time per block).  The integer path (blocks of two 64 x 64 tiles, one
+// per compute unit, 64-row steps) and the FP64 path (64 x 64 tiles, or 32 x 32
+// tiles of one wave for p <= GRAM_SMALL_P, 16-row slabs) have their own
+// models.  Depends on (n, p, dtype) and the device only, so a dataset gets the
+// same ks, and the same bits, alone and in any batch.
 int corr_k_splits(int64_t n, int p, int dtype) {
     const bool i8 = gram_int8(dtype, p);
-    // work per split in 4-wave units: a 32 x 32 FP64 tile is one wave
-    const int64_t tu = i8 || gram_tile(p) == CT ? upper_tiles(p) : (upper_tiles(p, 32) + 3) / 4;
+    // blocks per split: integer pair blocks, or FP64 work in 4-wave units
+    const int64_t tu = i8 ? i8_pair_blocks(p) : gram_tile(p) == CT ? upper_tiles(p) : (upper_tiles(p, 32) + 3) / 4;
     const int64_t rows_per_step = i8 ? 64 : KT;
-    const double over = i8 ? 2.0 : 4.0;
+    const double over = 4.0;
     const int64_t cus = device_cus();
     int64_t kmax = (n + 255) / 256;
     if (kmax > 16) kmax = 16;
@@ -754,7 +820,7 @@ int corr_k_splits(int64_t n, int p, int dtype) {
 static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, const Job *one, hipStream_t s) {
     if (nj < 1) return OK;
     int maxp = 0, maxks = 1, maxtiles = 0, maxtiles32 = 0, maxnt = 1, dtype = jobs_host[0].x_dtype;
-    int maxi8 = 0, maxi8p = 0;  // integer path: upper tiles and columns
+    int maxi8 = 0, maxi8p = 0;  // integer path: Gram blocks and columns
     int64_t maxn = 0, maxn8 = 0;
     for (int b = 0; b < nj; ++b) {
         const Job &J = jobs_host[b];
@@ -769,7 +835,7 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
         maxks = J.k_splits > maxks ? J.k_splits : maxks;
         // Gram tiles by path and tile size; normalisation tiles are always 64 x 64
         if (gram_int8(dtype, J.p)) {
-            maxi8 = upper_tiles(J.p) > maxi8 ? upper_tiles(J.p) : maxi8;
+            maxi8 = i8_pair_blocks(J.p) > maxi8 ? i8_pair_blocks(J.p) : maxi8;
             maxi8p = J.p > maxi8p ? J.p : maxi8p;
             maxn8 = J.n > maxn8 ? J.n : maxn8;
         } else {
@@ -794,7 +860,27 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
     if (maxi8 > 0) {
         const unsigned rb = (unsigned)((i8_rows(maxn8) / 64 + I8_SLICE_STEPS - 1) / I8_SLICE_STEPS);
         GPUCSL_LAUNCH("corr_slice_i8", slice_i8_kernel, dim3(rb, (maxi8p + 63) / 64, nj), dim3(256), 0, s, jobs_dev);
-        GPUCSL_LAUNCH("corr_gram_i8", gram_i8_kernel, dim3(maxi8, maxks, nj), dim3(256), 0, s, jobs_dev);
+        // the integer-path jobs, I8_LAUNCH_JOBS per launch, each launch's jobs
+        // longest blocks (row steps per split) first
+        auto block_steps = [&](int b) {
+            return (i8_rows(jobs_host[b].n) / 64 + jobs_host[b].k_splits - 1) / jobs_host[b].k_splits;
+        };
+        I8Launch L{};
+        for (int b = 0; b < nj; ++b) {
+            if (gram_int8(dtype, jobs_host[b].p)) L.job[L.count++] = b;
+            if (L.count > 0 && (L.count == I8_LAUNCH_JOBS || b == nj - 1)) {
+                for (int k = 1; k < L.count; ++k)  // insertion sort, stable
+                    for (int m = k; m > 0 && block_steps(L.job[m]) > block_steps(L.job[m - 1]); --m) {
+                        const int t = L.job[m];
+                        L.job[m] = L.job[m - 1];
+                        L.job[m - 1] = t;
+                    }
+                for (int k = 0; k < L.count; ++k)
+                    L.first[k + 1] = L.first[k] + i8_pair_blocks(jobs_host[L.job[k]].p) * jobs_host[L.job[k]].k_splits;
+                GPUCSL_LAUNCH("corr_gram_i8", gram_i8_kernel, dim3(L.first[L.count]), dim3(256), 0, s, jobs_dev, L);
+                L = I8Launch{};
+            }
+        }
     }
     // 64 x 64 FP64 tiles: float64 data above GRAM_SMALL_P (float32 data there
     // takes the integer path)
