@@ -573,14 +573,34 @@ __global__ __launch_bounds__(256, 1) void gram_i8_kernel(const Job *__restrict__
             for (int j = 0; j < 4; ++j) acc[u][i][j] = i32x4{0, 0, 0, 0};
 
     // One step: the products of the panels in buf, and (with nxt) the direct
-    // loads of step nst into the other buffer.  The digit pairs are taken in
-    // shells of max(a, b) = 0, 1, ..., 5, so that the first products need only
-    // the fragments of digit 0: the fragments of digit d + 1 are read while the
-    // products of shell d run, and all of them stay in registers for the step.
-    // The next step's 18 tile loads go out two after each of the first nine
-    // digit pairs, between their products, so that they have the rest of the
-    // step to land and never hold up the matrix pipe for long.
-    auto step = [&](const I8Panels &buf, I8Panels *nxt, int64_t nst) {
+    // loads of step nst into the other buffer.  The fragments of every digit
+    // are read once per step and held in registers.  The six digit pairs that
+    // use neither digit 0 nor digit 1, (2,2), (2,3), (3,2), (3,3), (2,4) and
+    // (4,2), are deferred to the start of the next step (pend), where they run
+    // on the previous step's fragments of digits 2..4 while the new fragments
+    // of digits 0 and 1 are read, so the matrix pipe does not wait for LDS
+    // after the step's barrier.  The other pairs follow digit by digit, each
+    // digit's fragments read while the previous pairs run.  The next step's
+    // 18 tile loads go out two after each of the first nine of those pairs,
+    // so that they have the rest of the step to land.
+    i32x4 af[I8_PLANES][2], bf[I8_PLANES][4];
+    auto pair = [&](int a, int b) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[a + b][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a][i], bf[b][j], acc[a + b][i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    auto deferred = [&]() {
+        pair(2, 2);
+        pair(2, 3);
+        pair(3, 2);
+        pair(3, 3);
+        pair(2, 4);
+        pair(4, 2);
+    };
+    auto step = [&](const I8Panels &buf, I8Panels *nxt, int64_t nst, bool pend) {
         if (idle) {
             if (nxt)
 #pragma unroll
@@ -588,21 +608,15 @@ __global__ __launch_bounds__(256, 1) void gram_i8_kernel(const Job *__restrict__
             return;
         }
         const int bslot = 1 + half;
-        i32x4 af[I8_PLANES][2], bf[I8_PLANES][4];
         auto read = [&](int d) {
 #pragma unroll
             for (int i = 0; i < 2; ++i) af[d][i] = *(const i32x4 *)&buf[d][0][wr + 16 * i + ci][16 * rchunk];
 #pragma unroll
             for (int j = 0; j < 4; ++j) bf[d][j] = *(const i32x4 *)&buf[d][bslot][16 * j + ci][16 * rchunk];
         };
-        int k = 0;  // digit pairs done this step
-        auto pair = [&](int a, int b) {
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[a + b][i][j] =
-                        __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a][i], bf[b][j], acc[a + b][i][j], 0, 0, 0);
+        int k = 0;  // this step's pairs done
+        auto own = [&](int a, int b) {
+            pair(a, b);
             if (nxt && 2 * k < I8_STEP_TILES) {
                 issue(*nxt, nst, 2 * k);
                 issue(*nxt, nst, 2 * k + 1);
@@ -611,19 +625,25 @@ __global__ __launch_bounds__(256, 1) void gram_i8_kernel(const Job *__restrict__
             __builtin_amdgcn_sched_barrier(0);
         };
         read(0);
+        read(1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (pend) deferred();
+        own(0, 0);
+        read(2);
+        __builtin_amdgcn_sched_barrier(0);
+        own(0, 1);
+        own(1, 0);
+        own(1, 1);
 #pragma unroll
-        for (int d = 0; d < I8_PLANES; ++d) {
-            // the reads of digit d + 1 go out after the products of shell d - 1
-            // and before those of shell d (at most 12 reads in flight)
+        for (int d = 2; d < I8_PLANES; ++d) {
+            // reads of digit d + 1 go out before the pairs of digit d with
+            // digits 0 and 1 (at most 12 reads in flight)
             if (d + 1 < I8_PLANES) read(d + 1);
             __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int e = 0; e < d; ++e)
-                if (e + d <= I8_MAXU) {
-                    pair(e, d);
-                    pair(d, e);
-                }
-            if (2 * d <= I8_MAXU) pair(d, d);
+            own(0, d);
+            own(d, 0);
+            own(1, d);
+            own(d, 1);
         }
     };
 
@@ -633,12 +653,13 @@ __global__ __launch_bounds__(256, 1) void gram_i8_kernel(const Job *__restrict__
         for (int t = 0; t < I8_STEP_TILES; ++t) issue(lds0, s0, t);
     __syncthreads();
     for (int64_t st = s0; st < s1; st += 2) {
-        step(lds0, st + 1 < s1 ? &lds1 : nullptr, st + 1);
+        step(lds0, st + 1 < s1 ? &lds1 : nullptr, st + 1, st > s0);
         __syncthreads();
         if (st + 1 >= s1) break;
-        step(lds1, st + 2 < s1 ? &lds0 : nullptr, st + 2);
+        step(lds1, st + 2 < s1 ? &lds0 : nullptr, st + 2, true);
         __syncthreads();
     }
+    if (!idle && s0 < s1) deferred();
     if (idle) return;
 
     // D layout of v_mfma_i32_16x16x64_i8 (checked on gfx950): accumulator item v
