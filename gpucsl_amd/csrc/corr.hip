@@ -311,7 +311,7 @@ __global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const J
 // tile of every digit.  The column centre and scale come from the row
 // splits' sums and extremes, the splits taken four at a time by the four waves
 // and their partial sums added in wave order.
-constexpr int I8_SLICE_STEPS = 4;
+constexpr int I8_SLICE_STEPS = 2;
 __global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ jobs) {
     const Job &J = jobs[blockIdx.z];
     if (!gram_int8(J.x_dtype, J.p)) return;
@@ -730,33 +730,35 @@ __global__ __launch_bounds__(256) void normalize_tile_kernel(const Job *__restri
                     for (int e = 0; e < R; ++e) g[e] += x[e][u];
         }
     };
-    __shared__ double s_sdr[QR], s_sdc[CT];
-    __shared__ double s_v[QR][CT + 1];
-    if (tid < CT + QR) {
-        const int idx = tid < CT ? b0 + tid : a0 + (tid - CT);
-        double g[1];
-        const size_t off[1] = {idx < p ? (size_t)idx * p + idx : 0};
-        gsum_rows(g, off);
-        const double sd = idx < p ? sqrt(g[0] / nm1) : 1.0;
-        if (tid < CT)
-            s_sdc[tid] = sd;
-        else
-            s_sdr[tid - CT] = sd;
-    }
-    __syncthreads();
-    // thread: column c, rows r0, r0 + 4, r0 + 8, r0 + 12 of the quarter
+    // thread: column c, rows r0, r0 + 4, r0 + 8, r0 + 12 of the quarter, and
+    // (threads below CT + QR) the diagonal entry of one of the tile's columns
+    // or of the quarter's rows, for the standard deviations: all in one pass
+    // of loads
     constexpr int RPT = QR / 4;
     const int c = tid & 63, r0 = tid >> 6;
     const int b = b0 + c;
-    size_t off[RPT];
+    const int didx = tid < CT ? b0 + tid : a0 + (tid - CT);
+    const bool dgo = tid < CT + QR && didx < p;
+    size_t off[RPT + 1];
 #pragma unroll
     for (int e = 0; e < RPT; ++e) {
         const int a = a0 + r0 + 4 * e;
         // lower half of a diagonal tile: the mirrored upper element
         off[e] = (a < p && b < p) ? (a <= b ? (size_t)a * p + b : (size_t)b * p + a) : 0;
     }
-    double g[RPT];
+    off[RPT] = dgo ? (size_t)didx * p + didx : 0;
+    double g[RPT + 1];
     gsum_rows(g, off);
+    __shared__ double s_sdr[QR], s_sdc[CT];
+    __shared__ double s_v[QR][CT + 1];
+    if (tid < CT + QR) {
+        const double sd = dgo ? sqrt(g[RPT] / nm1) : 1.0;
+        if (tid < CT)
+            s_sdc[tid] = sd;
+        else
+            s_sdr[tid - CT] = sd;
+    }
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < RPT; ++e) {
         const int r = r0 + 4 * e, a = a0 + r;
