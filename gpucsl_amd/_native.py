@@ -66,6 +66,11 @@ SIGNATURES = {
     "gpucsl_disc_finish": (_I, [_P, _I, _P, _P, _P]),
     "gpucsl_disc_skeleton": (_I, [_P, _I64, _I64, _P, _I, _I, _I, _D, _P, _P, _P, _P, _P, _SZ, _P]),
     "gpucsl_disc_ci_test": (_I, [_P, _I64, _I64, _P, _I, _I, _P, _I, _I, _P, _P, _P]),
+    "gpucsl_disc_batch_init": (_I, [_P, _I, _P]),
+    "gpucsl_disc_batch_level0": (_I, [_P, _I, _P]),
+    "gpucsl_disc_batch_level": (_I, [_I, _P, _I, _P]),
+    "gpucsl_disc_batch_finish": (_I, [_P, _I, _P]),
+    "gpucsl_disc_batch_skeleton": (_I, [_P, _I, _P]),
     "gpucsl_profile_begin": (_I, []),
     "gpucsl_profile_end": (_I, [ct.c_char_p, _SZ]),
 }

@@ -127,6 +127,10 @@ int disc_single_test_launch(const uint8_t *data, int64_t n, int64_t stride, cons
                             const int32_t *S, int l, int sequential, uint32_t *table_out, double *out,
                             hipStream_t s);
 int lower_minus_one_launch(float *pmax, int p, hipStream_t s);
+int disc_batch_init_launch(const gpucsl_disc_job *jobs, int nj, hipStream_t s);
+int disc_batch_level0_launch(const gpucsl_disc_job *jobs, int nj, hipStream_t s);
+int disc_batch_level_launch(int level, const gpucsl_disc_job *jobs, int nj, hipStream_t s);
+int disc_batch_finish_launch(const gpucsl_disc_job *jobs, int nj, hipStream_t s);
 int to_u16_launch(const uint8_t *Gm, int p, uint16_t *out, hipStream_t s);
 
 // test hook: lexicographic unranking
@@ -455,6 +459,45 @@ int gpucsl_disc_skeleton(const uint8_t *data, int64_t n, int64_t stride, const u
         [&](int lv, unsigned long long *st) { return compact_launch(G, p, lv, ws, st, S(stream)); }, S(stream));
     if (rc) return rc;
     return lower_minus_one_launch(pmax, p, S(stream));
+}
+
+int gpucsl_disc_batch_init(const gpucsl_disc_job *jobs, int nj, void *stream) {
+    return disc_batch_init_launch(jobs, nj, S(stream));
+}
+
+int gpucsl_disc_batch_level0(const gpucsl_disc_job *jobs, int nj, void *stream) {
+    return disc_batch_level0_launch(jobs, nj, S(stream));
+}
+
+int gpucsl_disc_batch_level(int level, const gpucsl_disc_job *jobs, int nj, void *stream) {
+    return disc_batch_level_launch(level, jobs, nj, S(stream));
+}
+
+int gpucsl_disc_batch_finish(const gpucsl_disc_job *jobs, int nj, void *stream) {
+    return disc_batch_finish_launch(jobs, nj, S(stream));
+}
+
+int gpucsl_disc_batch_skeleton(const gpucsl_disc_job *jobs, int nj, void *stream) {
+    int rc = gpucsl_disc_batch_init(jobs, nj, stream);
+    if (rc) return rc;
+    rc = gpucsl_disc_batch_level0(jobs, nj, stream);
+    if (rc) return rc;
+    int top = 0;
+    for (int b = 0; b < nj; ++b) top = jobs[b].max_level > top ? jobs[b].max_level : top;
+    const int built = top < GPUCSL_MAX_LEVEL ? top : GPUCSL_MAX_LEVEL;
+    for (int l = 1; l <= built; ++l) {
+        rc = gpucsl_disc_batch_level(l, jobs, nj, stream);
+        if (rc) return rc;
+    }
+    for (int b = 0; b < nj; ++b) {
+        const gpucsl_disc_job &J = jobs[b];
+        rc = check_unbuilt_levels(
+            J.max_level, J.stats,
+            [&](int lv, unsigned long long *st) { return compact_launch(J.G, J.p, lv, J.ws, st, S(stream)); },
+            S(stream));
+        if (rc) return rc;
+    }
+    return gpucsl_disc_batch_finish(jobs, nj, stream);
 }
 
 int gpucsl_disc_encode(const void *values, int dtype, int64_t n, int p, int64_t ld, int row_major, uint8_t *codes,

@@ -252,19 +252,20 @@ __device__ __forceinline__ uint64_t onehot8(uint64_t x, unsigned a) {
 // gives row/column l % 16 of its block and observations 8 (l / 16) .. +7 of the
 // 32-observation step; accumulator item v holds row 4 (l / 16) + v, column
 // l % 16.  Split-K shares add with int32 atomics, exact in any order.
-__global__ __launch_bounds__(256) void disc_onehot_gram_kernel(const uint8_t *__restrict__ data, int64_t n,
-                                                               int64_t stride, int p, int D, int W,
-                                                               int32_t *__restrict__ N) {
+// Upper tile `tile` of share `split` of `nsplit`.
+__device__ __forceinline__ void disc_gram_body(const uint8_t *__restrict__ data, int64_t n, int64_t stride, int p,
+                                               int D, int W, int32_t *__restrict__ N, int tile, int split,
+                                               int nsplit) {
     const int tiles = W / GRAM_TILE;
-    int t = blockIdx.x, tr = 0;
+    int t = tile, tr = 0;
     while (t >= tiles - tr) {
         t -= tiles - tr;
         ++tr;
     }
     const int tc = tr + t;
     const int64_t steps = (n + 31) / 32;
-    const int64_t per = (steps + gridDim.y - 1) / gridDim.y;
-    const int64_t s0 = (int64_t)blockIdx.y * per;
+    const int64_t per = (steps + nsplit - 1) / nsplit;
+    const int64_t s0 = (int64_t)split * per;
     const int64_t s1 = s0 + per < steps ? s0 + per : steps;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -325,6 +326,12 @@ __global__ __launch_bounds__(256) void disc_onehot_gram_kernel(const uint8_t *__
             }
 }
 
+__global__ __launch_bounds__(256) void disc_onehot_gram_kernel(const uint8_t *__restrict__ data, int64_t n,
+                                                               int64_t stride, int p, int D, int W,
+                                                               int32_t *__restrict__ N) {
+    disc_gram_body(data, n, stride, p, D, W, N, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
 // One thread per pair (vj < vi): the table N[a][b] (a a value of vi, b of vj)
 // read from the Gram matrix, integer marginals, chi^2 summed over the cells in
 // (a, b) order with the per-cell term of chi2_term, Perlman's pchisq, then pmax
@@ -332,14 +339,16 @@ __global__ __launch_bounds__(256) void disc_onehot_gram_kernel(const uint8_t *__
 // marginal arrays.  Each cell is zeroed after its last read: the cells never
 // read are those of categories without observations (never added to) and the
 // diagonal blocks (never read), so the next level-0 run starts from zero.
+// Pair t of the block of pair blocks `blk` (block 0 also clears G's diagonal
+// and writes the level's statistics).
 template <int MD>
-__global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__restrict__ dims, int p, int D, int W,
-                                                               int32_t *__restrict__ N, double alpha,
-                                                               uint8_t *__restrict__ Gm, float *__restrict__ pmax,
-                                                               unsigned long long *__restrict__ st) {
+__device__ __forceinline__ void disc_level0_chi2_body(const uint8_t *__restrict__ dims, int p, int D, int W,
+                                                      int32_t *__restrict__ N, double alpha,
+                                                      uint8_t *__restrict__ Gm, float *__restrict__ pmax,
+                                                      unsigned long long *__restrict__ st, int blk) {
 #pragma clang fp contract(off)
     const long long pairs = (long long)p * (p - 1) / 2;
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         for (int v = threadIdx.x; v < p; v += blockDim.x) Gm[(size_t)v * p + v] = 0;
         if (threadIdx.x == 0) {
             st[DS_EXEC] = pairs;
@@ -347,7 +356,7 @@ __global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__
             st[DS_USEFUL] = pairs;
         }
     }
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blk * blockDim.x + threadIdx.x;
     if (t >= pairs) return;
     int vi = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
     while ((long long)vi * (vi - 1) / 2 > t) --vi;
@@ -393,13 +402,20 @@ __global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__
     Gm[(size_t)vj * p + vi] = keep;
 }
 
+template <int MD>
+__global__ __launch_bounds__(256) void disc_level0_chi2_kernel(const uint8_t *__restrict__ dims, int p, int D, int W,
+                                                               int32_t *__restrict__ N, double alpha,
+                                                               uint8_t *__restrict__ Gm, float *__restrict__ pmax,
+                                                               unsigned long long *__restrict__ st) {
+    disc_level0_chi2_body<MD>(dims, p, D, W, N, alpha, Gm, pmax, st, blockIdx.x);
+}
+
 // Buffer set-up of a discrete skeleton in one launch: G = 1, sepsets = -1,
 // pmax = 0, removal keys = none, the level-0 Gram matrix and the statistics 0.
-__global__ void disc_init_kernel(uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, float *__restrict__ pmax,
-                                 unsigned long long *__restrict__ keys, int32_t *__restrict__ gram,
-                                 unsigned long long *__restrict__ st, size_t pp, size_t ns, size_t ng, int nst) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void disc_init_body(uint8_t *__restrict__ Gm, int32_t *__restrict__ sep,
+                                               float *__restrict__ pmax, unsigned long long *__restrict__ keys,
+                                               int32_t *__restrict__ gram, unsigned long long *__restrict__ st,
+                                               size_t pp, size_t ns, size_t ng, int nst, size_t t0, size_t stride) {
     for (size_t t = t0; t < pp; t += stride) {
         Gm[t] = 1;
         pmax[t] = 0.0f;
@@ -409,16 +425,24 @@ __global__ void disc_init_kernel(uint8_t *__restrict__ Gm, int32_t *__restrict__
         for (size_t t = t0; t < ns; t += stride) sep[t] = -1;
     if (gram)
         for (size_t t = t0; t < ng; t += stride) gram[t] = 0;
-    if (st && t0 < (size_t)nst) st[t0] = 0;
+    for (size_t t = t0; st && t < (size_t)nst; t += stride) st[t] = 0;
+}
+
+__global__ void disc_init_kernel(uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, float *__restrict__ pmax,
+                                 unsigned long long *__restrict__ keys, int32_t *__restrict__ gram,
+                                 unsigned long long *__restrict__ st, size_t pp, size_t ns, size_t ng, int nst) {
+    disc_init_body(Gm, sep, pmax, keys, gram, st, pp, ns, ng, nst, (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+                   (size_t)gridDim.x * blockDim.x);
 }
 
 // ----------------------------------------------------------------------------
 // level l >= 1: gpucsl/pc/cuda/discrete_ci.cu:113-246
 // ----------------------------------------------------------------------------
 // testoff[r] = sum over rows < r of deg * C(deg - 1, L); also T_l, max degree.
-__global__ __launch_bounds__(1024) void disc_testoff_kernel(const int32_t *__restrict__ deg, int p, int L,
-                                                            unsigned long long *__restrict__ testoff,
-                                                            unsigned long long *__restrict__ st) {
+// By one block of 1024 threads.
+__device__ __forceinline__ void disc_testoff_body(const int32_t *__restrict__ deg, int p, int L,
+                                                  unsigned long long *__restrict__ testoff,
+                                                  unsigned long long *__restrict__ st) {
     __shared__ unsigned long long sT[1024];
     __shared__ int sM[1024];
     const int t = threadIdx.x;
@@ -460,15 +484,24 @@ __global__ __launch_bounds__(1024) void disc_testoff_kernel(const int32_t *__res
     if (t == 0) st[DS_MAXDEG] = (unsigned long long)sM[0];
 }
 
+__global__ __launch_bounds__(1024) void disc_testoff_kernel(const int32_t *__restrict__ deg, int p, int L,
+                                                            unsigned long long *__restrict__ testoff,
+                                                            unsigned long long *__restrict__ st) {
+    disc_testoff_body(deg, p, L, testoff, st);
+}
+
+// Test t (< testoff[p]) of a level: the row i whose tests hold t, the
+// neighbour j and the sepset of rank `rank` among the C(deg - 1, L) subsets of
+// i's other neighbours; the table in LDS (gslice: this workgroup's global
+// table when it does not fit).  All threads of the block call it.
 template <int L>
-__global__ __launch_bounds__(DT_THREADS) void disc_level_kernel(
-    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int p,
-    double alpha, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
-    const unsigned long long *__restrict__ testoff, float *__restrict__ pmax,
-    unsigned long long *__restrict__ keys, uint32_t *__restrict__ gtab, size_t gtab_words) {
-    extern __shared__ uint32_t lds[];
-    const unsigned long long T = testoff[p];
-    for (unsigned long long t = blockIdx.x; t < T; t += gridDim.x) {
+__device__ __forceinline__ void disc_test(const uint8_t *__restrict__ data, int64_t n, int64_t stride,
+                                          const uint8_t *__restrict__ dims, int p, double alpha,
+                                          const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+                                          const unsigned long long *__restrict__ testoff, float *__restrict__ pmax,
+                                          unsigned long long *__restrict__ keys, uint32_t *gslice,
+                                          unsigned long long t, uint32_t *lds) {
+    {
         int lo = 0, hi = p;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
@@ -506,7 +539,7 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level_kernel(
             tab = lds;
             replicas = 1;
         } else {
-            tab = gtab + (size_t)blockIdx.x * gtab_words;
+            tab = gslice;
             replicas = 1;
         }
         count_table<L>(data, n, stride, i, j, S, di, dj, radix, tab, cells, replicas);
@@ -528,14 +561,26 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level_kernel(
     }
 }
 
+template <int L>
+__global__ __launch_bounds__(DT_THREADS) void disc_level_kernel(
+    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int p,
+    double alpha, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+    const unsigned long long *__restrict__ testoff, float *__restrict__ pmax,
+    unsigned long long *__restrict__ keys, uint32_t *__restrict__ gtab, size_t gtab_words) {
+    extern __shared__ uint32_t lds[];
+    const unsigned long long T = testoff[p];
+    for (unsigned long long t = blockIdx.x; t < T; t += gridDim.x)
+        disc_test<L>(data, n, stride, dims, p, alpha, nbr, deg, testoff, pmax, keys,
+                     gtab + (size_t)blockIdx.x * gtab_words, t, lds);
+}
+
 // Removes the edges whose key was set, writes their sepsets in both
 // directions and resets the keys for the next level.
 template <int L>
-__global__ void disc_writeback_kernel(int p, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
-                                      unsigned long long *__restrict__ keys, uint8_t *__restrict__ Gm,
-                                      int32_t *__restrict__ sep, int maxL) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
+__device__ __forceinline__ void disc_writeback_pair(int p, const int32_t *__restrict__ nbr,
+                                                    const int32_t *__restrict__ deg,
+                                                    unsigned long long *__restrict__ keys, uint8_t *__restrict__ Gm,
+                                                    int32_t *__restrict__ sep, int maxL, int a, int b) {
     if (b >= p || b <= a) return;
     const unsigned long long key = keys[(size_t)a * p + b];
     if (key == KEY_NONE) return;
@@ -561,6 +606,13 @@ __global__ void disc_writeback_kernel(int p, const int32_t *__restrict__ nbr, co
     }
     Gm[(size_t)a * p + b] = 0;
     Gm[(size_t)b * p + a] = 0;
+}
+
+template <int L>
+__global__ void disc_writeback_kernel(int p, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+                                      unsigned long long *__restrict__ keys, uint8_t *__restrict__ Gm,
+                                      int32_t *__restrict__ sep, int maxL) {
+    disc_writeback_pair<L>(p, nbr, deg, keys, Gm, sep, maxL, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -614,16 +666,16 @@ __device__ void count_table_rt(const uint8_t *__restrict__ data, int64_t n, int6
     }
 }
 
-__global__ __launch_bounds__(DT_THREADS) void disc_level_rt_kernel(
-    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int p, int L,
-    double alpha, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
-    const unsigned long long *__restrict__ testoff, float *__restrict__ pmax,
-    unsigned long long *__restrict__ keys, uint32_t *__restrict__ gtab, size_t gtab_words) {
-    extern __shared__ uint32_t lds[];
-    __shared__ int sS[GPUCSL_MAX_LEVEL], sRadix[GPUCSL_MAX_LEVEL];
-    __shared__ int sDimS;
-    const unsigned long long T = testoff[p];
-    for (unsigned long long t = blockIdx.x; t < T; t += gridDim.x) {
+// disc_test with the level taken at run time; sS, sRadix (GPUCSL_MAX_LEVEL
+// ints each) and sDimS are LDS scratch of the block.
+__device__ __forceinline__ void disc_test_rt(const uint8_t *__restrict__ data, int64_t n, int64_t stride,
+                                             const uint8_t *__restrict__ dims, int p, int L, double alpha,
+                                             const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+                                             const unsigned long long *__restrict__ testoff,
+                                             float *__restrict__ pmax, unsigned long long *__restrict__ keys,
+                                             uint32_t *gslice, unsigned long long t, uint32_t *lds, int *sS,
+                                             int *sRadix, int *sDimS) {
+    {
         int lo = 0, hi = p;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
@@ -647,10 +699,10 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level_rt_kernel(
                 sRadix[c] = dim_s;
                 dim_s *= dims[sS[c]];
             }
-            sDimS = dim_s;
+            *sDimS = dim_s;
         }
         __syncthreads();
-        const int dim_s = sDimS;
+        const int dim_s = *sDimS;
         const int di = dims[i], dj = dims[j];
         const int cells = dim_s * di * dj;
         const int marg = dim_s * (di + dj + 1);
@@ -663,7 +715,7 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level_rt_kernel(
             tab = lds;
             replicas = 1;
         } else {
-            tab = gtab + (size_t)blockIdx.x * gtab_words;
+            tab = gslice;
             replicas = 1;
         }
         count_table_rt(data, n, stride, i, j, sS, sRadix, L, di, dj, tab, cells, replicas);
@@ -684,11 +736,25 @@ __global__ __launch_bounds__(DT_THREADS) void disc_level_rt_kernel(
     }
 }
 
-__global__ void disc_writeback_rt_kernel(int p, int L, const int32_t *__restrict__ nbr,
-                                         const int32_t *__restrict__ deg, unsigned long long *__restrict__ keys,
-                                         uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, int maxL) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
+__global__ __launch_bounds__(DT_THREADS) void disc_level_rt_kernel(
+    const uint8_t *__restrict__ data, int64_t n, int64_t stride, const uint8_t *__restrict__ dims, int p, int L,
+    double alpha, const int32_t *__restrict__ nbr, const int32_t *__restrict__ deg,
+    const unsigned long long *__restrict__ testoff, float *__restrict__ pmax,
+    unsigned long long *__restrict__ keys, uint32_t *__restrict__ gtab, size_t gtab_words) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int sS[GPUCSL_MAX_LEVEL], sRadix[GPUCSL_MAX_LEVEL];
+    __shared__ int sDimS;
+    const unsigned long long T = testoff[p];
+    for (unsigned long long t = blockIdx.x; t < T; t += gridDim.x)
+        disc_test_rt(data, n, stride, dims, p, L, alpha, nbr, deg, testoff, pmax, keys,
+                     gtab + (size_t)blockIdx.x * gtab_words, t, lds, sS, sRadix, &sDimS);
+}
+
+__device__ __forceinline__ void disc_writeback_pair_rt(int p, int L, const int32_t *__restrict__ nbr,
+                                                       const int32_t *__restrict__ deg,
+                                                       unsigned long long *__restrict__ keys,
+                                                       uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, int maxL,
+                                                       int a, int b) {
     if (b >= p || b <= a) return;
     const unsigned long long key = keys[(size_t)a * p + b];
     if (key == KEY_NONE) return;
@@ -714,6 +780,12 @@ __global__ void disc_writeback_rt_kernel(int p, int L, const int32_t *__restrict
     }
     Gm[(size_t)a * p + b] = 0;
     Gm[(size_t)b * p + a] = 0;
+}
+
+__global__ void disc_writeback_rt_kernel(int p, int L, const int32_t *__restrict__ nbr,
+                                         const int32_t *__restrict__ deg, unsigned long long *__restrict__ keys,
+                                         uint8_t *__restrict__ Gm, int32_t *__restrict__ sep, int maxL) {
+    disc_writeback_pair_rt(p, L, nbr, deg, keys, Gm, sep, maxL, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Single test for known-answer checks: table (cells words), chi^2 and p.
@@ -947,6 +1019,303 @@ int disc_level_launch(int level, const uint8_t *data, int64_t n, int64_t stride,
         break;
     }
     return check_hip(hipGetLastError(), "disc_level launch");
+}
+
+// ----------------------------------------------------------------------------
+// batched jobs: the jobs of a launch travel in its arguments (a captured
+// launch keeps them); per job the same kernels' arithmetic as the
+// single-dataset path, on the job's own buffers and workspace, so the results
+// are bit-identical.
+// ----------------------------------------------------------------------------
+struct DiscEntry {
+    const uint8_t *data, *dims;
+    int64_t n, stride;
+    int p, max_level, max_dim, gram_w;
+    int ks, grid;  // level-0 Gram shares; workgroups the job's table slices allow
+    double alpha;
+    uint8_t *G;
+    int32_t *sep;
+    float *pmax;
+    unsigned long long *stats;
+    uint16_t *skel;
+    int32_t *nbr, *deg;
+    unsigned long long *testoff, *keys;
+    int32_t *gram;
+    uint32_t *gtab;
+    size_t gtab_words;
+};
+struct DiscLaunch {
+    int count;
+    DiscEntry e[GPUCSL_DISC_BATCH_JOBS];
+};
+
+__global__ void disc_batch_init_kernel(DiscLaunch B) {
+    const DiscEntry &e = B.e[blockIdx.y];
+    const size_t pp = (size_t)e.p * e.p, ns = e.max_level > 0 && e.sep ? pp * e.max_level : 0;
+    const size_t ng = (size_t)e.gram_w * e.gram_w;
+    disc_init_body(e.G, e.max_level > 0 ? e.sep : nullptr, e.pmax, e.keys, ng ? e.gram : nullptr, e.stats, pp, ns,
+                   ng, (e.max_level + 1) * GPUCSL_STAT_SLOTS, (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+                   (size_t)gridDim.x * blockDim.x);
+}
+
+__global__ __launch_bounds__(256) void disc_batch_gram_kernel(DiscLaunch B) {
+    const DiscEntry &e = B.e[blockIdx.z];
+    const int tiles = e.gram_w / GRAM_TILE;
+    if (e.gram_w == 0 || (int)blockIdx.x >= tiles * (tiles + 1) / 2 || (int)blockIdx.y >= e.ks) return;
+    disc_gram_body(e.data, e.n, e.stride, e.p, e.max_dim, e.gram_w, e.gram, blockIdx.x, blockIdx.y, e.ks);
+}
+
+template <int MD>
+__global__ __launch_bounds__(256) void disc_batch_level0_chi2_kernel(DiscLaunch B) {
+    const DiscEntry &e = B.e[blockIdx.z];
+    const long long pairs = (long long)e.p * (e.p - 1) / 2;
+    if (e.gram_w == 0 || (long long)blockIdx.x * 256 >= (pairs > 0 ? pairs : 1)) return;
+    disc_level0_chi2_body<MD>(e.dims, e.p, e.max_dim, e.gram_w, e.gram, e.alpha, e.G, e.pmax, e.stats, blockIdx.x);
+}
+
+// Level-start neighbour lists of every job at `level`: one wave per row,
+// ascending neighbours (diagonal excluded) and the degree, as the
+// single-dataset compaction leaves them.
+__global__ __launch_bounds__(256) void disc_batch_compact_kernel(DiscLaunch B, int level) {
+    const DiscEntry &e = B.e[blockIdx.y];
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (level > e.max_level || row >= e.p) return;
+    const int p = e.p, lane = __lane_id();
+    const uint8_t *g = e.G + (size_t)row * p;
+    int32_t *out = e.nbr + (size_t)row * p;
+    int base = 0;
+    for (int c0 = 0; c0 < p; c0 += 64) {
+        const int c = c0 + lane;
+        const bool on = c < p && c != row && g[c] != 0;
+        const uint64_t m = __ballot(on);
+        if (on) out[base + __popcll(m & lanemask_lt())] = c;
+        base += __popcll(m);
+    }
+    if (lane == 0) e.deg[row] = base;
+}
+
+__global__ __launch_bounds__(1024) void disc_batch_testoff_kernel(DiscLaunch B, int level) {
+    const DiscEntry &e = B.e[blockIdx.x];
+    if (level > e.max_level) return;
+    disc_testoff_body(e.deg, e.p, level, e.testoff, e.stats + (size_t)level * GPUCSL_STAT_SLOTS);
+}
+
+// The tests of every job at this level, numbered one after the other in job
+// order; workgroup w runs the tests whose number is w modulo the grid.
+template <int L>
+__global__ __launch_bounds__(DT_THREADS) void disc_batch_level_kernel(DiscLaunch B) {
+    extern __shared__ uint32_t lds[];
+    const unsigned long long grid = gridDim.x;
+    unsigned long long base = 0;
+    for (int k = 0; k < B.count; ++k) {
+        const DiscEntry &e = B.e[k];
+        if (L > e.max_level) continue;
+        const unsigned long long T = e.testoff[e.p];
+        for (unsigned long long t = (blockIdx.x + grid - base % grid) % grid; t < T; t += grid)
+            disc_test<L>(e.data, e.n, e.stride, e.dims, e.p, e.alpha, e.nbr, e.deg, e.testoff, e.pmax, e.keys,
+                         e.gtab + (size_t)blockIdx.x * e.gtab_words, t, lds);
+        base += T;
+    }
+}
+
+__global__ __launch_bounds__(DT_THREADS) void disc_batch_level_rt_kernel(DiscLaunch B, int L) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int sS[GPUCSL_MAX_LEVEL], sRadix[GPUCSL_MAX_LEVEL];
+    __shared__ int sDimS;
+    const unsigned long long grid = gridDim.x;
+    unsigned long long base = 0;
+    for (int k = 0; k < B.count; ++k) {
+        const DiscEntry &e = B.e[k];
+        if (L > e.max_level) continue;
+        const unsigned long long T = e.testoff[e.p];
+        for (unsigned long long t = (blockIdx.x + grid - base % grid) % grid; t < T; t += grid)
+            disc_test_rt(e.data, e.n, e.stride, e.dims, e.p, L, e.alpha, e.nbr, e.deg, e.testoff, e.pmax, e.keys,
+                         e.gtab + (size_t)blockIdx.x * e.gtab_words, t, lds, sS, sRadix, &sDimS);
+        base += T;
+    }
+}
+
+template <int L>
+__global__ void disc_batch_writeback_kernel(DiscLaunch B) {
+    const DiscEntry &e = B.e[blockIdx.z];
+    if (L > e.max_level || (int)blockIdx.y >= e.p) return;
+    disc_writeback_pair<L>(e.p, e.nbr, e.deg, e.keys, e.G, e.sep, e.max_level, blockIdx.y,
+                           blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void disc_batch_writeback_rt_kernel(DiscLaunch B, int L) {
+    const DiscEntry &e = B.e[blockIdx.z];
+    if (L > e.max_level || (int)blockIdx.y >= e.p) return;
+    disc_writeback_pair_rt(e.p, L, e.nbr, e.deg, e.keys, e.G, e.sep, e.max_level, blockIdx.y,
+                           blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// pmax = -1 on and below the diagonal, and the uint16 skeleton when asked for.
+__global__ void disc_batch_finish_kernel(DiscLaunch B) {
+    const DiscEntry &e = B.e[blockIdx.z];
+    const int a = blockIdx.y, b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= e.p || b >= e.p) return;
+    const size_t at = (size_t)a * e.p + b;
+    if (b <= a) e.pmax[at] = -1.0f;
+    if (e.skel) e.skel[at] = e.G[at];
+}
+
+// Entry of one job, checked as the single-dataset calls check their arguments.
+static int disc_entry(const gpucsl_disc_job &J, int idx, DiscEntry &e) {
+    if (!J.data || !J.dims || !J.G || !J.pmax || !J.stats || !J.ws || J.p < 1 || J.n < 1 || J.max_level < 0 ||
+        (J.max_level > 0 && !J.sepsets))
+        return set_error(ERR_ARG, "disc_batch: job %d has bad arguments", idx);
+    if (J.stride < (J.n + 15) / 16 * 16 || J.stride % 16 != 0)
+        return set_error(ERR_ARG, "disc_batch: job %d: stride must be a multiple of 16 and >= n rounded up to 16",
+                         idx);
+    if (((uintptr_t)J.data & 15) != 0) return set_error(ERR_ARG, "disc_batch: job %d: data must be 16-byte aligned", idx);
+    const int md = J.max_dim < 1 ? 1 : J.max_dim;
+    if (J.ws_bytes < disc_min_workspace_bytes(J.p, J.max_level, md))
+        return set_error(ERR_WORKSPACE, "disc_batch: job %d: workspace too small", idx);
+    DiscWs w = disc_carve(J.ws, J.p, J.max_level, md, J.ws_bytes);
+    e = DiscEntry{};
+    e.data = J.data;
+    e.dims = J.dims;
+    e.n = J.n;
+    e.stride = J.stride;
+    e.p = J.p;
+    e.max_level = J.max_level;
+    e.max_dim = md;
+    e.gram_w = w.gram_w;
+    e.grid = w.grid < 1 ? 1 : w.grid;
+    e.alpha = J.alpha;
+    e.G = J.G;
+    e.sep = J.sepsets;
+    e.pmax = J.pmax;
+    e.stats = reinterpret_cast<unsigned long long *>(J.stats);
+    e.skel = J.skeleton;
+    level_list_views(J.ws, J.p, &e.nbr, &e.deg);
+    e.testoff = w.testoff;
+    e.keys = w.keys;
+    e.gram = w.gram;
+    e.gtab = w.gtab;
+    e.gtab_words = w.gtab_words;
+    if (w.gram_w) {
+        const int tiles = w.gram_w / GRAM_TILE, ut = tiles * (tiles + 1) / 2;
+        const int64_t steps = (J.n + 31) / 32;
+        int64_t ks = (4 * (int64_t)device_cus() + ut - 1) / ut;
+        const int64_t kmax = (steps + 63) / 64;
+        if (ks > kmax) ks = kmax;
+        e.ks = ks < 1 ? 1 : (int)ks;
+    }
+    return OK;
+}
+
+// Runs f(launch, first job index) for the jobs in launches of at most
+// GPUCSL_DISC_BATCH_JOBS.
+template <class F>
+static int disc_batch_chunks(const gpucsl_disc_job *jobs, int nj, F f) {
+    if (nj < 0 || (nj > 0 && !jobs)) return set_error(ERR_ARG, "disc_batch: bad job table");
+    for (int c0 = 0; c0 < nj; c0 += GPUCSL_DISC_BATCH_JOBS) {
+        DiscLaunch B{};
+        B.count = nj - c0 < GPUCSL_DISC_BATCH_JOBS ? nj - c0 : GPUCSL_DISC_BATCH_JOBS;
+        for (int k = 0; k < B.count; ++k) {
+            const int rc = disc_entry(jobs[c0 + k], c0 + k, B.e[k]);
+            if (rc) return rc;
+        }
+        const int rc = f(B, c0);
+        if (rc) return rc;
+    }
+    return OK;
+}
+
+int disc_batch_init_launch(const gpucsl_disc_job *jobs, int nj, hipStream_t s) {
+    return disc_batch_chunks(jobs, nj, [&](const DiscLaunch &B, int) -> int {
+        size_t most = 1;
+        for (int k = 0; k < B.count; ++k) {
+            const DiscEntry &e = B.e[k];
+            const size_t pp = (size_t)e.p * e.p, ns = e.max_level > 0 ? pp * e.max_level : 0;
+            const size_t ng = (size_t)e.gram_w * e.gram_w, nst = (size_t)(e.max_level + 1) * GPUCSL_STAT_SLOTS;
+            for (size_t v : {pp, ns, ng, nst}) most = v > most ? v : most;
+        }
+        size_t blocks = (most + 255) / 256;
+        if (blocks > 4096) blocks = 4096;
+        GPUCSL_LAUNCH("disc_init", disc_batch_init_kernel, dim3((unsigned)blocks, B.count), dim3(256), 0, s, B);
+        return check_hip(hipGetLastError(), "disc_batch_init launch");
+    });
+}
+
+int disc_batch_level0_launch(const gpucsl_disc_job *jobs, int nj, hipStream_t s) {
+    return disc_batch_chunks(jobs, nj, [&](const DiscLaunch &B, int c0) -> int {
+        int max_ut = 0, max_ks = 0, max_md = 1;
+        long long max_pairs = 1;
+        for (int k = 0; k < B.count; ++k) {
+            const DiscEntry &e = B.e[k];
+            if (!e.gram_w) {
+                // tables counted pair by pair, as the single-dataset path does
+                const gpucsl_disc_job &J = jobs[c0 + k];
+                const int rc = disc_level0_launch(J.data, J.n, J.stride, J.dims, J.p, J.max_level, e.max_dim,
+                                                  J.alpha, J.G, J.pmax, e.stats, J.ws, J.ws_bytes, s);
+                if (rc) return rc;
+                continue;
+            }
+            const int tiles = e.gram_w / GRAM_TILE, ut = tiles * (tiles + 1) / 2;
+            max_ut = ut > max_ut ? ut : max_ut;
+            max_ks = e.ks > max_ks ? e.ks : max_ks;
+            max_md = e.max_dim > max_md ? e.max_dim : max_md;
+            const long long pairs = (long long)e.p * (e.p - 1) / 2;
+            max_pairs = pairs > max_pairs ? pairs : max_pairs;
+        }
+        if (max_ut == 0) return OK;
+        GPUCSL_LAUNCH("disc_level0_gram", disc_batch_gram_kernel, dim3(max_ut, max_ks, B.count), dim3(256), 0, s, B);
+        const dim3 grid((unsigned)((max_pairs + 255) / 256), 1, B.count);
+        if (max_md <= 4)
+            GPUCSL_LAUNCH("disc_level0", disc_batch_level0_chi2_kernel<4>, grid, dim3(256), 0, s, B);
+        else if (max_md <= 8)
+            GPUCSL_LAUNCH("disc_level0", disc_batch_level0_chi2_kernel<8>, grid, dim3(256), 0, s, B);
+        else
+            GPUCSL_LAUNCH("disc_level0", disc_batch_level0_chi2_kernel<16>, grid, dim3(256), 0, s, B);
+        return check_hip(hipGetLastError(), "disc_batch_level0 launch");
+    });
+}
+
+int disc_batch_level_launch(int level, const gpucsl_disc_job *jobs, int nj, hipStream_t s) {
+    if (level < 1) return set_error(ERR_ARG, "disc_batch_level: level %d < 1", level);
+    return disc_batch_chunks(jobs, nj, [&](const DiscLaunch &B, int) -> int {
+        int maxp = 0, grid = disc_grid();
+        for (int k = 0; k < B.count; ++k) {
+            const DiscEntry &e = B.e[k];
+            if (level > e.max_level) continue;
+            maxp = e.p > maxp ? e.p : maxp;
+            grid = e.grid < grid ? e.grid : grid;
+        }
+        if (maxp == 0) return OK;  // no job of this launch runs the level
+        if (level > GPUCSL_MAX_LEVEL)
+            return set_error(ERR_UNSUPPORTED, "disc_batch_level: levels above %d are not built", GPUCSL_MAX_LEVEL);
+        GPUCSL_LAUNCH("compact", disc_batch_compact_kernel, dim3((maxp + 3) / 4, B.count), dim3(256), 0, s, B, level);
+        GPUCSL_LAUNCH("disc_testoff", disc_batch_testoff_kernel, dim3(B.count), dim3(1024), 0, s, B, level);
+        const dim3 wb((maxp + 255) / 256, maxp, B.count);
+        switch (level) {
+#define GPUCSL_DB(LV)                                                                                             \
+    case LV:                                                                                                      \
+        GPUCSL_LAUNCH(kDiscName[LV], disc_batch_level_kernel<LV>, dim3(grid), dim3(DT_THREADS), DT_LDS_BYTES, s, B); \
+        GPUCSL_LAUNCH("disc_writeback", disc_batch_writeback_kernel<LV>, wb, dim3(256), 0, s, B);                 \
+        break;
+            GPUCSL_DB(1) GPUCSL_DB(2) GPUCSL_DB(3) GPUCSL_DB(4) GPUCSL_DB(5) GPUCSL_DB(6) GPUCSL_DB(7) GPUCSL_DB(8)
+#undef GPUCSL_DB
+        default:
+            GPUCSL_LAUNCH("disc_level_rt", disc_batch_level_rt_kernel, dim3(grid), dim3(DT_THREADS), DT_LDS_BYTES, s,
+                          B, level);
+            GPUCSL_LAUNCH("disc_writeback", disc_batch_writeback_rt_kernel, wb, dim3(256), 0, s, B, level);
+            break;
+        }
+        return check_hip(hipGetLastError(), "disc_batch_level launch");
+    });
+}
+
+int disc_batch_finish_launch(const gpucsl_disc_job *jobs, int nj, hipStream_t s) {
+    return disc_batch_chunks(jobs, nj, [&](const DiscLaunch &B, int) -> int {
+        int maxp = 1;
+        for (int k = 0; k < B.count; ++k) maxp = B.e[k].p > maxp ? B.e[k].p : maxp;
+        GPUCSL_LAUNCH("disc_finish", disc_batch_finish_kernel, dim3((maxp + 255) / 256, maxp, B.count), dim3(256), 0,
+                      s, B);
+        return check_hip(hipGetLastError(), "disc_batch_finish launch");
+    });
 }
 
 // ----------------------------------------------------------------------------

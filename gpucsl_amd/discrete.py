@@ -7,6 +7,7 @@ the device part of gpucsl/pc/discover_skeleton_discrete.py:24-128.
 """
 from __future__ import annotations
 
+import ctypes as ct
 from dataclasses import dataclass
 
 import numpy as np
@@ -15,6 +16,17 @@ from gpucsl_amd import _native as nat
 from gpucsl_amd.cli.cli_util import GpucslError
 
 COLUMN_ALIGN = 64   # bytes; a multiple of 16 as the kernels require
+BATCH_JOBS = 16     # GPUCSL_DISC_BATCH_JOBS: jobs per batched launch
+
+
+class DiscJob(ct.Structure):
+    """ctypes mirror of struct gpucsl_disc_job (include/gpucsl_hip.h)."""
+    _fields_ = [
+        ("data", ct.c_void_p), ("dims", ct.c_void_p), ("n", ct.c_int64), ("stride", ct.c_int64),
+        ("p", ct.c_int32), ("max_level", ct.c_int32), ("max_dim", ct.c_int32), ("reserved", ct.c_int32),
+        ("alpha", ct.c_double), ("G", ct.c_void_p), ("sepsets", ct.c_void_p), ("pmax", ct.c_void_p),
+        ("stats", ct.c_void_p), ("skeleton", ct.c_void_p), ("ws", ct.c_void_p), ("ws_bytes", ct.c_size_t),
+    ]
 
 
 def ordinal_encode(data: np.ndarray):
@@ -213,6 +225,12 @@ class DiscreteEngine:
     def level_stats(self):
         return nat.decode_stats(self.stats.cpu().numpy().view(np.uint64), self.max_level)
 
+    def descriptor(self) -> DiscJob:
+        """This job's entry in a batched launch (its own buffers and workspace)."""
+        return DiscJob(self.data.data_ptr(), self.dims.data_ptr(), self.n, self.stride, self.p, self.max_level,
+                       self.max_dim, 0, self.alpha, self.G.data_ptr(), self.sep.data_ptr(), self.pmax.data_ptr(),
+                       self.stats.data_ptr(), self.skel16.data_ptr(), self.ws.data_ptr(), self.ws_bytes)
+
     def results(self) -> DiscreteResult:
         skel, sep, pmax, st = nat.to_host(self.skel16, self.sep, self.pmax, self.stats)
         return DiscreteResult(skel.view(np.uint16), sep[:, :, : self.max_level], pmax,
@@ -234,3 +252,69 @@ class DiscreteEngine:
         if o[2] < 0:
             raise GpucslError(f"ci_test: the table of {cells} cells and its marginals exceed one workgroup's LDS")
         return table.cpu().numpy().view(np.uint32)[:cells].copy(), float(o[0]), float(o[1]), int(o[2])
+
+
+class DiscreteBatch:
+    """Discrete jobs (DiscreteEngine, each with its own buffers) advancing
+    together through the batched entry points: one launch per kernel of a
+    level for up to BATCH_JOBS jobs, where the single-dataset path makes one per
+    job.  No counterpart in gpucsl, which runs
+    gpucsl/pc/discover_skeleton_discrete.py:24-128 once per dataset; each job's
+    results are bit-identical to its single-dataset run.  The whole skeleton
+    can be captured into one HIP graph (``capture``) and replayed."""
+
+    def __init__(self, engines: list):
+        torch = nat.require_gpu()
+        if not engines:
+            raise GpucslError("empty batch")
+        self.torch = torch
+        self.engines = list(engines)
+        self.max_level = max(e.max_level for e in self.engines)
+        self.table = (DiscJob * len(self.engines))(*[e.descriptor() for e in self.engines])
+        self._graph = None
+
+    def _args(self):
+        return ct.byref(self.table), len(self.engines)
+
+    # ---- level by level
+    def start(self) -> None:
+        """Buffer set-up and level 0 of every job."""
+        s = nat.stream_handle()
+        nat.call("gpucsl_disc_batch_init", *self._args(), s)
+        nat.call("gpucsl_disc_batch_level0", *self._args(), s)
+
+    def level(self, l: int) -> None:
+        """Level l of every job whose max_level reaches it."""
+        nat.call("gpucsl_disc_batch_level", l, *self._args(), nat.stream_handle())
+
+    def finish(self) -> None:
+        """pmax lower triangle and the uint16 skeletons."""
+        nat.call("gpucsl_disc_batch_finish", *self._args(), nat.stream_handle())
+
+    def launch(self) -> None:
+        """Set-up, levels 0..max_level and the finish, no host sync."""
+        nat.call("gpucsl_disc_batch_skeleton", *self._args(), nat.stream_handle())
+
+    def capture(self) -> None:
+        """Capture ``launch`` into one HIP graph."""
+        torch = self.torch
+        self.launch()                     # warm the code objects outside capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            g.capture_begin()
+            self.launch()
+            g.capture_end()
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph = g
+
+    def run(self) -> None:
+        if self._graph is not None:
+            self._graph.replay()
+        else:
+            self.launch()
+
+    def results(self) -> list:
+        return [e.results() for e in self.engines]

@@ -10,14 +10,17 @@ scheduling round ("tick"):
   kernel (stage 0 = correlation + level 0, stage l = level l, last stage =
   pmax/skeleton), so a job that arrives while others are at level 2 starts its
   level 0 in the same round as their level 3;
-- discrete jobs run level by level on their own streams (round robin), so
-  they overlap with the Gaussian batches and with each other.
+- discrete jobs are spread over their own streams (round robin at
+  admission) and advance the same way: the jobs of one stream at the same
+  stage share one batched launch per kernel (gpucsl_disc_batch_*), and the
+  streams overlap with the Gaussian batches and with each other.
 
 Results are copied to pinned host memory and handed back when their copy has
 completed; latency is measured from ``submit`` to that point.  A job's result
 does not depend on the other jobs in flight: the Gaussian kernels give the
 same bits in any batch (the correlation split-K is a function of the job's own
-n and p) and discrete jobs run alone on their stream.
+n and p), and so do the discrete ones (exact integer tables, the same per-test
+arithmetic, removal keys and pmax by atomic min and max).
 
 This replaces calling gpucsl's discover_skeleton_gpu_gaussian / _discrete
 (gpucsl/pc/discover_skeleton_gaussian.py:116-172,
@@ -35,7 +38,7 @@ import numpy as np
 from gpucsl_amd import _native as nat
 from gpucsl_amd.batch import GaussJob, GaussJobState, job_footprint
 from gpucsl_amd.cli.cli_util import GpucslError
-from gpucsl_amd.discrete import DiscreteEngine, DiscreteResult, disc_layout
+from gpucsl_amd.discrete import DiscJob, DiscreteEngine, DiscreteResult, disc_layout
 from gpucsl_amd.gaussian import DEFAULT_GROUP_SIZE, DEFAULT_MARGIN, GaussianResult
 
 MAX_SERVED_LEVEL = 32    # GPUCSL_MAX_LEVEL: the deepest level any kernel runs
@@ -362,14 +365,18 @@ class SkeletonServer:
         ready = {st for st, q in self.outstanding.items() if len(q) < self.ticks_in_flight}
         used, keep = set(), []
         groups: dict = {}
+        dgroups: dict = {}
         for j in self.active:
             if j.done_event is not None or j.stream not in ready:
                 continue
             if j.req.kind == "gaussian":
                 groups.setdefault(j.stage, []).append(j)
             else:
-                self._step_discrete(j)
-                used.add(j.stream)
+                key = (self.dstreams.index(j.stream), j.stage, j.stage == j.last)
+                dgroups.setdefault(key, []).append(j)
+        for (si, stage, finish), jobs in sorted(dgroups.items()):
+            self._launch_discrete_group(jobs, stage, finish, self.dstreams[si])
+            used.add(self.dstreams[si])
         if groups:
             used.add(self.gstream)
             with torch.cuda.stream(self.gstream):
@@ -444,16 +451,26 @@ class SkeletonServer:
         j.done_event = torch.cuda.Event()
         j.done_event.record(self.gstream)
 
-    def _step_discrete(self, j) -> None:
+    def _launch_discrete_group(self, jobs, stage, finish, stream) -> None:
+        """One stage of discrete jobs that share a stream: set-up and level 0,
+        a level, or (finish) their finish and the result copies, batched."""
         torch = self.torch
-        eng: DiscreteEngine = j.state
-        with torch.cuda.stream(j.stream):
-            if j.stage == 0:
-                eng.start()
-            elif j.stage < j.last:
-                eng.level(j.stage)
+        table = (DiscJob * len(jobs))(*[j.state.descriptor() for j in jobs])
+        self.max_batch = max(self.max_batch, len(jobs))
+        with torch.cuda.stream(stream):
+            s = nat.stream_handle()
+            if finish:
+                nat.call("gpucsl_disc_batch_finish", ct.byref(table), len(jobs), s)
+            elif stage == 0:
+                nat.call("gpucsl_disc_batch_init", ct.byref(table), len(jobs), s)
+                nat.call("gpucsl_disc_batch_level0", ct.byref(table), len(jobs), s)
             else:
-                eng.finish()
+                nat.call("gpucsl_disc_batch_level", stage, ct.byref(table), len(jobs), s)
+            for j in jobs:
+                if not finish:
+                    j.stage += 1
+                    continue
+                eng: DiscreteEngine = j.state
                 h = self._host_views(j, torch)
                 h["stats"].copy_(eng.stats, non_blocking=True)
                 if self.results_on_device:
@@ -464,9 +481,7 @@ class SkeletonServer:
                     h["pmax"].copy_(eng.pmax, non_blocking=True)
                 j.host = h
                 j.done_event = torch.cuda.Event()
-                j.done_event.record(j.stream)
-                return
-        j.stage += 1
+                j.done_event.record(stream)
 
     def _reap(self) -> list:
         """Jobs whose device work and result copy are complete give back their

@@ -331,6 +331,48 @@ int gpucsl_disc_ci_test(const uint8_t *data, int64_t n, int64_t stride, const ui
                         void *stream);
 
 /* ---------------------------------------------------------------------------
+ * batched discrete skeletons: many datasets (or resamples) of uneven n, p and
+ * categories advance level by level together, one launch per kernel of a
+ * level for up to GPUCSL_DISC_BATCH_JOBS jobs (the job table travels in the
+ * kernel arguments, so a captured launch keeps it).  No counterpart in gpucsl,
+ * which runs gpucsl/pc/discover_skeleton_discrete.py:24-128 once per dataset;
+ * each job's results are bit-identical to the single-dataset entry points on
+ * the same buffers.  A job's buffers and workspace are those of the
+ * single-dataset calls (gpucsl_disc_workspace_bytes or at least
+ * gpucsl_disc_min_workspace_bytes); the tests of one launch run in as many
+ * workgroups at once as the smallest workspace among its jobs allows.
+ * ------------------------------------------------------------------------- */
+#define GPUCSL_DISC_BATCH_JOBS 16
+
+typedef struct gpucsl_disc_job {
+    const uint8_t *data;   /* codes, column-major, padded stride (as above) */
+    const uint8_t *dims;   /* p category counts */
+    int64_t n;
+    int64_t stride;
+    int32_t p;
+    int32_t max_level;
+    int32_t max_dim;
+    int32_t reserved;
+    double alpha;
+    uint8_t *G;            /* p x p */
+    int32_t *sepsets;      /* p x p x max_level (may be NULL when max_level is 0) */
+    float *pmax;           /* p x p */
+    uint64_t *stats;       /* (max_level + 1) x GPUCSL_STAT_SLOTS */
+    uint16_t *skeleton;    /* may be NULL; filled by gpucsl_disc_batch_finish */
+    void *ws;
+    size_t ws_bytes;
+} gpucsl_disc_job;
+
+/* gpucsl_disc_init, gpucsl_disc_level0, gpucsl_disc_level and
+ * gpucsl_disc_finish for every job (a job whose max_level is below `level`
+ * is left alone), and all of them in order. */
+int gpucsl_disc_batch_init(const gpucsl_disc_job *jobs, int nj, void *stream);
+int gpucsl_disc_batch_level0(const gpucsl_disc_job *jobs, int nj, void *stream);
+int gpucsl_disc_batch_level(int level, const gpucsl_disc_job *jobs, int nj, void *stream);
+int gpucsl_disc_batch_finish(const gpucsl_disc_job *jobs, int nj, void *stream);
+int gpucsl_disc_batch_skeleton(const gpucsl_disc_job *jobs, int nj, void *stream);
+
+/* ---------------------------------------------------------------------------
  * graph helpers
  * ------------------------------------------------------------------------- */
 /* Reference layout of compact (gpucsl/pc/cuda/helpers/graph_helpers.cu:35-56):

@@ -85,6 +85,31 @@ def test_job_struct_layout_matches_header():
     assert names == [f[0] for f in GaussJob._fields_]
 
 
+def test_disc_job_struct_layout_matches_header(tmp_path):
+    """The ctypes mirror of gpucsl_disc_job has the field names of the header
+    and the size and offsets the C compiler gives it; the batch size constant
+    matches GPUCSL_DISC_BATCH_JOBS."""
+    import ctypes as ct
+    import subprocess
+
+    from gpucsl_amd.discrete import BATCH_JOBS, DiscJob
+
+    src = (ROOT / "include" / "gpucsl_hip.h").read_text()
+    assert f"#define GPUCSL_DISC_BATCH_JOBS {BATCH_JOBS}" in src
+    body = src[src.index("typedef struct gpucsl_disc_job"):src.index("} gpucsl_disc_job;")]
+    names = [ln.split(";")[0].replace("*", " ").split()[-1] for ln in body.splitlines()[1:] if ";" in ln]
+    assert names == [f[0] for f in DiscJob._fields_]
+    prog = tmp_path / "layout.c"
+    prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "gpucsl_hip.h"\nint main(void) {\n'
+                    '  printf("%zu", sizeof(gpucsl_disc_job));\n'
+                    + "".join(f'  printf(" %zu", offsetof(gpucsl_disc_job, {n}));\n' for n in names)
+                    + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(prog), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == [ct.sizeof(DiscJob)] + [getattr(DiscJob, n).offset for n in names]
+
+
 def test_level_tuning_roundtrip_and_validation():
     """Launch shapes are host state of the library, per (bucket, level):
     settable for one bucket and level, every level or every bucket,
