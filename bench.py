@@ -387,6 +387,71 @@ def measure_batched(n_boot: int, n_sets: int, steps: int, warmup: int, profile_s
                 jobs=len(jobs), hbm=hbm, plain_sec=plain_sec, same=bool(same))
 
 
+def measure_disc_batched(n_boot: int, n_sets: int, steps: int, warmup: int) -> dict:
+    """Discrete jobs in shared launches (gpucsl_disc_batch_*, one captured
+    graph per step): bootstrap resamples of one dataset (p=50, n=5000,
+    cardinalities 2..5) plus uneven datasets (p ~ U[20,120], n ~ U[1000,20000]),
+    max_level 3, against the same jobs one after another through the
+    single-dataset entry points."""
+    import torch
+
+    from gpucsl_amd.discrete import DiscreteBatch, DiscreteEngine
+    from gpucsl_amd import synthetic as syn
+
+    alpha, L = 0.01, 3
+    rng = np.random.Generator(np.random.PCG64(700))
+    base = syn.discrete_bn(50, 5000, 2.0, seed=701).data
+    datasets = [base[rng.integers(0, base.shape[0], base.shape[0])] for _ in range(n_boot)]
+    for k in range(n_sets):
+        p, n = int(rng.integers(20, 121)), int(rng.integers(1000, 20001))
+        datasets.append(syn.discrete_bn(p, n, 2.0, seed=710 + k).data)
+    log(f"[bench] discrete batched: {n_boot} resamples + {n_sets} datasets")
+
+    def engines():
+        out = []
+        for codes in datasets:
+            dims = (codes.max(axis=0).astype(np.int64) + 1).astype(np.uint8)
+            e = DiscreteEngine(codes.shape[1], codes.shape[0], L, alpha, int(dims.max()))
+            e.set_codes(codes, dims)
+            out.append(e)
+        return out
+
+    torch.cuda.set_device(0)
+    batch = DiscreteBatch(engines())
+    batch.capture()
+    for _ in range(warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / max(steps, 1)
+    got = batch.results()
+    useful = sum(sum(st["useful"] for st in r.stats) for r in got)
+
+    plain = engines()
+    for e in plain:
+        e.run()
+    torch.cuda.synchronize()
+    passes = max(1, min(steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        for e in plain:
+            e.run()
+    torch.cuda.synchronize()
+    plain_sec = (time.perf_counter() - t0) / passes
+    want = [e.results() for e in plain]
+    same = all((a.skeleton == b.skeleton).all() and (a.separation_sets == b.separation_sets).all()
+               and np.array_equal(a.pmax.view(np.uint32), b.pmax.view(np.uint32)) for a, b in zip(got, want))
+    log(f"[bench] discrete batched: {sec * 1e3:.2f} ms per step, plain {plain_sec * 1e3:.2f} ms, identical {same}")
+    return {"ms_per_step": sec * 1e3, "ci_tests_per_s": useful / sec, "plain_ms_per_pass": plain_sec * 1e3,
+            "plain_ci_tests_per_s": useful / plain_sec, "batched_over_plain": plain_sec / sec,
+            "results_identical": bool(same), "jobs": len(datasets),
+            "workload": f"{n_boot} bootstrap resamples of a discrete p=50 n=5000 dataset (cardinalities 2..5) + "
+                        f"{n_sets} datasets p~U[20,120] n~U[1000,20000], alpha=0.01, max_level=3"}
+
+
 def batched_summary(m: dict) -> dict:
     return {"ms_per_step": m["sec"] * 1e3, "ci_tests_per_s": m["useful"] / m["sec"],
             "plain_ms_per_pass": m["plain_sec"] * 1e3, "plain_ci_tests_per_s": m["useful"] / m["plain_sec"],
@@ -769,10 +834,12 @@ def main():
         log("[bench] cpu baseline (numpy corrcoef + oracle skeleton)")
         cpu = cpu_baseline(data, n, alpha, L)
 
-    batched = serving = None
+    batched = batched_disc = serving = None
     if args.workload == "gaussian_p1000_n10000" and not args.no_extras:
         batched = batched_summary(measure_batched(256, 64, 5, 2, 2, "batched_bootstrap256_p200_n5000_plus64",
                                                   False))
+        torch.cuda.empty_cache()
+        batched_disc = measure_disc_batched(64, 16, 5, 2)
         torch.cuda.empty_cache()
         serving = serving_summary(measure_serving(args.serving_requests, args.load))
         torch.cuda.empty_cache()
@@ -803,6 +870,7 @@ def main():
         "skeleton_ms_note": "levels 0..L + pmax with C resident (correlation excluded); ms_per_step includes it",
         "dropin_execute": dropin,
         "batched": batched,
+        "batched_discrete": batched_disc,
         "serving": serving,
         "tests_per_step": {"useful": useful, "executed": executed},
         "gpu_vs_cpu": (useful / sec) / cpu["value"] if cpu else None,
