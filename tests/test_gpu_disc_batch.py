@@ -149,3 +149,41 @@ def test_disc_batch_graph_replay_matches_eager():
         for k, (g, w) in enumerate(zip(b.results(), want)):
             _same(g, w, f"replay {rep}, job {k}")
         _alone(data[:1])   # eager work of the program between replays
+
+
+def test_disc_batch_edge_shapes_and_restricted_workspace():
+    """A two-variable job, an n that is not a multiple of 16, and a job whose
+    tables exceed the LDS (8 categories at level 3) with a workspace of only a
+    few global table slices, which caps the workgroups of every launch it is
+    in: each equals its run alone on the same workspace size."""
+    from gpucsl_amd.discrete import DiscreteBatch, DiscreteEngine, disc_workspace
+
+    d_big = syn.discrete_bn(10, 3000, 3.0, seed=41, card_low=8, card_high=8).data
+    dims_big = (d_big.max(axis=0).astype(np.int64) + 1).astype(np.uint8)
+    from gpucsl_amd import _native as nat
+
+    full = disc_workspace(10, 3, int(dims_big.max()))
+    least = int(nat.load().gpucsl_disc_min_workspace_bytes(10, 3, int(dims_big.max())))
+    assert full > least
+    limit = least + 3 * (full - least) // 1024   # about three table slices
+    cases = [(syn.discrete_bn(2, 500, 1.0, seed=42).data, 2, None),
+             (syn.discrete_bn(15, 1001, 2.0, seed=43).data, 3, None),
+             (d_big, 3, limit),
+             (syn.discrete_bn(25, 2000, 2.0, seed=44).data, 2, None)]
+
+    def engine(codes, L, ws_limit):
+        dims = (codes.max(axis=0).astype(np.int64) + 1).astype(np.uint8)
+        e = DiscreteEngine(codes.shape[1], codes.shape[0], L, ALPHA, int(dims.max()), ws_limit=ws_limit)
+        e.set_codes(codes, dims)
+        return e
+
+    want = []
+    for codes, L, lim in cases:
+        e = engine(codes, L, lim)
+        e.run()
+        want.append(e.results())
+    b = DiscreteBatch([engine(codes, L, lim) for codes, L, lim in cases])
+    b.launch()
+    torch.cuda.synchronize()
+    for k, (g, w) in enumerate(zip(b.results(), want)):
+        _same(g, w, f"edge case {k}")
