@@ -53,14 +53,13 @@ __host__ __device__ inline int corr_row_splits(int64_t n) {
 // extremes over the same rows): 64 columns x 4 row groups per block (group g
 // takes rows g, g+4, ... of the split), the four group sums added in order.
 template <typename T>
-__device__ __forceinline__ void colsum_body(const Job &J) {
+__device__ __forceinline__ void colsum_body(const Job &J, int bx, int s) {
     const int p = J.p;
     const int64_t n = J.n;
     const int rs = corr_row_splits(n);
-    const int s = blockIdx.y;
-    if (s >= rs || (int)blockIdx.x * 64 >= p) return;
+    if (s >= rs || bx * 64 >= p) return;
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int a = blockIdx.x * 64 + c;
+    const int a = bx * 64 + c;
     const T *X = (const T *)J.X;
     const int64_t per = (n + rs - 1) / rs;
     const int64_t r0 = s * per;
@@ -119,9 +118,13 @@ __device__ __forceinline__ double column_mean(const CorrWs &cw, int64_t n, int p
     return m / (double)n;
 }
 
+// A batch: job range R.job[k] has (p + 63) / 64 column blocks x its row splits.
 template <typename T>
-__global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ jobs) {
-    colsum_body<T>(jobs[blockIdx.z]);
+__global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ jobs, BlockRanges R) {
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
+    const int cb = (J.p + 63) / 64;
+    colsum_body<T>(J, local % cb, local / cb);
 }
 
 // One job passed by value; block (0, 0) also stores its descriptor at dst for
@@ -129,7 +132,7 @@ __global__ __launch_bounds__(256) void colsum_kernel(const Job *__restrict__ job
 template <typename T>
 __global__ __launch_bounds__(256) void colsum_one_kernel(Job one, Job *dst) {
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *dst = one;
-    colsum_body<T>(one);
+    colsum_body<T>(one, blockIdx.x, blockIdx.y);
 }
 
 // gram[s] (p x p, only tiles with ta <= tb written) = Zs^T Zs over k-range s,
@@ -139,31 +142,31 @@ __global__ __launch_bounds__(256) void colsum_one_kernel(Job one, Job *dst) {
 // flight while the MFMAs of slab s run, and centring (x - mean, in FP64)
 // happens when a fragment is read.
 template <typename T, int CTT>
-__global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const Job *__restrict__ jobs) {
+__global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const Job *__restrict__ jobs,
+                                                                          BlockRanges R) {
     constexpr int NT = CTT == CT ? 256 : 64;  // 4 waves of 32 x 32, or one
     constexpr int RPP = NT / CTT;             // slab rows per loader pass
     __shared__ T As[2][KT][CTT + 4];
     __shared__ T Bs[2][KT][CTT + 4];
 
-    const Job &J = jobs[blockIdx.z];
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
     const int p = J.p;
     const int64_t n = J.n;
-    if (gram_int8(J.x_dtype, p) || gram_tile(p) != CTT || (int)blockIdx.x >= upper_tiles(p, CTT) ||
-        (int)blockIdx.y >= J.k_splits)
-        return;
+    const int tile_id = local % upper_tiles(p, CTT), split = local / upper_tiles(p, CTT);
+    if (gram_int8(J.x_dtype, p) || gram_tile(p) != CTT || split >= J.k_splits) return;
     const T *__restrict__ X = (const T *)J.X;
     CorrWs cw = carve_corr(J);
 
     // upper-triangle tile id -> (ta, tb), ta <= tb, row-major over the triangle
     const int tiles = (p + CTT - 1) / CTT;
-    int t = blockIdx.x, ta = 0;
+    int t = tile_id, ta = 0;
     while (t >= tiles - ta) {
         t -= tiles - ta;
         ++ta;
     }
     const int tb = ta + t;
     const int a0 = ta * CTT, b0 = tb * CTT;
-    const int split = blockIdx.y;
     int64_t k_per_split = (n + J.k_splits - 1) / J.k_splits;
     k_per_split = (k_per_split + KT - 1) / KT * KT;
     const int64_t k0 = split * k_per_split;
@@ -312,15 +315,16 @@ __global__ __launch_bounds__(CTT == CT ? 256 : 64) void gram_mfma_kernel(const J
 // splits' sums and extremes, the splits taken four at a time by the four waves
 // and their partial sums added in wave order.
 constexpr int I8_SLICE_STEPS = 2;
-__global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ jobs) {
-    const Job &J = jobs[blockIdx.z];
+__global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ jobs, BlockRanges R) {
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
     if (!gram_int8(J.x_dtype, J.p)) return;
     const int p = J.p;
     const int64_t n = J.n;
     const int64_t steps = i8_rows(n) / 64;
     const int CB = i8_cols(p) / 64;
-    const int cb = blockIdx.y;
-    const int64_t st0 = (int64_t)blockIdx.x * I8_SLICE_STEPS;
+    const int cb = local % CB;
+    const int64_t st0 = (int64_t)(local / CB) * I8_SLICE_STEPS;
     if (st0 >= steps || cb >= CB) return;
     const CorrWs cw = carve_corr(J);
     const int tid = threadIdx.x;
@@ -690,13 +694,14 @@ __global__ __launch_bounds__(256, 1) void gram_i8_kernel(const Job *__restrict__
 // mirrored so that C is exactly symmetric; C32 = (float)C64.
 constexpr int NQ = 4;          // row quarters per tile
 constexpr int QR = CT / NQ;    // rows per quarter
-__global__ __launch_bounds__(256) void normalize_tile_kernel(const Job *__restrict__ jobs) {
-    const Job &J = jobs[blockIdx.z];
+__global__ __launch_bounds__(256) void normalize_tile_kernel(const Job *__restrict__ jobs, BlockRanges R) {
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
     const int p = J.p;
-    const int quarter = blockIdx.x % NQ;
-    if ((int)(blockIdx.x / NQ) >= upper_tiles(p)) return;
+    const int quarter = local % NQ;
+    if (local / NQ >= upper_tiles(p)) return;
     const int tiles = (p + CT - 1) / CT;
-    int t = blockIdx.x / NQ, ta = 0;
+    int t = local / NQ, ta = 0;
     while (t >= tiles - ta) {
         t -= tiles - ta;
         ++ta;
@@ -842,9 +847,9 @@ int corr_k_splits(int64_t n, int p, int dtype) {
 // With `one` (nj == 1), jobs_dev is where the first launch stores *one.
 static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, const Job *one, hipStream_t s) {
     if (nj < 1) return OK;
-    int maxp = 0, maxks = 1, maxtiles = 0, maxtiles32 = 0, maxnt = 1, dtype = jobs_host[0].x_dtype;
-    int maxi8 = 0, maxi8p = 0;  // integer path: Gram blocks and columns
-    int64_t maxn = 0, maxn8 = 0;
+    int maxp = 0, dtype = jobs_host[0].x_dtype;
+    bool any_i8 = false;  // some job takes the integer path
+    int64_t maxn = 0;
     for (int b = 0; b < nj; ++b) {
         const Job &J = jobs_host[b];
         if (!J.X || !J.C32 || !J.corr_ws || J.n < 2 || J.p < 1 || J.k_splits < 1)
@@ -855,18 +860,7 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
                              (int)(64 * I8_MAX_STEPS));
         maxp = J.p > maxp ? J.p : maxp;
         maxn = J.n > maxn ? J.n : maxn;
-        maxks = J.k_splits > maxks ? J.k_splits : maxks;
-        // Gram tiles by path and tile size; normalisation tiles are always 64 x 64
-        if (gram_int8(dtype, J.p)) {
-            maxi8 = i8_pair_blocks(J.p) > maxi8 ? i8_pair_blocks(J.p) : maxi8;
-            maxi8p = J.p > maxi8p ? J.p : maxi8p;
-            maxn8 = J.n > maxn8 ? J.n : maxn8;
-        } else {
-            const int ut = upper_tiles(J.p, gram_tile(J.p));
-            int &mt = gram_tile(J.p) == CT ? maxtiles : maxtiles32;
-            mt = ut > mt ? ut : mt;
-        }
-        maxnt = upper_tiles(J.p) > maxnt ? upper_tiles(J.p) : maxnt;
+        any_i8 = any_i8 || gram_int8(dtype, J.p);
     }
     if (dtype != 0 && dtype != 1) return set_error(ERR_ARG, "corr: dtype must be 0 (f32) or 1 (f64)");
     const int rs = corr_row_splits(maxn);
@@ -876,13 +870,34 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
         GPUCSL_LAUNCH("corr_colsum", colsum_one_kernel<float>, dim3(cb, rs, 1), dim3(256), 0, s, *one, dst);
     else if (one)
         GPUCSL_LAUNCH("corr_colsum", colsum_one_kernel<double>, dim3(cb, rs, 1), dim3(256), 0, s, *one, dst);
-    else if (dtype == 0)
-        GPUCSL_LAUNCH("corr_colsum", colsum_kernel<float>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
-    else
-        GPUCSL_LAUNCH("corr_colsum", colsum_kernel<double>, dim3(cb, rs, nj), dim3(256), 0, s, jobs_dev);
-    if (maxi8 > 0) {
-        const unsigned rb = (unsigned)((i8_rows(maxn8) / 64 + I8_SLICE_STEPS - 1) / I8_SLICE_STEPS);
-        GPUCSL_LAUNCH("corr_slice_i8", slice_i8_kernel, dim3(rb, (maxi8p + 63) / 64, nj), dim3(256), 0, s, jobs_dev);
+    else {
+        const int rc = for_block_ranges(
+            nj,
+            [&](int b) { return (long long)((jobs_host[b].p + 63) / 64) * corr_row_splits(jobs_host[b].n); },
+            [&](const BlockRanges &R) -> int {
+                if (dtype == 0)
+                    GPUCSL_LAUNCH("corr_colsum", colsum_kernel<float>, dim3(R.first[R.count]), dim3(256), 0, s,
+                                  jobs_dev, R);
+                else
+                    GPUCSL_LAUNCH("corr_colsum", colsum_kernel<double>, dim3(R.first[R.count]), dim3(256), 0, s,
+                                  jobs_dev, R);
+                return check_hip(hipGetLastError(), "corr_colsum launch");
+            });
+        if (rc) return rc;
+    }
+    if (any_i8) {
+        int rc = for_block_ranges(
+            nj,
+            [&](int b) -> long long {
+                const Job &J = jobs_host[b];
+                if (!gram_int8(dtype, J.p)) return 0;
+                return (i8_rows(J.n) / 64 + I8_SLICE_STEPS - 1) / I8_SLICE_STEPS * (long long)(i8_cols(J.p) / 64);
+            },
+            [&](const BlockRanges &R) -> int {
+                GPUCSL_LAUNCH("corr_slice_i8", slice_i8_kernel, dim3(R.first[R.count]), dim3(256), 0, s, jobs_dev, R);
+                return check_hip(hipGetLastError(), "corr_slice_i8 launch");
+            });
+        if (rc) return rc;
         // the integer-path jobs, I8_LAUNCH_JOBS per launch, each launch's jobs
         // longest blocks (row steps per split) first
         auto block_steps = [&](int b) {
@@ -905,20 +920,40 @@ static int corr_jobs_launch(const Job *jobs_host, const Job *jobs_dev, int nj, c
             }
         }
     }
-    // 64 x 64 FP64 tiles: float64 data above GRAM_SMALL_P (float32 data there
-    // takes the integer path)
-    if (maxtiles > 0)
-        GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, CT>), dim3(maxtiles, maxks, nj), dim3(256), 0, s,
-                      jobs_dev);
-    if (maxtiles32 > 0) {
-        if (dtype == 0)
-            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, 32>), dim3(maxtiles32, maxks, nj), dim3(64), 0,
-                          s, jobs_dev);
-        else
-            GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, 32>), dim3(maxtiles32, maxks, nj), dim3(64),
-                          0, s, jobs_dev);
+    // FP64 tiles: float64 data, and float32 data up to GRAM_SMALL_P columns
+    // (64 x 64 tiles above GRAM_SMALL_P, 32 x 32 tiles of one wave up to it)
+    for (int tile : {CT, 32}) {
+        const int rc = for_block_ranges(
+            nj,
+            [&](int b) -> long long {
+                const Job &J = jobs_host[b];
+                if (gram_int8(dtype, J.p) || gram_tile(J.p) != tile) return 0;
+                return (long long)upper_tiles(J.p, tile) * J.k_splits;
+            },
+            [&](const BlockRanges &R) -> int {
+                const dim3 grid(R.first[R.count]);
+                if (tile == CT)
+                    GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, CT>), grid, dim3(256), 0, s, jobs_dev,
+                                  R);
+                else if (dtype == 0)
+                    GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<float, 32>), grid, dim3(64), 0, s, jobs_dev, R);
+                else
+                    GPUCSL_LAUNCH("corr_gram_mfma_f64", (gram_mfma_kernel<double, 32>), grid, dim3(64), 0, s, jobs_dev,
+                                  R);
+                return check_hip(hipGetLastError(), "corr_gram_mfma_f64 launch");
+            });
+        if (rc) return rc;
     }
-    GPUCSL_LAUNCH("corr_normalize", normalize_tile_kernel, dim3(maxnt * NQ, 1, nj), dim3(256), 0, s, jobs_dev);
+    {
+        const int rc = for_block_ranges(
+            nj, [&](int b) { return (long long)upper_tiles(jobs_host[b].p) * NQ; },
+            [&](const BlockRanges &R) -> int {
+                GPUCSL_LAUNCH("corr_normalize", normalize_tile_kernel, dim3(R.first[R.count]), dim3(256), 0, s,
+                              jobs_dev, R);
+                return check_hip(hipGetLastError(), "corr_normalize launch");
+            });
+        if (rc) return rc;
+    }
     return check_hip(hipGetLastError(), "corr launch");
 }
 

@@ -165,11 +165,19 @@ __device__ __forceinline__ int job_of(const int32_t *__restrict__ base, int nj, 
 // ----------------------------------------------------------------------------
 // init: G = 1, sepsets = -1, zmin = +inf (gaussian_device_manager.py:106-133)
 // ----------------------------------------------------------------------------
-__global__ void job_init_kernel(const Job *__restrict__ jobs) {
-    const Job &J = jobs[blockIdx.y];
+// Job R.job[k] takes init_blocks(p, max_level) workgroups.
+__host__ __device__ inline int init_blocks(int p, int max_level) {
+    const size_t work = (size_t)p * p * (max_level > 1 ? max_level : 1);
+    const size_t b = (work + 256 * 16 - 1) / (256 * 16);
+    return b < 1 ? 1 : (b > 1024 ? 1024 : (int)b);
+}
+
+__global__ void job_init_kernel(const Job *__restrict__ jobs, BlockRanges R) {
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
     const size_t pp = (size_t)J.p * J.p;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)init_blocks(J.p, J.max_level) * blockDim.x;
+    const size_t t0 = (size_t)local * blockDim.x + threadIdx.x;
     for (size_t t = t0; t < pp; t += stride) {
         J.G[t] = 1;
         J.zmin[t] = __builtin_inff();
@@ -181,13 +189,16 @@ __global__ void job_init_kernel(const Job *__restrict__ jobs) {
 // ----------------------------------------------------------------------------
 // level 0: gpucsl/pc/cuda/gaussian_ci.cu:18-51
 // ----------------------------------------------------------------------------
+// Job R.job[k]: one workgroup per row and 256 columns, p * ceil(p / 256).
 __global__ __launch_bounds__(256) void gauss_level0_kernel(const Job *__restrict__ jobs, float margin_k,
-                                                           unsigned long long *__restrict__ st) {
-    const Job &J = jobs[blockIdx.z];
+                                                           unsigned long long *__restrict__ st, BlockRanges R) {
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
     const int p = J.p;
-    const int a = blockIdx.y;
+    const int cbk = (p + 255) / 256;
+    const int a = local / cbk;
     if (a >= p) return;
-    const int b = blockIdx.x * 256 + threadIdx.x;
+    const int b = (local % cbk) * 256 + threadIdx.x;
     const double thr = J.thr[0];
     const float thr32 = (float)thr;
     const float zscale = level_zscale(J, 0);
@@ -225,7 +236,7 @@ __global__ __launch_bounds__(256) void gauss_level0_kernel(const Job *__restrict
     }
     // executed = total = useful = p(p-1)/2 per job; only the rare recheck
     // counters need per-wave atomics.
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    if (local == 0 && threadIdx.x == 0) {
         const unsigned long long pairs = (unsigned long long)p * (p - 1) / 2;
         atomicAdd(&st[ST_EXEC], pairs);
         atomicAdd(&st[ST_TOTAL], pairs);
@@ -418,9 +429,9 @@ __device__ void scan_job(const Job &J, int level, const Team &team, unsigned lon
 // the scan launch: item counts in itemoffA/itemoffB and pairs per item in
 // rowoffA, until the scans replace them with offsets.
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ void compact_rows(const Job &J, int level, const Team &team, bool pre) {
+__device__ __forceinline__ void compact_rows(const Job &J, int level, const Team &team, bool pre, int rblk) {
     const int p = J.p;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int row = rblk * 4 + (threadIdx.x >> 6);
     JobWs w = carve_job(J.ws, p);
     const bool active = level <= J.max_level;
     if (row < p) {
@@ -471,17 +482,21 @@ __device__ __forceinline__ void compact_rows(const Job &J, int level, const Team
     }
 }
 
+// Job R.job[k]: (p + 3) / 4 workgroups of four rows.
 __global__ __launch_bounds__(256) void compact_kernel(const Job *__restrict__ jobs, int level, Team team,
-                                                      unsigned long long *__restrict__ st, PlanArgs plan) {
+                                                      unsigned long long *__restrict__ st, PlanArgs plan,
+                                                      BlockRanges R) {
     const bool pre = plan.done != nullptr;
-    if (pre && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *plan.done = 0;
-    compact_rows(jobs[blockIdx.y], level, team, pre);
+    if (pre && blockIdx.x == 0 && threadIdx.x == 0) *plan.done = 0;
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
+    compact_rows(J, level, team, pre, local);
 }
 
 // One dataset passed by value: a compaction that no scan follows needs no
 // descriptor in device memory.
 __global__ __launch_bounds__(256) void compact_one_kernel(Job one, int level, Team team) {
-    compact_rows(one, level, team, false);
+    compact_rows(one, level, team, false, blockIdx.x);
 }
 
 // Per job (one block each) scan_job; in a batch with a plan, the block that
@@ -1783,11 +1798,14 @@ __global__ void pmax_kernel(const float *__restrict__ zmin, int p, double sqrt_n
 }
 
 // pmax and (optionally) the uint16 skeleton of every job of a launch
-__global__ void job_finish_kernel(const Job *__restrict__ jobs) {
-    const Job &J = jobs[blockIdx.z];
+// Job R.job[k]: one workgroup per row and 256 columns, p * ceil(p / 256).
+__global__ void job_finish_kernel(const Job *__restrict__ jobs, BlockRanges R) {
+    int local;
+    const Job &J = jobs[range_job(R, blockIdx.x, local)];
     const int p = J.p;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
+    const int cbk = (p + 255) / 256;
+    const int b = (local % cbk) * 256 + threadIdx.x;
+    const int a = local / cbk;
     if (b >= p || a >= p) return;
     const size_t t = (size_t)a * p + b;
     // pcalg mode: zmin already carries sqrt(n - |S| - 3)
@@ -1831,18 +1849,17 @@ static int max_p(const Job *jh, int nj) {
 }
 
 int gauss_batch_init(const Job *jh, const Job *jd, int nj, hipStream_t s) {
-    size_t maxpp = 1;
     for (int b = 0; b < nj; ++b) {
         const Job &J = jh[b];
         if (J.p < 1 || J.max_level < 0 || !J.G || !J.zmin || (J.max_level > 0 && !J.sepsets) || !J.ws)
             return set_error(ERR_ARG, "gauss_init: job %d has bad arguments", b);
-        const size_t pp = (size_t)J.p * J.p * (J.max_level > 1 ? J.max_level : 1);
-        maxpp = pp > maxpp ? pp : maxpp;
     }
-    size_t blocks = (maxpp + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    GPUCSL_LAUNCH("gauss_init", job_init_kernel, dim3((unsigned)blocks, nj), dim3(256), 0, s, jd);
-    return check_hip(hipGetLastError(), "gauss_init launch");
+    return for_block_ranges(
+        nj, [&](int b) { return (long long)init_blocks(jh[b].p, jh[b].max_level); },
+        [&](const BlockRanges &R) -> int {
+            GPUCSL_LAUNCH("gauss_init", job_init_kernel, dim3(R.first[R.count]), dim3(256), 0, s, jd, R);
+            return check_hip(hipGetLastError(), "gauss_init launch");
+        });
 }
 
 int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, unsigned long long *st,
@@ -1850,10 +1867,13 @@ int gauss_batch_level0(const Job *jh, const Job *jd, int nj, float margin_k, uns
     for (int b = 0; b < nj; ++b)
         if (!jh[b].C32 || !jh[b].C64 || !jh[b].G || !jh[b].zmin)
             return set_error(ERR_ARG, "gauss_level0: job %d has a null matrix", b);
-    const int mp = max_p(jh, nj);
-    GPUCSL_LAUNCH("gauss_level0", gauss_level0_kernel, dim3((mp + 255) / 256, mp, nj), dim3(256), 0, s, jd,
-                  margin_k, st);
-    return check_hip(hipGetLastError(), "gauss_level0 launch");
+    return for_block_ranges(
+        nj, [&](int b) { return (long long)jh[b].p * ((jh[b].p + 255) / 256); },
+        [&](const BlockRanges &R) -> int {
+            GPUCSL_LAUNCH("gauss_level0", gauss_level0_kernel, dim3(R.first[R.count]), dim3(256), 0, s, jd, margin_k,
+                          st, R);
+            return check_hip(hipGetLastError(), "gauss_level0 launch");
+        });
 }
 
 static int compact_and_plan(const Job *jh, const Job *jd, int nj, int level, int gs_min, unsigned long long *st,
@@ -1862,7 +1882,14 @@ static int compact_and_plan(const Job *jh, const Job *jd, int nj, int level, int
     const Team team = team_for(tuning_bucket(nj, mp),
                                level < 1 ? 1 : (level > GPUCSL_MAX_LEVEL_TEMPLATE ? GPUCSL_MAX_LEVEL_TEMPLATE : level),
                                gs_min);
-    GPUCSL_LAUNCH("compact", compact_kernel, dim3((mp + 3) / 4, nj), dim3(256), 0, s, jd, level, team, st, plan);
+    const int rc = for_block_ranges(
+        nj, [&](int b) { return (long long)(jh[b].p + 3) / 4; },
+        [&](const BlockRanges &R) -> int {
+            GPUCSL_LAUNCH("compact", compact_kernel, dim3(R.first[R.count]), dim3(256), 0, s, jd, level, team, st, plan,
+                          R);
+            return check_hip(hipGetLastError(), "compact launch");
+        });
+    if (rc) return rc;
     // the scan only when a plan or the level statistics are wanted (the
     // neighbour lists and degrees alone come from the compaction)
     if (plan.done || st)
@@ -1953,9 +1980,12 @@ int gauss_batch_level(int level, const Job *jh, const Job *jd, int nj, float mar
 int gauss_batch_finish(const Job *jh, const Job *jd, int nj, hipStream_t s) {
     for (int b = 0; b < nj; ++b)
         if (jh[b].n < 4) return set_error(ERR_ARG, "finish: job %d has fewer than 4 observations", b);
-    const int mp = max_p(jh, nj);
-    GPUCSL_LAUNCH("finish", job_finish_kernel, dim3((mp + 255) / 256, mp, nj), dim3(256), 0, s, jd);
-    return check_hip(hipGetLastError(), "finish launch");
+    return for_block_ranges(
+        nj, [&](int b) { return (long long)jh[b].p * ((jh[b].p + 255) / 256); },
+        [&](const BlockRanges &R) -> int {
+            GPUCSL_LAUNCH("finish", job_finish_kernel, dim3(R.first[R.count]), dim3(256), 0, s, jd, R);
+            return check_hip(hipGetLastError(), "finish launch");
+        });
 }
 
 // ---- single dataset: [job descriptor][launch workspace][job workspace] -------

@@ -11,6 +11,56 @@ constexpr int CORR_ROW_SPLITS_MAX = 64;
 constexpr int JOB_DESC_BYTES = 512;  // room for one job descriptor at the head of a workspace
 static_assert(sizeof(Job) <= JOB_DESC_BYTES, "job descriptor too large");
 
+// Launches over a table of jobs that give each job only the workgroups it
+// needs: job base + rel[k] owns workgroups first[k] .. first[k + 1] - 1 of a
+// 1-D grid.  Passed by value (a captured launch keeps it; about 3 KB of
+// kernel arguments); a batch with more jobs than RANGE_JOBS takes several
+// launches.
+constexpr int RANGE_JOBS = 512;
+struct BlockRanges {
+    int count, base;
+    uint16_t rel[RANGE_JOBS];
+    int first[RANGE_JOBS + 1];
+};
+
+// The job of workgroup id and the workgroup's index within that job.
+__device__ __forceinline__ int range_job(const BlockRanges &R, int id, int &local) {
+    int lo = 0, hi = R.count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (R.first[mid] <= id)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    local = id - R.first[lo];
+    return R.base + R.rel[lo];
+}
+
+// Calls launch(ranges) for the jobs in chunks of at most RANGE_JOBS, job b
+// taking blocks_of(b) workgroups (jobs with none are left out); returns the
+// first non-zero status.
+template <class BlocksOf, class Launch>
+inline int for_block_ranges(int nj, BlocksOf blocks_of, Launch launch) {
+    BlockRanges R{};
+    for (int b = 0; b < nj; ++b) {
+        const long long nb = blocks_of(b);
+        if (nb > 0) {
+            if (R.count == 0) R.base = b;
+            R.rel[R.count] = (uint16_t)(b - R.base);
+            R.first[R.count + 1] = R.first[R.count] + (int)nb;
+            ++R.count;
+        }
+        const bool full = R.count == RANGE_JOBS || (R.count > 0 && b + 1 - R.base > 65535);
+        if (R.count > 0 && (full || b == nj - 1)) {
+            const int rc = launch(R);
+            if (rc) return rc;
+            R = BlockRanges{};
+        }
+    }
+    return 0;
+}
+
 // FP64-recheck queue entry
 struct SlowItem {
     uint64_t start;     // first subset rank still to evaluate
