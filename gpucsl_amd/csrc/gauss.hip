@@ -970,7 +970,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
                 if (tid == 0) {
                     const float zb = z_of_rho(fminf(fminf(s_r[0], s_r[1]), fminf(s_r[2], s_r[3])));
                     if (any_near) {
-                        enqueue_slow(c, j, SlowItem{iter * 256, ldg(c.rowoff + c.i) + q, c.job, zb, 0}, queue, qcount);
+                        enqueue_slow(c, j, SlowItem{iter * 256, jpos, c.job, zb, c.i}, queue, qcount);
                     } else {
                         update_zmin(*c.J, c.p, c.i, j, zb * c.zscale);
                     }
@@ -1025,7 +1025,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
             }
             if (__ballot(near) & gmask) {
                 rl = team_min(rl, team);
-                if (gl == 0) enqueue_slow(c, j, SlowItem{iter * team, ldg(c.rowoff + c.i) + q, c.job, z_of_rho(rl), 0}, queue, qcount);
+                if (gl == 0) enqueue_slow(c, j, SlowItem{iter * team, jpos, c.job, z_of_rho(rl), c.i}, queue, qcount);
                 break;
             }
             if (act) ++n_exec;
@@ -1383,16 +1383,9 @@ __global__ __launch_bounds__(256) void gauss_slow_kernel(
                 J = &jobs[qi.job];
                 p = J->p;
                 JobWs jw = carve_job(J->ws, p);
-                const int32_t *rowoff = side == 0 ? jw.rowoffA : jw.rowoffB;
-                int lo = 0, hi = p;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (rowoff[mid] <= qi.w) lo = mid; else hi = mid;
-                }
-                i = lo;
-                const int k = qi.w - rowoff[i];
+                i = qi.row;
+                jpos = qi.jpos;
                 const int d = jw.deg[i];
-                jpos = side == 0 ? d - (rowoff[i + 1] - rowoff[i]) + k : k;
                 gnb = jw.nbr + (size_t)i * p;
                 GPUCSL_DCHECK(i >= 0 && i < p && jpos >= 0 && jpos < d);
                 j = gnb[jpos];

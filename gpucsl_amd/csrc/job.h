@@ -64,10 +64,10 @@ inline int for_block_ranges(int nj, BlocksOf blocks_of, Launch launch) {
 // FP64-recheck queue entry
 struct SlowItem {
     uint64_t start;     // first subset rank still to evaluate
-    int w;              // pair index in the phase numbering of the job
+    int jpos;           // position of the partner j in the row's neighbour list
     int job;            // index into the launch's job table
     float zl;           // zmin over the tests already counted
-    int pad;
+    int row;            // the row i whose side the pair was queued from
 };
 
 // level work item: `pairs` consecutive pairs of one row, from pair q0
