@@ -58,6 +58,7 @@ SIGNATURES = {
     "gpucsl_disc_encode": (_I, [_P, _I, _I64, _I, _I64, _I, _P, _I64, _P, _P, _P]),
     "gpucsl_unrank": (_I, [_I, _I, _P, _I, _P, _P]),
     "gpucsl_orient_edges": (_I, [_P, _I, _P, _I, _P, _P]),
+    "gpucsl_orient_edges_arcs": (_I, [_P, _I, _P, _I, _P, _P, _I64, _P, _P]),
     "gpucsl_disc_workspace_bytes": (_SZ, [_I, _I, _I]),
     "gpucsl_disc_min_workspace_bytes": (_SZ, [_I, _I, _I]),
     "gpucsl_disc_init": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _SZ, _P]),

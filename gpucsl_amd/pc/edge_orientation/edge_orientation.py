@@ -39,12 +39,32 @@ def orient_dense(skeleton: np.ndarray, separation_sets: np.ndarray) -> np.ndarra
     return dag
 
 
-def _digraph_of(dag: np.ndarray) -> nx.DiGraph:
-    p = dag.shape[0]
+def orient_arcs(skeleton: np.ndarray, separation_sets: np.ndarray):
+    """orient_dense, and the CPDAG's arcs as an (m, 2) int32 array in
+    row-major order (the order of np.nonzero(dag)), listed by the library
+    from the skeleton's neighbour lists instead of a scan of the p x p bytes."""
+    from gpucsl_amd import _native as nat
+
+    skel = np.ascontiguousarray(np.asarray(skeleton) != 0).view(np.uint8)  # bool bytes are 0/1
+    p = skel.shape[0]
+    sep = np.ascontiguousarray(np.asarray(separation_sets, dtype=np.int32).reshape(p, p, -1))
+    L = sep.shape[2]
+    dag = np.empty((p, p), np.uint8)
+    cap = int(np.count_nonzero(skel))  # every CPDAG arc is a skeleton arc
+    arcs = np.empty((max(cap, 1), 2), np.int32)
+    m = ct.c_int64(0)
+    passes = ct.c_int(0)
+    rc = nat.load().gpucsl_orient_edges_arcs(skel.ctypes.data, p, sep.ctypes.data if L else None, L,
+                                             dag.ctypes.data, arcs.ctypes.data, cap, ct.byref(m), ct.byref(passes))
+    if rc:
+        raise ValueError(f"orient_arcs: bad arguments (code {rc})")
+    return dag, arcs[: m.value]
+
+
+def _digraph_of_arcs(p: int, arcs: np.ndarray) -> nx.DiGraph:
     out = nx.DiGraph()
     out.add_nodes_from(range(p))
-    ii, jj = np.nonzero(dag)
-    out.add_edges_from(zip(ii.tolist(), jj.tolist()))
+    out.add_edges_from(arcs.tolist())
     return out
 
 
@@ -53,7 +73,8 @@ def orient_skeleton(skeleton: np.ndarray, separation_sets: np.ndarray) -> nx.DiG
     """orient_edges for a skeleton given as its (p, p) adjacency array (what
     the skeleton routines return): the same CPDAG without building a networkx
     graph of the skeleton first."""
-    return _digraph_of(orient_dense(skeleton, separation_sets))
+    _, arcs = orient_arcs(skeleton, separation_sets)
+    return _digraph_of_arcs(np.asarray(skeleton).shape[0], arcs)
 
 
 @timed
@@ -64,7 +85,7 @@ def orient_edges(skeleton: nx.Graph, separation_sets: np.ndarray) -> nx.DiGraph:
         p = skeleton.number_of_nodes()
         adj = nx.to_numpy_array(skeleton, nodelist=range(p), weight=None, dtype=np.uint8)
         adj = np.maximum(adj, adj.T)
-        return _digraph_of(orient_dense(adj, separation_sets))
+        return _digraph_of_arcs(p, orient_arcs(adj, separation_sets)[1])
     dag = skeleton.to_directed()
     orient_v_structure(dag, separation_sets, skeleton)
     apply_rules(dag)

@@ -256,6 +256,16 @@ int gpucsl_pmax_from_zmin(const float *zmin, int p, int64_t n, float *pmax, void
  * ------------------------------------------------------------------------- */
 int gpucsl_orient_edges(const uint8_t *skeleton, int p, const int32_t *sepsets, int max_level, uint8_t *dag,
                         int *passes);
+/* The same orientation, and the CPDAG's arcs as (i, j) int32 pairs in
+ * row-major order (i ascending, then j): at most arcs_cap pairs are written,
+ * *n_arcs receives the number of arcs (at most the skeleton's nonzero count).
+ * Replaces the DiGraph the reference builds edge by edge
+ * (gpucsl/pc/edge_orientation/edge_orientation.py:25-36); the Python layer
+ * builds its networkx result from the pairs without scanning the p x p
+ * bytes.  Returns 0, 1 on bad arguments, or 2 when arcs_cap is too small
+ * (dag and *n_arcs are then complete). */
+int gpucsl_orient_edges_arcs(const uint8_t *skeleton, int p, const int32_t *sepsets, int max_level, uint8_t *dag,
+                             int32_t *arcs, int64_t arcs_cap, int64_t *n_arcs, int *passes);
 
 /* ---------------------------------------------------------------------------
  * discrete skeleton (Pearson chi^2)

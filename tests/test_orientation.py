@@ -133,6 +133,30 @@ def test_native_orientation_equals_the_reference_order_restatement():
         assert got == want, f"trial {trial} p={p}"
 
 
+def test_arc_list_is_the_dense_result_in_row_major_order():
+    """orient_arcs lists exactly the nonzeros of its dense CPDAG, in the order
+    of np.nonzero, and equals the networkx restatement; also for a skeleton
+    with no edges and for a CPDAG with every skeleton edge oriented."""
+    from gpucsl_amd.pc.edge_orientation.edge_orientation import orient_arcs
+
+    rng = np.random.default_rng(11)
+    for trial in range(30):
+        p = int(rng.integers(1, 60))
+        skel = np.triu((rng.random((p, p)) < rng.uniform(0.0, 0.4)).astype(np.uint16), 1)
+        skel = skel + skel.T
+        sep = np.full((p, p, 2), -1, np.int32)
+        dag, arcs = orient_arcs(skel, sep)
+        ii, jj = np.nonzero(dag)
+        assert arcs.dtype == np.int32 and arcs.shape == (len(ii), 2), f"trial {trial}"
+        assert np.array_equal(arcs[:, 0], ii) and np.array_equal(arcs[:, 1], jj), f"trial {trial}"
+        assert {(int(a), int(b)) for a, b in arcs} == _python_orientation(skel, sep), f"trial {trial}"
+    # a collider 0 -> 2 <- 1 (0 and 1 separated by the empty set): both arcs oriented
+    skel = np.zeros((3, 3), np.uint16)
+    skel[0, 2] = skel[2, 0] = skel[1, 2] = skel[2, 1] = 1
+    dag, arcs = orient_arcs(skel, np.full((3, 3, 1), -1, np.int32))
+    assert arcs.tolist() == [[0, 2], [1, 2]]
+
+
 def test_orient_edges_uses_native_and_matches_on_pc_output():
     rng = np.random.default_rng(3)
     p = 30
