@@ -140,6 +140,14 @@ def test_small_parity(seed):
     _compare(res, ref, f"disc seed {seed}")
 
 
+def test_more_variables_than_samples():
+    """p = 80 discrete columns from n = 60 rows (sparse contingency tables,
+    many empty strata) against the oracle, every level's test count included."""
+    d = syn.discrete_bn(80, 60, 2.0, seed=61)
+    res, ref = _run(d.data, 0.05, 3)
+    _compare(res, ref, "disc p > n")
+
+
 def test_c3_parity():
     """Config C3: p=200, n=50000, cardinalities 2..5, degree 2, alpha=0.01, L=3 (seed 200)."""
     d = syn.discrete_bn(200, 50000, 2.0, seed=200)

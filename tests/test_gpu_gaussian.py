@@ -326,6 +326,19 @@ def test_integer_gram_parity(seed):
         assert res.stats[l]["useful"] == ref["executed"][l]
 
 
+def test_more_variables_than_samples():
+    """p = 300 float32 columns (integer Gram path) from n = 150 rows: C has
+    rank n - 1, and the skeleton, sepsets and useful test counts still equal
+    the FP64 oracle's on the same values."""
+    n = 150
+    X = syn.gaussian_sem(300, n, 3.0, seed=77).data.astype(np.float32)
+    _, res = _engine_run(X, n, 0.01, 3)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), n, 0.01, 3)
+    _compare(res, ref, "p > n", pmax_atol=5e-5)
+    for l in range(4):
+        assert res.stats[l]["useful"] == ref["executed"][l]
+
+
 def test_user_correlation_matrix_path():
     d = syn.gaussian_sem(120, 1500, 3.0, seed=21)
     C = oracle.corrcoef(d.data)
