@@ -449,8 +449,10 @@ __global__ __launch_bounds__(256) void slice_i8_kernel(const Job *__restrict__ j
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 constexpr int I8_MAXU = 6;        // digit pairs (a, b) with a + b <= I8_MAXU
 // LDS panel layout: 64 bytes per column and digit, the four 16-byte chunks of
-// column c in the order q ^ ((c / 4) % 4), so the 16 lanes of a fragment read
-// (16 consecutive columns, one chunk) hit 16 different groups of four banks.
+// column c in the order q ^ ((c / 2) % 4).  A fragment read (lane l: column
+// l % 16 of a 16-column group, chunk l / 16) then measures free of LDS bank
+// conflicts on gfx950 (SQ_LDS_BANK_CONFLICT of a probe kernel), where the
+// order q ^ ((c / 4) % 4) and the plain order are two-way conflicted.
 constexpr int I8_COLB = 64;
 
 // Blocks of the integer Gram launch for one dataset: the upper 64 x 64 tiles
@@ -546,20 +548,20 @@ __global__ __launch_bounds__(256, 1) void gram_i8_kernel(const Job *__restrict__
     const bool idle = half == 1 && !two;
 
     // LDS panel tile: 64 bytes per column, the four 16-byte chunks of column c
-    // in the order q ^ ((c / 4) % 4).  A wave's direct load writes 1 KB of a
+    // in the order q ^ ((c / 2) % 4).  A wave's direct load writes 1 KB of a
     // tile, lane l at byte 16 l, so thread tid fetches the chunk that belongs at
-    // byte 16 tid: column tid / 4, chunk (tid % 4) ^ ((tid / 16) % 4).  Slot 1
+    // byte 16 tid: column tid / 4, chunk (tid % 4) ^ ((tid / 8) % 4).  Slot 1
     // holds B tile tb0 and slot 2 B tile tb0 + 1, where a tile that does not
     // exist is replaced by the A tile, so that the loads take no branches.
     // Tile offsets and LDS addresses are wave-uniform (M0 comes from scalars).
     const int CB = ppad / 64;
     const size_t plane_stride = (size_t)CB * steps * I8_TILE;
-    const unsigned gsrc = 16u * (tid ^ ((tid >> 4) & 3));
+    const unsigned gsrc = 16u * (tid ^ ((tid >> 3) & 3));
     const int wave_u = __builtin_amdgcn_readfirstlane(wave);
     const size_t offA = i8_tile(0, ta, 0, CB, steps);
     const size_t offB0 = i8_tile(0, tb0, 0, CB, steps);
     const size_t offB1 = two ? offB0 + (size_t)steps * I8_TILE : offA;  // column tile tb0 + 1
-    const int rchunk = kq ^ ((ci >> 2) & 3);                           // column 16 m + ci
+    const int rchunk = kq ^ ((ci >> 1) & 3);                           // column 16 m + ci
     // direct load of panel tile t (digit t / 3, slot t % 3) of step st into buf
     auto issue = [&](I8Panels &buf, int64_t st, int t) {
         const int a = t / I8_PANEL_SLOTS, slot = t % I8_PANEL_SLOTS;
