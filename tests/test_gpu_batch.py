@@ -85,6 +85,34 @@ def test_batch_equals_alone_and_oracle():
     assert [d["useful"] for d in st.levels] == oracle_exec
 
 
+def test_batch_recheck_everything_equals_plain_and_oracle():
+    """Margin 1e30 sends every level >= 1 test of every job through the FP64
+    recheck queue (items from several jobs in one queue, pending side-A pairs
+    continued on their side B): each job equals its plain run at the same
+    margin bit for bit and the oracle's skeleton and sepsets, and every
+    executed test was rechecked."""
+    from gpucsl_amd.batch import GaussianBatch, run_plain
+
+    specs = SPECS[:6]
+    jobs, data = _jobs(specs)
+    b = GaussianBatch(jobs, margin=1e30)
+    b.launch()
+    torch.cuda.synchronize()
+    got = b.results()
+    plain, _ = _jobs(specs)
+    run_plain(plain, margin=1e30)
+    torch.cuda.synchronize()
+    for k, ((p, n, L, alpha, seed), X) in enumerate(zip(specs, data)):
+        ref = oracle.gauss_skeleton(oracle.corrcoef(X.astype(np.float64)), n, alpha, L)
+        assert _same(got[k], plain[k].results(), f"job {k} batch vs plain at margin 1e30")
+        assert (got[k].skeleton == ref["skeleton"]).all(), f"job {k} skeleton vs oracle"
+        assert (got[k].separation_sets == ref["sepsets"]).all(), f"job {k} sepsets vs oracle"
+    st = b.batch_stats()
+    for l in range(1, 4):
+        say(f"  level {l}: {st.levels[l]}")
+        assert st.levels[l]["rechecked"] == st.levels[l]["executed"]
+
+
 def test_batch_with_deep_levels_equals_alone_and_oracle():
     """Jobs that stop at different levels up to 8 (dense SEMs, degree 8, with
     tests up to level 7) in one batch: each job bit-identical to its run alone,
