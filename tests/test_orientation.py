@@ -157,6 +157,36 @@ def test_arc_list_is_the_dense_result_in_row_major_order():
     assert arcs.tolist() == [[0, 2], [1, 2]]
 
 
+def test_arc_list_capacity_contract():
+    """gpucsl_orient_edges_arcs: a buffer too small for the arcs returns 2
+    with the full count and a complete dense result; a null count pointer or
+    a negative capacity is a bad argument (1)."""
+    import ctypes as ct
+
+    from gpucsl_amd import _native as nat
+    from gpucsl_amd.pc.edge_orientation.edge_orientation import orient_arcs
+
+    rng = np.random.default_rng(5)
+    p = 25
+    skel = np.triu((rng.random((p, p)) < 0.3).astype(np.uint8), 1)
+    skel = np.ascontiguousarray(skel + skel.T)
+    sep = np.full((p, p, 1), -1, np.int32)
+    want_dag, want_arcs = orient_arcs(skel, sep)
+    assert len(want_arcs) > 2
+    lib = nat.load()
+    dag = np.zeros((p, p), np.uint8)
+    arcs = np.full((2, 2), -7, np.int32)
+    m = ct.c_int64(-1)
+    rc = lib.gpucsl_orient_edges_arcs(skel.ctypes.data, p, sep.ctypes.data, 1, dag.ctypes.data, arcs.ctypes.data, 2,
+                                      ct.byref(m), None)
+    assert rc == 2 and m.value == len(want_arcs)
+    assert np.array_equal(dag, want_dag) and np.array_equal(arcs, want_arcs[:2])
+    assert lib.gpucsl_orient_edges_arcs(skel.ctypes.data, p, sep.ctypes.data, 1, dag.ctypes.data,
+                                        arcs.ctypes.data, 2, None, None) == 1
+    assert lib.gpucsl_orient_edges_arcs(skel.ctypes.data, p, sep.ctypes.data, 1, dag.ctypes.data,
+                                        arcs.ctypes.data, -1, ct.byref(m), None) == 1
+
+
 def test_orient_edges_uses_native_and_matches_on_pc_output():
     rng = np.random.default_rng(3)
     p = 30
