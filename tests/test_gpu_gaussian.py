@@ -105,7 +105,7 @@ def test_compact_random_large():
 @pytest.mark.parametrize("p,n,dtype", [(50, 2000, np.float64), (64, 1000, np.float32),
                                        (65, 333, np.float32), (130, 2049, np.float64),
                                        (256, 3001, np.float32), (257, 3001, np.float32),
-                                       (1000, 10000, np.float32)])
+                                       (300, 2500, np.float64), (1000, 10000, np.float32)])
 def test_corr_matches_numpy(p, n, dtype):
     X = syn.gaussian_sem(p, n, 3.0, seed=p + n).data.astype(dtype)
     from gpucsl_amd.gaussian import GaussianEngine
@@ -337,6 +337,37 @@ def test_more_variables_than_samples():
     _compare(res, ref, "p > n", pmax_atol=5e-5)
     for l in range(4):
         assert res.stats[l]["useful"] == ref["executed"][l]
+
+
+def test_float64_data_above_256_columns():
+    """float64 data with p = 300 (the FP64 Gram on 64 x 64 tiles): skeleton,
+    sepsets and useful test counts equal the FP64 oracle's."""
+    n = 2500
+    X = syn.gaussian_sem(300, n, 3.0, seed=303).data
+    _, res = _engine_run(X, n, 0.01, 3)
+    ref = oracle.gauss_skeleton(oracle.corrcoef(X), n, 0.01, 3)
+    _compare(res, ref, "float64 p=300", pmax_atol=5e-5)
+    for l in range(4):
+        assert res.stats[l]["useful"] == ref["executed"][l]
+
+
+def test_few_samples_zero_thresholds_at_high_levels():
+    """n = 9 leaves nonzero thresholds |Phi^-1(a/2)| / sqrt(n - l - 4) only for
+    levels 0..3 (min(50, n - 3) - 2 of them); from level 4 the threshold is 0
+    and no test removes an edge.  An equicorrelated C (rho = 0.9, p = 8) at
+    alpha = 0.9 keeps every edge, so every level 1..6 runs all of its tests;
+    at alpha = 0.5 every edge goes at level 2.  Skeleton, sepsets, pmax and
+    per-level counts against the oracle."""
+    n, p = 9, 8
+    C = np.full((p, p), 0.9)
+    np.fill_diagonal(C, 1.0)
+    for alpha, edges in ((0.9, 28), (0.5, 0)):
+        _, res = _engine_run(None, n, alpha, 6, C=C)
+        ref = oracle.gauss_skeleton(C, n, alpha, 6)
+        _compare(res, ref, f"n=9 alpha={alpha}", pmax_atol=5e-5)
+        assert int(res.skeleton.sum()) // 2 == edges
+        for l in range(7):
+            assert res.stats[l]["useful"] == ref["executed"][l]
 
 
 def test_user_correlation_matrix_path():
