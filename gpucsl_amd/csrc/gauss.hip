@@ -907,7 +907,7 @@ __device__ __forceinline__ void enqueue_slow(const ItemCtx &c, int j, const Slow
 
 template <int L, int MODE>
 __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsigned int *__restrict__ qcount,
-                            unsigned long long &n_exec, unsigned long long &n_useful,
+                            unsigned int &n_exec, unsigned int &n_useful,
                             unsigned long long (&s_pm)[4], unsigned long long (&s_nm)[4], float (&s_r)[4]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint8_t *__restrict__ Gm = c.J->G;
@@ -985,8 +985,8 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
 // teams of `team` lanes inside a wave pull pairs from s_next
 template <int L, int MODE>
 __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__restrict__ queue,
-                           unsigned int *__restrict__ qcount, unsigned long long &n_exec,
-                           unsigned long long &n_useful) {
+                           unsigned int *__restrict__ qcount, unsigned int &n_exec,
+                           unsigned int &n_useful) {
     const int lane = threadIdx.x & 63;
     const int gbase = lane & ~(team - 1);
     const int gl = lane & (team - 1);
@@ -1066,7 +1066,7 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
     __shared__ int s_next;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nitems = itembase[nj];
-    unsigned long long n_exec = 0, n_useful = 0;
+    unsigned int n_exec = 0, n_useful = 0;  // per lane; widened for the block totals
 
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
         const WorkItem it = items[item];
@@ -1191,14 +1191,15 @@ __global__ __launch_bounds__(256) void gauss_level_fast_kernel(
         }
     }
     // block totals -> one statistics stripe
+    unsigned long long t_exec = n_exec, t_useful = n_useful;
     for (int off = 32; off >= 1; off >>= 1) {
-        n_exec += __shfl_xor(n_exec, off);
-        n_useful += __shfl_xor(n_useful, off);
+        t_exec += __shfl_xor(t_exec, off);
+        t_useful += __shfl_xor(t_useful, off);
     }
     __syncthreads();
     if (lane == 0) {
-        s_pm[wave] = n_exec;
-        s_nm[wave] = n_useful;
+        s_pm[wave] = t_exec;
+        s_nm[wave] = t_useful;
     }
     __syncthreads();
     if (tid == 0) {
@@ -1229,7 +1230,7 @@ __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nitems = itembase[nj];
     const int nwaves = gridDim.x * 4;
-    unsigned long long n_exec = 0, n_useful = 0;
+    unsigned int n_exec = 0, n_useful = 0;  // per lane; widened for the block totals
     for (int item = blockIdx.x * 4 + wave; item < nitems; item += nwaves) {
         const WorkItem it = items[item];
         const Job &J = jobs[it.job];
@@ -1293,13 +1294,14 @@ __global__ __launch_bounds__(256) void gauss_level_wave_kernel(
         }
         __builtin_amdgcn_wave_barrier();  // every team is done with s_next before lane 0 resets it
     }
+    unsigned long long t_exec = n_exec, t_useful = n_useful;
     for (int off = 32; off >= 1; off >>= 1) {
-        n_exec += __shfl_xor(n_exec, off);
-        n_useful += __shfl_xor(n_useful, off);
+        t_exec += __shfl_xor(t_exec, off);
+        t_useful += __shfl_xor(t_useful, off);
     }
     if (lane == 0) {
-        s_e[wave] = n_exec;
-        s_u[wave] = n_useful;
+        s_e[wave] = t_exec;
+        s_u[wave] = t_useful;
     }
     __syncthreads();
     if (tid == 0) {
