@@ -875,8 +875,20 @@ __device__ __forceinline__ float team_min(float x, int team) {
     return __uint_as_float(v);
 }
 
+// Level 1 has C(d - 1, 1) = d - 1 < 2^31 subsets per pair, so its loops
+// count in 32-bit arithmetic; higher levels in 64 bits.
+template <int L>
+using rank_t = typename std::conditional<(L == 1), uint32_t, uint64_t>::type;
+
 template <int L>
 __device__ __forceinline__ void subset_of(const ItemCtx &c, uint64_t rank, int jpos, int (&sp)[L]) {
+    if constexpr (L == 1) {
+        // rank r < C(d - 1, 1): the r-th of the row's neighbours other than j
+        const int r = (int)rank;
+        sp[0] = r + (r >= jpos);
+        GPUCSL_DCHECK(sp[0] >= 0 && sp[0] < c.d && sp[0] != jpos);
+        return;
+    }
     int pos[L];
     if (L >= 2 && c.tab)
         unrank_lex_tab<L>(c.Nn, rank, pos, c.tab, c.Nn + 1);
@@ -921,9 +933,10 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
             __syncthreads();
         }
         float rl = __builtin_inff();  // min |rho| over the tests counted so far
-        for (uint64_t iter = 0;; ++iter) {
-            const uint64_t rank = iter * 256 + tid;
-            const bool act = rank < c.count;
+        const rank_t<L> cnt = (rank_t<L>)c.count;
+        for (rank_t<L> iter = 0;; ++iter) {
+            const rank_t<L> rank = iter * 256 + tid;
+            const bool act = rank < cnt;
             bool pass = false, near = false;
             float ar = __builtin_inff();
             int sp[L];
@@ -960,7 +973,7 @@ __device__ void block_pairs(const ItemCtx &c, SlowItem *__restrict__ queue, unsi
                         ++n_useful;
                         rl = fminf(rl, ar);
                     }
-                    finish = (iter + 1) * 256 >= c.count;
+                    finish = (iter + 1) * 256 >= cnt;
                 }
             }
             if (finish) {
@@ -1012,9 +1025,10 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         float rl = __builtin_inff();
-        for (uint64_t iter = 0;; ++iter) {
-            const uint64_t rank = iter * team + gl;
-            const bool act = rank < c.count;
+        const rank_t<L> cnt = (rank_t<L>)c.count;
+        for (rank_t<L> iter = 0;; ++iter) {
+            const rank_t<L> rank = iter * (rank_t<L>)team + (rank_t<L>)gl;
+            const bool act = rank < cnt;
             bool pass = false, near = false;
             float ar = __builtin_inff();
             int sp[L];
@@ -1044,7 +1058,7 @@ __device__ void team_pairs(const ItemCtx &c, int team, int *s_next, SlowItem *__
                     ++n_useful;
                     rl = fminf(rl, ar);
                 }
-                finish = (iter + 1) * team >= c.count;
+                finish = (iter + 1) * (rank_t<L>)team >= cnt;
             }
             if (finish) {
                 rl = team_min(rl, team);
