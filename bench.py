@@ -43,6 +43,7 @@ DISCRETE_WORKLOADS = {
 }
 BATCH_WORKLOADS = ("batched_bootstrap256_p200_n5000_plus64",)
 SERVING_WORKLOADS = ("serving_poisson_mix",)
+OPEN_LOOP_PASSES = 3
 
 
 def log(*a):
@@ -609,12 +610,22 @@ def measure_serving(n_requests: int, load: float) -> dict:
     rate = load * len(reqs) / makespan_p
     rng = np.random.Generator(np.random.PCG64(501))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
-    log(f"[bench] open loop: Poisson arrivals at {rate:.2f} requests/s")
-    srv2, done2, span2 = serve(reqs, arrivals)
-    lat = np.array([c.latency for c in done2])
-    lat_g = np.array([c.latency for c in done2 if c.kind == "gaussian"])
-    lat_d = np.array([c.latency for c in done2 if c.kind == "discrete"])
-    useful2 = srv2.useful_tests()
+    # one pass spans about 60 ms of wall time, so a single host stall moves its
+    # percentiles: run OPEN_LOOP_PASSES passes of the same arrivals and report
+    # the pass whose p50 is the median (every pass's p50/p99 goes in the line)
+    log(f"[bench] open loop: Poisson arrivals at {rate:.2f} requests/s, {OPEN_LOOP_PASSES} passes")
+    passes = []
+    for _ in range(OPEN_LOOP_PASSES):
+        srv2, done2, span2 = serve(reqs, arrivals)
+        lat = np.array([c.latency for c in done2])
+        passes.append(dict(p50=_pct(lat, 50), p99=_pct(lat, 99), span2=span2, lat=lat,
+                           useful2=srv2.useful_tests(),
+                           lat_g=np.array([c.latency for c in done2 if c.kind == "gaussian"]),
+                           lat_d=np.array([c.latency for c in done2 if c.kind == "discrete"])))
+        del srv2, done2
+    med = sorted(passes, key=lambda q: q["p50"])[len(passes) // 2]
+    lat, lat_g, lat_d, useful2, span2 = med["lat"], med["lat_g"], med["lat_d"], med["useful2"], med["span2"]
+    open_passes = [[q["p50"], q["p99"]] for q in passes]
 
     with nat.KernelTimer() as kt:
         for r in reqs:
@@ -627,7 +638,8 @@ def measure_serving(n_requests: int, load: float) -> dict:
                 max_batch=srv.max_batch, host_tick_s=srv.host_tick_s, host_time=srv.host_time,
                 makespan_p=makespan_p, makespan_c=makespan_c, plain_gpu_sec=plain_gpu_sec,
                 plain_host_sec=plain_host_sec, identical=identical, rate=rate, load=load, span2=span2,
-                useful2=useful2, lat=lat, lat_g=lat_g, lat_d=lat_d, timings=timings, dominant=dominant,
+                useful2=useful2, lat=lat, lat_g=lat_g, lat_d=lat_d, open_passes=open_passes,
+                timings=timings, dominant=dominant,
                 n_gauss=sum(r.kind == "gaussian" for r in reqs), n_disc=sum(r.kind == "discrete" for r in reqs))
 
 
@@ -639,6 +651,7 @@ def serving_summary(m: dict) -> dict:
     return {"requests": m["reqs"], "gaussian": m["n_gauss"], "discrete": m["n_disc"],
             "latency_p50_ms": _pct(m["lat"], 50), "latency_p99_ms": _pct(m["lat"], 99),
             "open_loop_rate_per_s": m["rate"], "open_loop_load": m["load"],
+            "open_loop_p50_p99_ms_each_pass": m["open_passes"],
             "served_over_plain_with_host_results": m["plain_host_sec"] / m["makespan_c"],
             "served_over_plain_results_on_device": m["plain_gpu_sec"] / m["makespan"],
             "served_ci_tests_per_s": m["useful"] / m["makespan"],
@@ -683,6 +696,7 @@ def bench_serving(args):
                       "throughput_requests_per_s": m["reqs"] / m["span2"],
                       "ci_tests_per_s": m["useful2"] / m["span2"],
                       "latency_p50_ms": _pct(lat, 50), "latency_p99_ms": _pct(lat, 99),
+                      "p50_p99_ms_each_pass": m["open_passes"], "reported_pass": "median p50",
                       "gaussian_p50_ms": _pct(lat_g, 50), "gaussian_p99_ms": _pct(lat_g, 99),
                       "discrete_p50_ms": _pct(lat_d, 50), "discrete_p99_ms": _pct(lat_d, 99)},
         "roofline": {"kernel": m["dominant"], "bound": None, "achieved": None, "peak": None, "unit": None,
