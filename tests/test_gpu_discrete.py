@@ -236,6 +236,29 @@ def test_dropin_api_and_encoding():
         discover_skeleton_gpu_discrete(init_pc_graph(raw), raw, 0.05, 2, 15, 3000, memory_restriction=10)
 
 
+def test_skeleton_twice_in_a_row():
+    """The whole level-1 skeleton twice on one engine (its buffers and workspace
+    left from the first run) and twice through the drop-in API, as the
+    reference's tests/test_skeleton_discrete.py:96-108 runs it: every run equals
+    the oracle, including the per-level test counts and pmax."""
+    from gpucsl_amd.pc.discover_skeleton_discrete import discover_skeleton_gpu_discrete
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    d = syn.discrete_bn(25, 4000, 2.0, seed=96)
+    dims = (d.data.max(axis=0) + 1).astype(np.uint8)
+    ref = oracle.disc_skeleton(np.ascontiguousarray(d.data.T), dims, 0.05, 1)
+    eng, _ = _engine(d.data, 1, 0.05)
+    for k in range(2):
+        eng.run()
+        _compare(eng.results(), ref, f"same engine, run {k + 1}")
+    raw = d.data.astype(np.int64)
+    for k in range(2):
+        res, _ = discover_skeleton_gpu_discrete(init_pc_graph(raw), raw, 0.05, 1, 25, 4000)
+        say(f"[drop-in run {k + 1}] edges={int(res.skeleton.sum()) // 2}")
+        assert (res.skeleton == ref["skeleton"]).all()
+        assert (res.seperation_sets == ref["sepsets"]).all()
+
+
 def test_memory_restriction_runs_fewer_tests_at_once_same_result():
     """Tables of max_dim=8 at level 3 exceed the LDS budget; a workspace with
     room for three global table slices runs three tests at a time and must
