@@ -440,6 +440,23 @@ def test_dropin_api_shapes():
     assert (res.seperation_sets == ref["sepsets"]).all()
 
 
+def test_dropin_runs_in_a_row_with_changing_shapes():
+    """The drop-in skeleton call several times in one process, with the
+    dataset's size changing between calls (p = 40, 300, 40 again): each call
+    equals the oracle, so no state of an earlier call (workspaces, launch
+    tables) leaks into the next."""
+    from gpucsl_amd.pc.discover_skeleton_gaussian import discover_skeleton_gpu_gaussian
+    from gpucsl_amd.pc.helpers import init_pc_graph
+
+    for p, n, seed in [(40, 1000, 41), (300, 3000, 42), (40, 1000, 41)]:
+        d = syn.gaussian_sem(p, n, 3.0, seed=seed)
+        res, _ = discover_skeleton_gpu_gaussian(init_pc_graph(d.data), d.data, None, 0.05, 3, p, n)
+        ref = oracle.gauss_skeleton(oracle.corrcoef(d.data), n, 0.05, 3)
+        say(f"[drop-in p={p} n={n}] edges={int(res.skeleton.sum()) // 2}")
+        assert (res.skeleton == ref["skeleton"]).all()
+        assert (res.seperation_sets == ref["sepsets"]).all()
+
+
 def test_single_dataset_graph_replay_matches_eager():
     """The whole single-dataset skeleton captured as one HIP graph: every
     replay, with other GPU work between replays, equals the eager run."""
